@@ -1,0 +1,212 @@
+/* zefr_amd.h -- C ABI of the MI355X-native ZEFR residual path.
+ *
+ * Two layers, both plain C (pointers, sizes, POD structs; no torch or C++ types):
+ *
+ *  1. Solver interface.  The entry points a SWIG/ctypes/cffi binding of ZEFR's library
+ *     interface binds: the `Zefr` object (include/zefr.hpp:24-209), the `zefr`
+ *     namespace functions (include/zefr_interface.hpp:89-159) and `InputStruct`
+ *     (include/inputstruct.hpp:113-194).  Each declaration cites the reference
+ *     interface it replaces.  `zefr_t` is the `Zefr` object; the `zefr_*` singleton
+ *     functions operate on the object `zefr_initialize` created, like the reference's
+ *     global `ZEFR` (src/zefr_interface.cpp:5).
+ *
+ *  2. Kernel interface.  The thin layer between the host classes and the HIP kernels;
+ *     it replaces `gimmik_mm_gpu` (include/gimmik.h:17-21) and the `*_wrapper`
+ *     launchers (include/elements_kernels.h:7-12, include/faces_kernels.h:26-37,
+ *     include/solver_kernels.h:18-22), which take C++ containers in the reference.
+ *
+ * Every function returning int returns 0 on success and a nonzero status otherwise;
+ * `zefr_last_error()` then holds a thread-local message.  No entry point aborts.
+ *
+ * Device data layout (element-major, "element-blocked"): per element e the solution
+ * block is U[e][var][spt], so the reference's stride-based accessors still describe it
+ * exactly: ele_stride = nVars*nSpts, var_stride = nSpts, spt_stride = 1.
+ */
+#ifndef ZEFR_AMD_H
+#define ZEFR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- input ---------- */
+
+#define ZEFR_MAX_BOUNDS 16
+#define ZEFR_NAME_LEN 64
+
+/* Hot-path subset of InputStruct (include/inputstruct.hpp:113-194); field names and
+ * meanings as in the reference.  Fields the residual path does not use are omitted. */
+typedef struct zefr_input {
+  unsigned int equation;          /* 1 = EulerNS (include/input.hpp:25-28)          */
+  unsigned int nDims, order;
+  int viscous;
+  unsigned int fconv_type;        /* 0 = Rusanov (include/input.hpp:30-33)           */
+  unsigned int fvisc_type;        /* 1 = LDG                                         */
+  double rus_k, ldg_b, ldg_tau;
+  char dt_scheme[16];             /* "Euler" or "RK44"                               */
+  unsigned int nStages;
+  unsigned int dt_type;           /* 0 = supplied dt (the only supported mode)       */
+  double dt;
+  unsigned int n_steps, report_freq, write_freq, res_type, res_field;
+  unsigned int ic_type, test_case;
+  double gamma, prandtl, mu, R, S, T_gas, rt, c_sth;
+  double rho_fs, P_fs, v_mag_fs, mach_fs, Re_fs, L_fs, T_fs, T_tot_fs, P_tot_fs;
+  double V_fs[3], norm_fs[3];
+  int fix_vis, disable_nondim;
+  char meshfile[512];
+  char output_prefix[256];
+  char spt_type[32];
+  int nMeshBounds;
+  char meshBoundName[ZEFR_MAX_BOUNDS][ZEFR_NAME_LEN]; /* mesh_bound <name> <bc> pairs */
+  char meshBoundBC[ZEFR_MAX_BOUNDS][ZEFR_NAME_LEN];
+  unsigned int iter, initIter;
+  double time;
+} zefr_input_t;
+
+typedef struct zefr_s zefr_t;
+
+const char *zefr_last_error(void);
+
+/* read_input_file (src/input.cpp:31-461), including the similarity-parameter and
+ * non-dimensionalisation step of src/input.cpp:431-539. */
+int zefr_read_input_file(const char *input_file, zefr_input_t *out);
+
+/* initialize_inputs (src/input.cpp:542-562): re-apply nStages/dt_scheme and the
+ * non-dimensionalisation after fields were changed by the caller. */
+int zefr_initialize_inputs(zefr_input_t *inp);
+
+/* ---------------------------------------------------------------- Zefr object ---- */
+
+/* Zefr::Zefr (src/zefr.cpp:228-257) */
+zefr_t *zefr_create(void);
+void zefr_destroy(zefr_t *z);
+/* Zefr::read_input (src/zefr.cpp:286-307) */
+int zefr_read_input(zefr_t *z, const char *input_file);
+/* Zefr::get_input (include/zefr.hpp:99): live pointer, edits take effect at init_inputs */
+zefr_input_t *zefr_get_input(zefr_t *z);
+/* Zefr::init_inputs (src/zefr.cpp:309-312) */
+int zefr_init_inputs(zefr_t *z);
+/* Zefr::setup_solver (src/zefr.cpp:314-363): mesh, connectivity, operators, metrics,
+ * initial condition, upload to the current HIP device. */
+int zefr_setup_solver(zefr_t *z);
+/* The host-only half of setup_solver (no GPU needed): everything up to the upload. */
+int zefr_setup_host(zefr_t *z);
+/* Zefr::do_step (src/zefr.cpp:371-383): one full time step (FRSolver::update). */
+int zefr_do_step(zefr_t *z);
+/* Zefr::do_rk_stage (src/zefr.cpp:385-396) */
+int zefr_do_rk_stage(zefr_t *z, int iter, int stage);
+/* Zefr::do_rk_stage_start / _mid / _finish (src/zefr.cpp:398-430) */
+int zefr_do_rk_stage_start(zefr_t *z, int iter, int stage);
+int zefr_do_rk_stage_mid(zefr_t *z, int iter, int stage);
+int zefr_do_rk_stage_finish(zefr_t *z, int iter, int stage);
+/* Zefr::write_residual -> FRSolver::report_residuals (src/solver.cpp:6990-7137);
+ * res[nVars] receives the per-variable norm divided by nDoF (no file is written). */
+int zefr_write_residual(zefr_t *z, double *res);
+
+/* FRSolver::compute_residual (src/solver.cpp:1494-1645) and its split
+ * (src/solver.cpp:1647-1840) on the solver's stream. */
+int zefr_compute_residual(zefr_t *z, unsigned int stage);
+int zefr_compute_residual_start(zefr_t *z, unsigned int stage);
+int zefr_compute_residual_mid(zefr_t *z, unsigned int stage);
+int zefr_compute_residual_finish(zefr_t *z, unsigned int stage);
+/* FRSolver::update with RK44 / Euler (src/solver.cpp:2599-2646, 2865-3097) */
+int zefr_update(zefr_t *z);
+/* Block the host until the solver's stream is idle. */
+int zefr_synchronize(zefr_t *z);
+
+/* Zefr::get_u_spts_d / get_du_spts_d (src/zefr.cpp:607-645): device pointer + strides.
+ * etype is the element-type slot (0 for single-type meshes).  get_du_spts_d materialises
+ * the physical gradient of the current state first.  dim_stride = nVars*nSpts. */
+double *zefr_get_u_spts_d(zefr_t *z, int *ele_stride, int *spt_stride, int *var_stride, int etype);
+double *zefr_get_du_spts_d(zefr_t *z, int *ele_stride, int *spt_stride, int *var_stride,
+                           int *dim_stride, int etype);
+/* Zefr::get_u_spts / get_du_spts: host mirror, refreshed from the device on each call
+ * (the reference returns its possibly stale host array, src/zefr.cpp:352-363). */
+double *zefr_get_u_spts(zefr_t *z, int *ele_stride, int *spt_stride, int *var_stride, int etype);
+double *zefr_get_du_spts(zefr_t *z, int *ele_stride, int *spt_stride, int *var_stride,
+                         int *dim_stride, int etype);
+/* Zefr::get_u_spt / get_grad_spt (include/zefr.hpp:142-143) */
+double zefr_get_u_spt(zefr_t *z, int ele, int spt, int var);
+double zefr_get_grad_spt(zefr_t *z, int ele, int spt, int dim, int var);
+
+/* Sizes: out[0..9] = nEles nSpts nFpts nVars nDims nGfpts nGfpts_int nStages order etype */
+int zefr_get_dims(zefr_t *z, int *out);
+/* Host <-> device copies of whole state arrays in the device layout [E][V][nS]:
+ * which = 0 U_spts, 1 U_ini, 2 divF(stage).  n = capacity in doubles. */
+int zefr_copy_state(zefr_t *z, int which, int stage, double *host, long n, int to_device);
+/* Connectivity in the reference layout [nFpts][nEles] (after orient_fpts): global
+ * flux point and slot of every element flux point (src/geometry.cpp:1727-1742). */
+int zefr_get_fpt_maps(zefr_t *z, int *fpt2gfpt, signed char *slot);
+/* Host arrays for inspection, in the reference layouts: "oppE", "oppD", "oppD_fpts",
+ * "oppDiv", "oppDiv_fpts" (dense, from the 1D tables the kernels use), "coord_spts",
+ * "jaco_det_spts", "inv_jaco_spts", "norm", "dA", "weights_spts". Returns the count. */
+long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n);
+/* Pick the elements-per-workgroup tile for this solver's kernels by timing the
+ * candidates on the device (0 = keep the built-in default). Returns the chosen value. */
+int zefr_autotune(zefr_t *z, int reps);
+int zefr_set_tile(zefr_t *z, int eles_per_block);
+
+/* ---------------------------------------------------------------- singleton ------ */
+
+/* zefr::initialize (include/zefr_interface.hpp:94, src/zefr_interface.cpp:27-33) */
+int zefr_initialize(const char *input_file);
+/* zefr::get_zefr_object / set_zefr_object / finalize (zefr_interface.hpp:97-102) */
+zefr_t *zefr_get_zefr_object(void);
+void zefr_set_zefr_object(zefr_t *z);
+int zefr_finalize(void);
+/* zefr::use_gpus (zefr_interface.hpp:104-111) */
+int zefr_use_gpus(void);
+/* zefr::get_q_spts(_d) / get_dq_spts(_d) / get_q_spt / get_grad_spt
+ * (zefr_interface.hpp:122-129) on the singleton */
+double *zefr_get_q_spts(int *ele_stride, int *spt_stride, int *var_stride, int etype);
+double *zefr_get_dq_spts(int *ele_stride, int *spt_stride, int *var_stride, int *dim_stride, int etype);
+double *zefr_get_q_spts_d(int *ele_stride, int *spt_stride, int *var_stride, int etype);
+double *zefr_get_dq_spts_d(int *ele_stride, int *spt_stride, int *var_stride, int *dim_stride, int etype);
+double zefr_get_q_spt(int ele, int spt, int var);
+
+/* ---------------------------------------------------------------- kernels -------- */
+
+/* One element-type block on the device.  All pointers are device pointers. */
+typedef struct zfr_ele_desc {
+  int nDims, order, nVars, nEles, nSpts, nFpts;
+  int viscous;
+  int eles_per_block;              /* workgroup tile (elements per workgroup), 0 = default */
+  double gamma, prandtl, mu, rt, c_sth;
+  int fix_vis;
+  double rus_k, ldg_b, ldg_tau;
+  const double *ops1d;             /* 1D tables: lE[2][N], Dss[N][N], Dsf[N][2]          */
+  const double *metric;            /* [E][D*D+1][nS]: adjugate inv_jaco(d1,d2), detJ     */
+  const double *fgeo;              /* [E][D+1][nF]: unit normal of the gfpt, signed dA   */
+  const unsigned int *nbr;         /* [E][nF]: partner (ele << 9 | slot << 8 | fpt)      */
+  double *Utr;                     /* [E][V][nF] solution trace (scratch)                */
+  double *Gtr;                     /* [E][D][V][nF] gradient trace (scratch, viscous)    */
+} zfr_ele_desc_t;
+
+/* Elements::extrapolate_U (src/elements.cpp:1337-1360): U[E][V][nS] -> Utr trace. */
+int zfr_extrapolate(const zfr_ele_desc_t *d, const double *U, void *stream);
+/* Faces::compute_common_U + Elements::compute_dU_spts/_fpts + the gradient part of
+ * Elements::compute_F + Elements::extrapolate_dU (src/faces.cpp:1366-1474,
+ * src/elements.cpp:1362-1447, 1588-1699): writes the gradient trace Gtr and, when
+ * dU_out is not NULL, the physical gradient dU_out[E][D][V][nS]. */
+int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void *stream);
+/* Elements::compute_F + compute_divF_spts + Faces::compute_common_F + compute_divF_fpts
+ * (src/elements.cpp:1449-1501, 1588-1699; src/faces.cpp:1244-1364, 1476-1676):
+ * divF[E][V][nS] from U and the traces. */
+int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream);
+/* RK stage update (src/solver.cpp:2897-2941): U = U_ini - (alpha*dt/detJ) * divF */
+int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
+                 double alpha, double dt, void *stream);
+/* Final RK combination (src/solver.cpp:2997-3062): U = U_ini - sum_s (beta_s*dt/detJ)*divF_s
+ * in stage order; divF holds nStages consecutive [E][V][nS] blocks. */
+int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
+                 const double *beta, int nStages, double dt, void *stream);
+/* Residual norm of divF/detJ (src/solver.cpp:7005-7026): res[V] = max | sum abs |
+ * sum squares (res_type 0/1/2), reduced on the device; work holds >= 1024*V doubles. */
+int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work,
+                      double *res_out_device, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZEFR_AMD_H */

@@ -1,0 +1,193 @@
+"""GPU parity: the HIP residual path (through the C ABI) against the CPU oracle on the same
+mesh, order and initial state.
+
+Bars (floating point, L-inf taken norm-wise per variable v):
+  * one residual: max|divF - divF_ref|_v <= max(1e-12 * max|divF_ref|_v, 64 eps * max|S|_v)
+    where S = |oppDiv| |F| + |oppDiv_fpts| |Fcomm| is the magnitude of the terms summed
+    into divF.  The first bound is the 1e-12 relative bar; the second only takes over where
+    divF is a cancellation of much larger terms (the Taylor-Green initial states are near
+    equilibrium: |divF| ~ 1e-4 |S|), where no two FP64 evaluations with different rounding
+    can agree to 1e-12 of |divF|.  The assertion message reports the plain relative error
+    and the cancellation factor kappa = max|S| / max|divF|.
+  * after RK steps (state U, well conditioned): relative <= 1e-10
+Integer maps are compared bit-exactly in test_host_setup.py.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from common import TWO_PI, paired, rel_err, residual_parity
+
+pytestmark = pytest.mark.gpu
+
+RES_TOL = 1e-12
+RK_TOL = 1e-10
+
+CASES = [
+    dict(name="q_euler_p3_vortex", n=(8, 8), lo=(-5.0, -5.0), hi=(5.0, 5.0), order=3, viscous=False,
+         state="vortex"),
+    dict(name="q_ns_p4_tgv", n=(6, 6), lo=(0.0, 0.0), hi=(TWO_PI, TWO_PI), order=4, viscous=True,
+         state="tgv2d"),
+    dict(name="h_ns_p3_tgv", n=(3, 3, 3), lo=(0.0, 0.0, 0.0), hi=(TWO_PI,) * 3, order=3, viscous=True,
+         state="tgv3d"),
+    dict(name="h_ns_p2_curved_central", n=(3, 3, 2), lo=(0.0, 0.0, 0.0), hi=(1.0, 2.0, 1.5), order=2,
+         viscous=True, state="smooth", perturb=0.04, extra={"ldg_b": 0.0, "ldg_tau": 1.0}),
+    dict(name="q_ns_p3_curved_sutherland", n=(5, 4), lo=(0.0, 0.0), hi=(2.0, 1.0), order=3, viscous=True,
+         state="smooth", perturb=0.05,
+         extra={"ldg_b": -0.5, "rus_k": 0.3, "fix_vis": 0, "disable_nondim": 0, "mach_fs": 0.3,
+                "Re_fs": 100.0, "rho_fs": 1.2}),
+    dict(name="q_euler_p5_ragged", n=(5, 3), lo=(0.0, 0.0), hi=(1.0, 1.0), order=5, viscous=False,
+         state="smooth"),
+    dict(name="h_euler_p1_ragged", n=(5, 3, 3), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), order=1,
+         viscous=False, state="smooth", perturb=0.03),
+    dict(name="q_ns_p0", n=(4, 3), lo=(0.0, 0.0), hi=(1.0, 1.0), order=0, viscous=True, state="smooth"),
+    dict(name="h_ns_p4_curved", n=(2, 2, 3), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), order=4, viscous=True,
+         state="smooth", perturb=0.03),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_one_residual(case):
+    z, o, _ = paired(case)
+    z.compute_residual(0)
+    o.compute_residual(0)
+    err, kappa, ok = residual_parity(z.state("divF", 0), o)
+    assert ok, f"{case['name']}: relative L-inf {err:.3e} (cancellation factor {kappa:.1e})"
+    if kappa < 100:  # well-conditioned states meet the plain relative bar
+        assert err <= RES_TOL, f"{case['name']}: relative L-inf {err:.3e}"
+
+
+@pytest.mark.parametrize("case", CASES[:5], ids=[c["name"] for c in CASES[:5]])
+def test_rk4_steps(case):
+    z, o, _ = paired(case)
+    nsteps = 10
+    for _ in range(nsteps):
+        z.do_step()
+    o.step_rk(nsteps)
+    err = rel_err(z.state("U"), o.U())
+    assert err <= RK_TOL, f"{case['name']}: relative L-inf after {nsteps} steps {err:.3e}"
+
+
+def test_vortex_100_rk4_steps():
+    """The CPU-runnable benchmark case: 2D Euler vortex, 32x32 quads on [-5,5]^2, p=3,
+    Rusanov, RK4 dt=1e-3, 100 steps."""
+    case = dict(name="c1_vortex", n=(32, 32), lo=(-5.0, -5.0), hi=(5.0, 5.0), order=3, viscous=False,
+                state="vortex")
+    z, o, _ = paired(case)
+    z.compute_residual(0)
+    o.compute_residual(0)
+    err, kappa, ok = residual_parity(z.state("divF", 0), o)
+    assert ok and err <= RES_TOL, (err, kappa)
+    z.set_state(o.U())
+    for _ in range(100):
+        z.do_step()
+    o.step_rk(100)
+    err = rel_err(z.state("U"), o.U())
+    assert err <= RK_TOL, f"relative L-inf after 100 steps {err:.3e}"
+    res_z = z.write_residual()
+    res_o = o.report_residuals()
+    assert np.allclose(res_z, res_o, rtol=1e-10, atol=0), (res_z, res_o)
+
+
+def test_forward_euler_scheme():
+    case = dict(name="q_euler_fe", n=(6, 6), lo=(-5.0, -5.0), hi=(5.0, 5.0), order=2, viscous=False,
+                state="vortex", dt_scheme="Euler")
+    z, o, _ = paired(case)
+    for _ in range(5):
+        z.do_step()
+    o.step_rk(5)
+    assert rel_err(z.state("U"), o.U()) <= RK_TOL
+
+
+def test_rk_stage_interface_matches_do_step():
+    """do_rk_stage_start/mid/finish over all stages == do_step (src/zefr.cpp:385-430)."""
+    case = dict(name="q_ns_stages", n=(4, 4), lo=(0.0, 0.0), hi=(TWO_PI, TWO_PI), order=3, viscous=True,
+                state="tgv2d")
+    z1, o, U0 = paired(case)
+    z2, _, _ = paired(dict(case, name="q_ns_stages_b"))
+    for it in range(3):
+        z1.do_step()
+        for st in range(4):
+            z2.do_rk_stage_start(it, st)
+            z2.do_rk_stage_mid(it, st)
+            z2.do_rk_stage_finish(it, st)
+    np.testing.assert_array_equal(z1.state("U"), z2.state("U"))
+    o.step_rk(3)
+    assert rel_err(z1.state("U"), o.U()) <= RK_TOL
+
+
+def test_residual_norm_types():
+    case = dict(name="h_norms", n=(3, 3, 3), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=2, viscous=True,
+                state="tgv3d")
+    z, o, _ = paired(case)
+    z.compute_residual(0)
+    o.compute_residual(0)
+    for rt in (0, 1, 2):
+        z.get_input().res_type = rt
+        o_res = _oracle_norm(o, rt)
+        assert np.allclose(z.write_residual(), o_res, rtol=1e-12, atol=0)
+
+
+def _oracle_norm(o, res_type):
+    d = o.divF(0)
+    j = o.jaco_det()[:, None, :]
+    x = d / j
+    if res_type == 0:
+        r = np.abs(x).max(axis=(0, 2))
+    elif res_type == 1:
+        r = np.abs(x).sum(axis=(0, 2))
+    else:
+        r = np.sqrt((x * x).sum(axis=(0, 2)))
+    return r / (o.nSpts * o.nEles)
+
+
+def test_freestream_preserved_on_curved_mesh():
+    case = dict(name="h_free", n=(3, 3, 3), lo=(0.0,) * 3, hi=(1.0,) * 3, order=3, viscous=True,
+                state="smooth", perturb=0.05)
+    z, o, U0 = paired(case)
+    U = np.zeros_like(U0)
+    U[:, 0] = 1.0; U[:, 1] = 0.3; U[:, 2] = -0.2; U[:, 3] = 0.1; U[:, 4] = 2.6
+    z.set_state(U)
+    z.compute_residual(0)
+    assert np.abs(z.state("divF", 0)).max() < 1e-11
+
+
+def test_determinism():
+    case = dict(name="h_det", n=(3, 3, 3), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=3, viscous=True,
+                state="tgv3d")
+    z, _, _ = paired(case)
+    z.compute_residual(0)
+    a = z.state("divF", 0)
+    z.compute_residual(0)
+    b = z.state("divF", 0)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_gradient_export_matches_oracle():
+    """get_du_spts: physical gradient at spts (the reference writes it into dU_spts in
+    compute_F, src/elements.cpp:1630-1650)."""
+    case = dict(name="q_grad", n=(4, 3), lo=(0.0, 0.0), hi=(TWO_PI, TWO_PI), order=3, viscous=True,
+                state="tgv2d", perturb=0.03)
+    z, o, _ = paired(case)
+    o.compute_residual(0)
+    dU, es, ss, vs, ds = z.get_du_spts()
+    ref = o.array("dU_spts", (o.nDims, o.nSpts, o.nVars, o.nEles))
+    got = dU.transpose(1, 3, 2, 0)  # [E][D][V][nS] -> [D][nS][V][E]
+    assert rel_err(got, ref, axis_var=2) <= RES_TOL
+    assert (es, ss, vs, ds) == (o.nDims * o.nVars * o.nSpts, 1, o.nSpts, o.nVars * o.nSpts)
+
+
+def test_tiles_agree():
+    """Every workgroup tile gives the same residual bit for bit (same per-element arithmetic)."""
+    case = dict(name="h_tiles", n=(5, 3, 3), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=3, viscous=True,
+                state="tgv3d")
+    z, _, _ = paired(case)
+    outs = []
+    for eb in (1, 2, 3, 4):
+        z.set_tile(eb)
+        z.compute_residual(0)
+        outs.append(z.state("divF", 0))
+    for a in outs[1:]:
+        np.testing.assert_array_equal(a, outs[0])

@@ -1,0 +1,153 @@
+"""ctypes binding of ``libzefr_amd.so`` (declarations in include/zefr_amd.h).
+
+The shared library is built in-tree by ``zefr_amd/csrc/Makefile`` (``build()`` below or
+``__graft_entry__.build()``).  Importing this module never builds anything and never
+falls back to another implementation: if the library is missing, ``lib()`` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libzefr_amd.so")
+CSRC = os.path.join(HERE, "csrc")
+
+MAX_BOUNDS = 16
+NAME_LEN = 64
+
+
+class ZefrInput(C.Structure):
+    """``zefr_input_t``: the residual-path subset of the reference's InputStruct."""
+
+    _fields_ = [
+        ("equation", C.c_uint), ("nDims", C.c_uint), ("order", C.c_uint), ("viscous", C.c_int),
+        ("fconv_type", C.c_uint), ("fvisc_type", C.c_uint),
+        ("rus_k", C.c_double), ("ldg_b", C.c_double), ("ldg_tau", C.c_double),
+        ("dt_scheme", C.c_char * 16), ("nStages", C.c_uint), ("dt_type", C.c_uint), ("dt", C.c_double),
+        ("n_steps", C.c_uint), ("report_freq", C.c_uint), ("write_freq", C.c_uint),
+        ("res_type", C.c_uint), ("res_field", C.c_uint), ("ic_type", C.c_uint), ("test_case", C.c_uint),
+        ("gamma", C.c_double), ("prandtl", C.c_double), ("mu", C.c_double), ("R", C.c_double),
+        ("S", C.c_double), ("T_gas", C.c_double), ("rt", C.c_double), ("c_sth", C.c_double),
+        ("rho_fs", C.c_double), ("P_fs", C.c_double), ("v_mag_fs", C.c_double), ("mach_fs", C.c_double),
+        ("Re_fs", C.c_double), ("L_fs", C.c_double), ("T_fs", C.c_double), ("T_tot_fs", C.c_double),
+        ("P_tot_fs", C.c_double), ("V_fs", C.c_double * 3), ("norm_fs", C.c_double * 3),
+        ("fix_vis", C.c_int), ("disable_nondim", C.c_int),
+        ("meshfile", C.c_char * 512), ("output_prefix", C.c_char * 256), ("spt_type", C.c_char * 32),
+        ("nMeshBounds", C.c_int),
+        ("meshBoundName", (C.c_char * NAME_LEN) * MAX_BOUNDS),
+        ("meshBoundBC", (C.c_char * NAME_LEN) * MAX_BOUNDS),
+        ("iter", C.c_uint), ("initIter", C.c_uint), ("time", C.c_double),
+    ]
+
+
+class EleDesc(C.Structure):
+    """``zfr_ele_desc_t``: one element block for the kernel-level entry points."""
+
+    _fields_ = [
+        ("nDims", C.c_int), ("order", C.c_int), ("nVars", C.c_int), ("nEles", C.c_int),
+        ("nSpts", C.c_int), ("nFpts", C.c_int), ("viscous", C.c_int), ("eles_per_block", C.c_int),
+        ("gamma", C.c_double), ("prandtl", C.c_double), ("mu", C.c_double), ("rt", C.c_double),
+        ("c_sth", C.c_double), ("fix_vis", C.c_int), ("rus_k", C.c_double), ("ldg_b", C.c_double),
+        ("ldg_tau", C.c_double), ("ops1d", C.c_void_p), ("metric", C.c_void_p), ("fgeo", C.c_void_p),
+        ("nbr", C.c_void_p), ("Utr", C.c_void_p), ("Gtr", C.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/zefr_amd.h
+_P = C.c_void_p
+_I = C.c_int
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+_CP = C.c_char_p
+SIGNATURES = {
+    "zefr_last_error": (C.c_char_p, []),
+    "zefr_read_input_file": (_I, [_CP, C.POINTER(ZefrInput)]),
+    "zefr_initialize_inputs": (_I, [C.POINTER(ZefrInput)]),
+    "zefr_create": (_P, []),
+    "zefr_destroy": (None, [_P]),
+    "zefr_read_input": (_I, [_P, _CP]),
+    "zefr_get_input": (C.POINTER(ZefrInput), [_P]),
+    "zefr_init_inputs": (_I, [_P]),
+    "zefr_setup_solver": (_I, [_P]),
+    "zefr_setup_host": (_I, [_P]),
+    "zefr_do_step": (_I, [_P]),
+    "zefr_do_rk_stage": (_I, [_P, _I, _I]),
+    "zefr_do_rk_stage_start": (_I, [_P, _I, _I]),
+    "zefr_do_rk_stage_mid": (_I, [_P, _I, _I]),
+    "zefr_do_rk_stage_finish": (_I, [_P, _I, _I]),
+    "zefr_write_residual": (_I, [_P, _DP]),
+    "zefr_compute_residual": (_I, [_P, C.c_uint]),
+    "zefr_compute_residual_start": (_I, [_P, C.c_uint]),
+    "zefr_compute_residual_mid": (_I, [_P, C.c_uint]),
+    "zefr_compute_residual_finish": (_I, [_P, C.c_uint]),
+    "zefr_update": (_I, [_P]),
+    "zefr_synchronize": (_I, [_P]),
+    "zefr_get_u_spts_d": (_P, [_P, _IP, _IP, _IP, _I]),
+    "zefr_get_du_spts_d": (_P, [_P, _IP, _IP, _IP, _IP, _I]),
+    "zefr_get_u_spts": (_DP, [_P, _IP, _IP, _IP, _I]),
+    "zefr_get_du_spts": (_DP, [_P, _IP, _IP, _IP, _IP, _I]),
+    "zefr_get_u_spt": (C.c_double, [_P, _I, _I, _I]),
+    "zefr_get_grad_spt": (C.c_double, [_P, _I, _I, _I, _I]),
+    "zefr_get_dims": (_I, [_P, _IP]),
+    "zefr_copy_state": (_I, [_P, _I, _I, _DP, C.c_long, _I]),
+    "zefr_get_fpt_maps": (_I, [_P, _IP, C.POINTER(C.c_byte)]),
+    "zefr_get_host_array": (C.c_long, [_P, _CP, _DP, C.c_long]),
+    "zefr_autotune": (_I, [_P, _I]),
+    "zefr_set_tile": (_I, [_P, _I]),
+    "zefr_initialize": (_I, [_CP]),
+    "zefr_get_zefr_object": (_P, []),
+    "zefr_set_zefr_object": (None, [_P]),
+    "zefr_finalize": (_I, []),
+    "zefr_use_gpus": (_I, []),
+    "zefr_get_q_spts": (_DP, [_IP, _IP, _IP, _I]),
+    "zefr_get_dq_spts": (_DP, [_IP, _IP, _IP, _IP, _I]),
+    "zefr_get_q_spts_d": (_P, [_IP, _IP, _IP, _I]),
+    "zefr_get_dq_spts_d": (_P, [_IP, _IP, _IP, _IP, _I]),
+    "zefr_get_q_spt": (C.c_double, [_I, _I, _I]),
+    "zfr_extrapolate": (_I, [C.POINTER(EleDesc), _P, _P]),
+    "zfr_gradient": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
+    "zfr_divergence": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
+    "zfr_rk_stage": (_I, [C.POINTER(EleDesc), _P, _P, _P, C.c_double, C.c_double, _P]),
+    "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),
+    "zfr_residual_norm": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P, _P]),
+}
+
+
+def build(verbose=False):
+    """Compile libzefr_amd.so for gfx950 (hipcc cross-compiles without a GPU)."""
+    jobs = str(min(16, os.cpu_count() or 4))
+    out = None if verbose else subprocess.DEVNULL
+    subprocess.run(["make", "-j", jobs, "-C", CSRC], check=True, stdout=out)
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libzefr_amd.so is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `make -C zefr_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().zefr_last_error().decode()
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(last_error())
+    return rc

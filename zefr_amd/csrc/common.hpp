@@ -1,0 +1,46 @@
+// common.hpp -- error handling shared by the host classes, the C ABI and the kernels.
+//
+// The reference turns errors into std::runtime_error("file:line:func: msg") through
+// ThrowException (include/macros.hpp:52-54) and checks cudaGetLastError after each launch
+// (include/macros.hpp:35-44).  Here host code throws zfr::Error; every C entry point
+// catches it, stores the message in a thread-local buffer and returns a nonzero status.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <stdexcept>
+#include <string>
+
+namespace zfr {
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+// Record msg and return a failure status (for the kernel layer, which does not throw).
+inline int fail(const std::string &msg) {
+  set_error(msg);
+  return 1;
+}
+
+inline int check_hip(hipError_t err, const char *what) {
+  if (err == hipSuccess) return 0;
+  set_error(std::string(what) + ": " + hipGetErrorString(err));
+  return 2;
+}
+
+[[noreturn]] inline void raise(const std::string &msg) { throw Error(msg); }
+
+inline void hip_or_throw(hipError_t err, const char *what) {
+  if (err != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(err));
+}
+
+// Convert a kernel-layer status into an exception carrying its message.
+inline void status_or_throw(int rc) {
+  if (rc != 0) throw Error(last_error());
+}
+
+}  // namespace zfr

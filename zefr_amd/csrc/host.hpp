@@ -1,0 +1,180 @@
+// host.hpp -- host-side classes of the MI355X residual path.
+//
+// The class names and public methods follow the reference (FRSolver, Elements, Faces,
+// PMGrid; include/solver.hpp:34-233, include/elements.hpp:237-308,
+// include/faces.hpp:147-167, include/multigrid.hpp:17-44) so callers are unchanged.
+// Their internals are a fresh design: the host builds connectivity, operators and
+// metrics once, lays them out element-major for the device, and every hot-path call is
+// a handful of HIP kernel launches through the kernel C ABI (zfr_* in zefr_amd.h).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <array>
+#include <cstddef>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "zefr_amd.h"
+
+namespace zfr {
+
+enum EleType { LINE_T = 0, QUAD_T = 2, HEX_T = 4 };
+
+// ---------------------------------------------------------------- input (input.cpp)
+void read_input_file(const std::string &file, zefr_input_t &in);
+void initialize_inputs(zefr_input_t &in);
+void apply_nondim(zefr_input_t &in);
+void finish_freestream(zefr_input_t &in);
+void set_stages(zefr_input_t &in);
+
+// ---------------------------------------------------------------- basis (basis.cpp)
+std::vector<double> gauss_legendre_points(unsigned n);
+std::vector<double> gauss_legendre_weights(unsigned n);
+double lagrange(const std::vector<double> &grid, unsigned mode, double x);
+double lagrange_d1(const std::vector<double> &grid, unsigned mode, double x);
+
+// Reference element of a tensor-product quad/hex of order p: point sets in the
+// reference's orderings (src/quads.cpp:45-190, src/hexas.cpp:51-292), the 1D DFR tables
+// the kernels contract with, and linear shape functions.
+struct RefElement {
+  int nDims = 0, order = 0, N = 0, nSpts = 0, nFpts = 0, nFptsPerFace = 0, nFaces = 0, nNodes = 0;
+  std::vector<double> loc1d, locDFR, wts1d;
+  std::vector<double> loc_spts, loc_fpts, tnorm, weights_spts;  // [n][nDims]
+  std::vector<int> idx_spts, idx_fpts;                          // [n][nDims]
+  std::vector<double> ops1d;  // lE[2][N], Dss[N][N], Dsf[N][2]
+
+  void build(int nDims, int order);
+  void shape(const double *loc, double *val, double *dval) const;  // linear, gmsh node order
+  std::vector<double> dense_operator(const std::string &name) const;
+};
+
+// ---------------------------------------------------------------- mesh (mesh.cpp)
+using FaceKey = std::array<unsigned, 4>;  // sorted node ids, UINT_MAX padded
+
+struct Mesh {
+  int nDims = 0, etype = 0, nNodes = 0, nEles = 0, nNodesPerEle = 0;
+  std::vector<double> coord;              // [nNodes][nDims]
+  std::vector<int> ele2nodes;             // [nEles][nNodesPerEle]
+  std::vector<FaceKey> per_faces;         // periodic boundary faces (sorted node ids)
+  std::vector<int> per_pair;              // index into per_faces of each face's partner
+};
+
+Mesh read_gmsh(const std::string &file, const zefr_input_t &in);
+void couple_periodic(Mesh &m);
+
+struct Connectivity {
+  int nFptsPerFace = 0, nGfpts = 0, nGfpts_int = 0, nFaces = 0;
+  std::vector<int> fpt2gfpt;         // [nFpts][nEles], after orientation
+  std::vector<signed char> slot;     // [nFpts][nEles]
+  std::vector<int> gf_ele, gf_fpt;   // [2][nGfpts]: element / element fpt of each side
+};
+
+Connectivity build_connectivity(const Mesh &m, const RefElement &re, const std::vector<double> &coord_fpts);
+
+// ---------------------------------------------------------------- device buffers
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf();
+  void alloc(size_t nbytes);
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------- solver classes
+class Faces;
+
+// One element-type block (include/elements.hpp:237-308).
+class Elements {
+ public:
+  int etype = 0, nDims = 0, order = 0, nEles = 0, nSpts = 0, nFpts = 0, nVars = 0, nNodes = 0;
+  RefElement re;
+  // host geometry in the reference layouts (src/elements.cpp:214-388, 845-996)
+  std::vector<double> nodes;          // [nNodes][nDims][nEles]
+  std::vector<double> coord_spts;     // [nSpts][nDims][nEles]
+  std::vector<double> coord_fpts;     // [nFpts][nDims][nEles]
+  std::vector<double> inv_jaco_spts;  // [nDims][nSpts][nDims][nEles] (adjugate)
+  std::vector<double> jaco_det_spts;  // [nSpts][nEles]
+  std::vector<double> inv_jaco_fpts;  // [nDims][nFpts][nDims][nEles]
+  // device data (element-major)
+  DevBuf U_spts_d, U_ini_d, divF_d, dU_spts_d, metric_d, fgeo_d, nbr_d, ops_d, Utr_d, Gtr_d;
+  std::vector<double> U_host, dU_host;  // host mirrors for get_u_spts / get_du_spts
+  int tile = 0;                         // elements per workgroup (0 = kernel default)
+
+  void setup_geometry(const Mesh &m);
+  void calc_transforms();
+  std::vector<double> initial_state(const zefr_input_t &in) const;  // [E][V][nS]
+  zfr_ele_desc_t desc(const zefr_input_t &in) const;
+  size_t state_size() const { return (size_t)nEles * nVars * nSpts; }
+};
+
+// Interface data (include/faces.hpp:147-167): normals, areas and the element-fpt
+// partner map that replaces the reference's per-gfpt pointer views.
+class Faces {
+ public:
+  int nGfpts = 0;
+  std::vector<double> norm;  // [nDims][nGfpts] unit normal, slot-0 side
+  std::vector<double> dA;    // [2][nGfpts]
+  void setup(const Elements &e, const Connectivity &c);
+  // element-major device tables
+  std::vector<double> fgeo_host(const Elements &e, const Connectivity &c) const;
+  std::vector<unsigned> nbr_host(const Elements &e, const Connectivity &c) const;
+};
+
+class FRSolver {
+ public:
+  explicit FRSolver(zefr_input_t *input, int order = -1);
+  ~FRSolver();
+  void setup_host();
+  void setup_device();
+  void setup() { setup_host(); setup_device(); }
+
+  // src/solver.cpp:1494-1840
+  void compute_residual(unsigned stage, int color = -1);
+  void compute_residual_start(unsigned stage, int color = -1);
+  void compute_residual_mid(unsigned stage, int color = -1);
+  void compute_residual_finish(unsigned stage, int color = -1);
+  // src/solver.cpp:2599-3097, 3458-3736
+  void update();
+  void step_RK();
+  void step_RK_stage(int stage);
+  void step_RK_stage_start(int stage);
+  void step_RK_stage_mid(int stage);
+  void step_RK_stage_finish(int stage);
+  // src/solver.cpp:6990-7137 (no history file; res[nVars])
+  void report_residuals(double *res);
+  void compute_gradient_export();  // fills dU_spts_d with the physical gradient of U
+  void synchronize();
+  int autotune(int reps);
+
+  zefr_input_t *input;
+  int order;
+  Mesh mesh;
+  Connectivity conn;
+  std::shared_ptr<Elements> eles;
+  std::shared_ptr<Faces> faces;
+  hipStream_t stream = nullptr;
+  double flow_time = 0.0, prev_time = 0.0, res_max = 0.0;
+  int current_iter = 0;
+  bool device_ready = false;
+  DevBuf norm_work_d, norm_out_d;
+
+ private:
+  void rk_stage_update(int stage);
+  void rk_final_update();
+};
+
+// p-multigrid (include/multigrid.hpp:17-44): the interface is kept; cycling is not
+// implemented yet and raises.
+class PMGrid {
+ public:
+  void setup(zefr_input_t *input, FRSolver &solver);
+  void cycle(FRSolver &solver);
+};
+
+}  // namespace zfr

@@ -1,0 +1,448 @@
+// solver.cpp -- Elements / Faces / FRSolver / PMGrid host classes.
+//
+// Setup follows the reference's sequence (src/solver.cpp:55-180): read the mesh, build
+// connectivity, compute geometry and metrics, set the initial state, move everything to
+// the device.  The residual and the RK stages are kernel launches on the solver's own
+// stream; nothing on the hot path touches the host.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "host.hpp"
+
+namespace zfr {
+
+// ---------------------------------------------------------------- DevBuf
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::alloc(size_t nbytes) {
+  release();
+  if (nbytes == 0) return;
+  hip_or_throw(hipMalloc(&p, nbytes), "hipMalloc");
+  bytes = nbytes;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+template <typename T>
+static void upload(DevBuf &b, const std::vector<T> &v) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) hip_or_throw(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+}
+
+// ---------------------------------------------------------------- Elements
+// Physical coordinates and mapping derivatives at spts / fpts (src/elements.cpp:325-388,
+// 845-996): x = sum_n shape_n(xi) x_n, J(r,p) = dx_p / dxi_r, adjugate and determinant.
+void Elements::setup_geometry(const Mesh &m) {
+  const int D = nDims, nN = nNodes, E = nEles;
+  nodes.assign((size_t)nN * D * E, 0.0);
+  for (int e = 0; e < E; e++)
+    for (int n = 0; n < nN; n++)
+      for (int d = 0; d < D; d++)
+        nodes[((size_t)n * D + d) * E + e] = m.coord[(size_t)m.ele2nodes[(size_t)e * nN + n] * D + d];
+  auto eval = [&](const std::vector<double> &locs, int npts, std::vector<double> &coord,
+                  std::vector<double> *jac) {
+    coord.assign((size_t)npts * D * E, 0.0);
+    if (jac) jac->assign((size_t)D * npts * D * E, 0.0);
+    std::vector<double> val(nN), dval((size_t)nN * D);
+    for (int p = 0; p < npts; p++) {
+      re.shape(&locs[(size_t)p * D], val.data(), dval.data());
+      for (int d = 0; d < D; d++)
+        for (int e = 0; e < E; e++) {
+          double x = 0.0;
+          for (int n = 0; n < nN; n++) x += val[n] * nodes[((size_t)n * D + d) * E + e];
+          coord[((size_t)p * D + d) * E + e] = x;
+        }
+      if (jac)
+        for (int r = 0; r < D; r++)
+          for (int d = 0; d < D; d++)
+            for (int e = 0; e < E; e++) {
+              double j = 0.0;
+              for (int n = 0; n < nN; n++) j += dval[(size_t)n * D + r] * nodes[((size_t)n * D + d) * E + e];
+              (*jac)[(((size_t)r * npts + p) * D + d) * E + e] = j;
+            }
+    }
+  };
+  std::vector<double> jaco_spts, jaco_fpts;
+  eval(re.loc_spts, nSpts, coord_spts, &jaco_spts);
+  eval(re.loc_fpts, nFpts, coord_fpts, &jaco_fpts);
+  auto invert = [&](const std::vector<double> &jac, int npts, std::vector<double> &inv, std::vector<double> *det) {
+    inv.assign(jac.size(), 0.0);
+    if (det) det->assign((size_t)npts * E, 0.0);
+    auto J = [&](int a, int p, int b, int e) { return jac[(((size_t)a * npts + p) * D + b) * E + e]; };
+    auto I = [&](int a, int p, int b, int e) -> double & { return inv[(((size_t)a * npts + p) * D + b) * E + e]; };
+    for (int e = 0; e < E; e++)
+      for (int p = 0; p < npts; p++) {
+        double dt;
+        if (D == 2) {
+          dt = J(0, p, 0, e) * J(1, p, 1, e) - J(0, p, 1, e) * J(1, p, 0, e);
+          I(0, p, 0, e) = J(1, p, 1, e);  I(0, p, 1, e) = -J(1, p, 0, e);
+          I(1, p, 0, e) = -J(0, p, 1, e); I(1, p, 1, e) = J(0, p, 0, e);
+        } else {
+          const double xr = J(0, p, 0, e), xs = J(1, p, 0, e), xt = J(2, p, 0, e);
+          const double yr = J(0, p, 1, e), ys = J(1, p, 1, e), yt = J(2, p, 1, e);
+          const double zr = J(0, p, 2, e), zs = J(1, p, 2, e), zt = J(2, p, 2, e);
+          dt = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
+          I(0, p, 0, e) = ys * zt - yt * zs; I(0, p, 1, e) = xt * zs - xs * zt; I(0, p, 2, e) = xs * yt - xt * ys;
+          I(1, p, 0, e) = yt * zr - yr * zt; I(1, p, 1, e) = xr * zt - xt * zr; I(1, p, 2, e) = xt * yr - xr * yt;
+          I(2, p, 0, e) = yr * zs - ys * zr; I(2, p, 1, e) = xs * zr - xr * zs; I(2, p, 2, e) = xr * ys - xs * yr;
+        }
+        if (det) {
+          if (!(dt > 0)) raise("Negative Jacobian at solution points.");
+          (*det)[(size_t)p * E + e] = dt;
+        }
+      }
+  };
+  invert(jaco_spts, nSpts, inv_jaco_spts, &jaco_det_spts);
+  invert(jaco_fpts, nFpts, inv_jaco_fpts, nullptr);
+}
+
+// Initial solution (src/elements.cpp:998-1074, src/funcs.cpp:31-155), element-major.
+std::vector<double> Elements::initial_state(const zefr_input_t &in) const {
+  const int D = nDims, V = nVars, E = nEles;
+  std::vector<double> U((size_t)E * V * nSpts, 0.0);
+  for (int e = 0; e < E; e++)
+    for (int s = 0; s < nSpts; s++) {
+      double u[5] = {0, 0, 0, 0, 0};
+      if (in.ic_type == 0) {
+        u[0] = in.rho_fs;
+        double Vsq = 0.0;
+        for (int d = 0; d < D; d++) { u[d + 1] = in.rho_fs * in.V_fs[d]; Vsq += in.V_fs[d] * in.V_fs[d]; }
+        u[D + 1] = in.P_fs / (in.gamma - 1.0) + 0.5 * in.rho_fs * Vsq;
+      } else if (in.ic_type == 1) {
+        const double x = coord_spts[((size_t)s * D + 0) * E + e];
+        const double y = coord_spts[((size_t)s * D + 1) * E + e];
+        const double z = (D == 3) ? coord_spts[((size_t)s * D + 2) * E + e] : 0.0;
+        if (in.test_case == 1) {  // isentropic vortex, src/funcs.cpp:63-75
+          const double G = 5.0, R = 1.;
+          const double f = (1.0 - x * x - y * y) / R;
+          const double rho = std::pow(1.0 - (G * G * (in.gamma - 1.)) / (8.0 * in.gamma * M_PI * M_PI) * std::exp(f),
+                                      1.0 / (in.gamma - 1.0));
+          const double Vx = 1.0 - G * y / (2.0 * M_PI) * std::exp(0.5 * f);
+          const double Vy = 1.0 + G * x / (2.0 * M_PI) * std::exp(0.5 * f);
+          const double P = std::pow(rho, in.gamma);
+          u[0] = rho; u[1] = rho * Vx; u[2] = rho * Vy;
+          u[D + 1] = P / (in.gamma - 1.0) + 0.5 * rho * (Vx * Vx + Vy * Vy);
+        } else if (in.test_case == 4) {  // Taylor-Green vortex, src/funcs.cpp:125-155
+          if (D != 3) raise("Taylor-Green Vortex test case for 3D cases only");
+          const double L = in.L_fs, U0 = in.v_mag_fs, Rho0 = in.rho_fs, P0 = in.P_fs;
+          const double uu = U0 * std::sin(x / L) * std::cos(y / L) * std::cos(z / L);
+          const double vv = -U0 * std::cos(x / L) * std::sin(y / L) * std::cos(z / L);
+          const double p = P0 + Rho0 * U0 * U0 / 16. * (std::cos(2 * x / L) + std::cos(2 * y / L)) * (std::cos(2 * z / L) + 2);
+          const double rho = p / (in.R * in.T_fs);
+          u[0] = rho; u[1] = rho * uu; u[2] = rho * vv; u[3] = 0.0;
+          u[4] = p / (in.gamma - 1.0) + 0.5 * rho * (uu * uu + vv * vv);
+        } else {
+          raise("test_case " + std::to_string(in.test_case) + " is not supported (1, 4)");
+        }
+      } else {
+        raise("ic_type not recognized!");
+      }
+      for (int v = 0; v < V; v++) U[((size_t)e * V + v) * nSpts + s] = u[v];
+    }
+  return U;
+}
+
+zfr_ele_desc_t Elements::desc(const zefr_input_t &in) const {
+  zfr_ele_desc_t d;
+  std::memset(&d, 0, sizeof(d));
+  d.nDims = nDims; d.order = order; d.nVars = nVars; d.nEles = nEles; d.nSpts = nSpts; d.nFpts = nFpts;
+  d.viscous = in.viscous;
+  d.eles_per_block = tile;
+  d.gamma = in.gamma; d.prandtl = in.prandtl; d.mu = in.mu; d.rt = in.rt; d.c_sth = in.c_sth;
+  d.fix_vis = in.fix_vis;
+  d.rus_k = in.rus_k; d.ldg_b = in.ldg_b; d.ldg_tau = in.ldg_tau;
+  d.ops1d = ops_d.as<double>();
+  d.metric = metric_d.as<double>();
+  d.fgeo = fgeo_d.as<double>();
+  d.nbr = nbr_d.as<unsigned>();
+  d.Utr = Utr_d.as<double>();
+  d.Gtr = Gtr_d.as<double>();
+  return d;
+}
+
+// ---------------------------------------------------------------- Faces
+// Unit normal and area of every global flux point (src/elements.cpp:876-941):
+// n = adj(J)^T tnorm at the element flux point, dA(slot) = |n|, unit normal from slot 0.
+void Faces::setup(const Elements &el, const Connectivity &c) {
+  const int D = el.nDims, E = el.nEles, nF = el.nFpts;
+  nGfpts = c.nGfpts;
+  norm.assign((size_t)D * nGfpts, 0.0);
+  dA.assign((size_t)2 * nGfpts, 0.0);
+  for (int e = 0; e < E; e++)
+    for (int f = 0; f < nF; f++) {
+      const int g = c.fpt2gfpt[(size_t)f * E + e], s = c.slot[(size_t)f * E + e];
+      double n[3] = {0, 0, 0};
+      for (int d1 = 0; d1 < D; d1++)
+        for (int d2 = 0; d2 < D; d2++)
+          n[d1] += el.inv_jaco_fpts[(((size_t)d2 * nF + f) * D + d1) * E + e] * el.re.tnorm[(size_t)f * D + d2];
+      double a = 0.0;
+      for (int d = 0; d < D; d++) a += n[d] * n[d];
+      a = std::sqrt(a);
+      dA[(size_t)s * nGfpts + g] = a;
+      if (s == 0)
+        for (int d = 0; d < D; d++) norm[(size_t)d * nGfpts + g] = n[d] / a;
+    }
+}
+
+// Per element flux point: the gfpt's unit normal (D) and this side's signed area.
+std::vector<double> Faces::fgeo_host(const Elements &el, const Connectivity &c) const {
+  const int D = el.nDims, E = el.nEles, nF = el.nFpts;
+  std::vector<double> out((size_t)E * (D + 1) * nF, 0.0);
+  for (int e = 0; e < E; e++)
+    for (int f = 0; f < nF; f++) {
+      const int g = c.fpt2gfpt[(size_t)f * E + e], s = c.slot[(size_t)f * E + e];
+      for (int d = 0; d < D; d++) out[((size_t)e * (D + 1) + d) * nF + f] = norm[(size_t)d * nGfpts + g];
+      const double a = dA[(size_t)s * nGfpts + g];
+      out[((size_t)e * (D + 1) + D) * nF + f] = s ? -a : a;
+    }
+  return out;
+}
+
+// Per element flux point: partner element, slot of this side, partner flux point.
+std::vector<unsigned> Faces::nbr_host(const Elements &el, const Connectivity &c) const {
+  const int E = el.nEles, nF = el.nFpts;
+  if ((long)E >= (1L << 23)) raise("element count exceeds the 23-bit partner index");
+  std::vector<unsigned> out((size_t)E * nF, 0u);
+  for (int e = 0; e < E; e++)
+    for (int f = 0; f < nF; f++) {
+      const int g = c.fpt2gfpt[(size_t)f * E + e], s = c.slot[(size_t)f * E + e];
+      const int pe = c.gf_ele[(size_t)(1 - s) * c.nGfpts + g], pf = c.gf_fpt[(size_t)(1 - s) * c.nGfpts + g];
+      out[(size_t)e * nF + f] = ((unsigned)pe << 9) | ((unsigned)s << 8) | (unsigned)pf;
+    }
+  return out;
+}
+
+// ---------------------------------------------------------------- FRSolver
+FRSolver::FRSolver(zefr_input_t *in, int ord) : input(in), order(ord < 0 ? (int)in->order : ord) {}
+
+FRSolver::~FRSolver() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void FRSolver::setup_host() {
+  if (input->equation != 1) raise("only the EulerNS equation set is supported");
+  if (input->nStages != 1 && input->nStages != 4) raise("nStages must be 1 (Euler) or 4 (RK44)");
+  mesh = read_gmsh(input->meshfile, *input);
+  eles = std::make_shared<Elements>();
+  Elements &el = *eles;
+  el.re.build((int)input->nDims, order);
+  el.etype = mesh.etype; el.nDims = mesh.nDims; el.order = order; el.nEles = mesh.nEles;
+  el.nSpts = el.re.nSpts; el.nFpts = el.re.nFpts; el.nVars = mesh.nDims + 2; el.nNodes = mesh.nNodesPerEle;
+  el.setup_geometry(mesh);
+  conn = build_connectivity(mesh, el.re, el.coord_fpts);
+  faces = std::make_shared<Faces>();
+  faces->setup(el, conn);
+  el.U_host = el.initial_state(*input);
+}
+
+void FRSolver::setup_device() {
+  Elements &el = *eles;
+  const int D = el.nDims, E = el.nEles, nS = el.nSpts, nF = el.nFpts, V = el.nVars, nM = D * D + 1;
+  if (!stream) hip_or_throw(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  upload(el.ops_d, el.re.ops1d);
+  std::vector<double> metric((size_t)E * nM * nS);
+  for (int e = 0; e < E; e++)
+    for (int s = 0; s < nS; s++) {
+      for (int a = 0; a < D; a++)
+        for (int b = 0; b < D; b++)
+          metric[((size_t)e * nM + a * D + b) * nS + s] = el.inv_jaco_spts[(((size_t)a * nS + s) * D + b) * E + e];
+      metric[((size_t)e * nM + D * D) * nS + s] = el.jaco_det_spts[(size_t)s * E + e];
+    }
+  upload(el.metric_d, metric);
+  upload(el.fgeo_d, faces->fgeo_host(el, conn));
+  upload(el.nbr_d, faces->nbr_host(el, conn));
+  const size_t st = el.state_size();
+  upload(el.U_spts_d, el.U_host);
+  el.U_ini_d.alloc(st * sizeof(double));
+  el.divF_d.alloc((size_t)input->nStages * st * sizeof(double));
+  hip_or_throw(hipMemset(el.divF_d.p, 0, el.divF_d.bytes), "hipMemset");
+  el.Utr_d.alloc((size_t)E * V * nF * sizeof(double));
+  if (input->viscous) el.Gtr_d.alloc((size_t)E * D * V * nF * sizeof(double));
+  norm_work_d.alloc(1024 * 8 * sizeof(double));
+  norm_out_d.alloc(8 * sizeof(double));
+  device_ready = true;
+}
+
+void FRSolver::synchronize() { hip_or_throw(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+static void need_device(const FRSolver &s) {
+  if (!s.device_ready) raise("solver is not set up on the device (call setup_solver)");
+}
+
+// compute_residual (src/solver.cpp:1494-1645): start = traces, mid = gradient, finish =
+// divergence of the common flux.
+void FRSolver::compute_residual_start(unsigned, int color) {
+  need_device(*this);
+  if (color > -1) raise("multi-colour (MCGS) residuals are not supported");
+  const auto d = eles->desc(*input);
+  status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
+}
+
+void FRSolver::compute_residual_mid(unsigned, int) {
+  if (!input->viscous) return;
+  const auto d = eles->desc(*input);
+  status_or_throw(zfr_gradient(&d, eles->U_spts_d.as<double>(), nullptr, stream));
+}
+
+void FRSolver::compute_residual_finish(unsigned stage, int) {
+  if (stage >= input->nStages) raise("stage out of range");
+  const auto d = eles->desc(*input);
+  double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
+  status_or_throw(zfr_divergence(&d, eles->U_spts_d.as<double>(), divF, stream));
+}
+
+void FRSolver::compute_residual(unsigned stage, int color) {
+  if (stage >= input->nStages) raise("stage out of range");
+  compute_residual_start(stage, color);
+  compute_residual_mid(stage, color);
+  compute_residual_finish(stage, color);
+}
+
+void FRSolver::compute_gradient_export() {
+  need_device(*this);
+  if (!input->viscous) raise("the gradient is only formed for viscous runs");
+  Elements &el = *eles;
+  if (!el.dU_spts_d.p) el.dU_spts_d.alloc((size_t)el.nDims * el.state_size() * sizeof(double));
+  const auto d = el.desc(*input);
+  status_or_throw(zfr_extrapolate(&d, el.U_spts_d.as<double>(), stream));
+  status_or_throw(zfr_gradient(&d, el.U_spts_d.as<double>(), el.dU_spts_d.as<double>(), stream));
+}
+
+static const double kRkAlpha[3] = {0.5, 0.5, 1.0};                         // src/solver.cpp:726-727
+static const double kRkBeta[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};     // :732-734
+
+void FRSolver::rk_stage_update(int stage) {
+  const auto d = eles->desc(*input);
+  const size_t st = eles->state_size();
+  status_or_throw(zfr_rk_stage(&d, eles->U_spts_d.as<double>(), eles->U_ini_d.as<double>(),
+                               eles->divF_d.as<double>() + (size_t)stage * st, kRkAlpha[stage], input->dt, stream));
+}
+
+void FRSolver::rk_final_update() {
+  const auto d = eles->desc(*input);
+  const int ns = (int)input->nStages;
+  const double one = 1.0;
+  const double *beta = (ns > 1) ? kRkBeta : &one;
+  const double *Uini = (ns > 1) ? eles->U_ini_d.as<double>() : eles->U_spts_d.as<double>();
+  status_or_throw(zfr_rk_final(&d, eles->U_spts_d.as<double>(), Uini, eles->divF_d.as<double>(), beta, ns,
+                               input->dt, stream));
+}
+
+// step_RK (src/solver.cpp:2865-3097) with a supplied dt.
+void FRSolver::step_RK() {
+  need_device(*this);
+  const int ns = (int)input->nStages;
+  if (ns > 1)
+    hip_or_throw(hipMemcpyAsync(eles->U_ini_d.p, eles->U_spts_d.p, eles->U_spts_d.bytes, hipMemcpyDeviceToDevice,
+                                stream), "hipMemcpyAsync");
+  for (int stage = 0; stage < ns - 1; stage++) {
+    compute_residual(stage);
+    rk_stage_update(stage);
+  }
+  compute_residual(ns - 1);
+  rk_final_update();
+}
+
+void FRSolver::update() {
+  prev_time = flow_time;
+  step_RK();
+  flow_time = prev_time + input->dt;
+  current_iter++;
+}
+
+// step_RK_stage (src/solver.cpp:3458-3690)
+void FRSolver::step_RK_stage(int stage) {
+  step_RK_stage_start(stage);
+  step_RK_stage_mid(stage);
+  step_RK_stage_finish(stage);
+}
+
+void FRSolver::step_RK_stage_start(int stage) {
+  need_device(*this);
+  const int ns = (int)input->nStages;
+  if (stage < 0 || stage >= ns) raise("stage out of range");
+  if (ns > 1 && stage == 0) {
+    prev_time = flow_time;
+    hip_or_throw(hipMemcpyAsync(eles->U_ini_d.p, eles->U_spts_d.p, eles->U_spts_d.bytes, hipMemcpyDeviceToDevice,
+                                stream), "hipMemcpyAsync");
+  }
+  compute_residual_start(stage);
+}
+
+void FRSolver::step_RK_stage_mid(int stage) { compute_residual_mid(stage); }
+
+void FRSolver::step_RK_stage_finish(int stage) {
+  const int ns = (int)input->nStages;
+  compute_residual_finish(stage);
+  if (stage < ns - 1) {
+    rk_stage_update(stage);
+  } else {
+    rk_final_update();
+    flow_time = prev_time + input->dt;
+    current_iter++;
+  }
+}
+
+// report_residuals (src/solver.cpp:6990-7137): norm of divF(0)/detJ over all spts,
+// reduced on the device, divided by nDoF = nSpts * nEles.
+void FRSolver::report_residuals(double *res) {
+  need_device(*this);
+  const auto d = eles->desc(*input);
+  const int V = eles->nVars;
+  status_or_throw(zfr_residual_norm(&d, eles->divF_d.as<double>(), (int)input->res_type,
+                                    norm_work_d.as<double>(), norm_out_d.as<double>(), stream));
+  std::vector<double> r(V);
+  hip_or_throw(hipMemcpyAsync(r.data(), norm_out_d.p, V * sizeof(double), hipMemcpyDeviceToHost, stream), "D2H");
+  synchronize();
+  const double nDoF = (double)eles->nSpts * eles->nEles;
+  for (int v = 0; v < V; v++) {
+    if (std::isnan(r[v])) raise("NaN in residual");
+    res[v] = ((input->res_type == 2) ? std::sqrt(r[v]) : r[v]) / nDoF;
+  }
+  res_max = res[std::min<int>(input->res_field, V - 1)];
+}
+
+// Time the residual with each feasible tile and keep the fastest.
+int FRSolver::autotune(int reps) {
+  need_device(*this);
+  Elements &el = *eles;
+  std::vector<int> cands;
+  for (int eb = 1; eb * el.nSpts <= 256; eb *= 2) cands.push_back(eb);
+  const int def = std::max(1, 256 / el.nSpts);
+  if (std::find(cands.begin(), cands.end(), def) == cands.end()) cands.push_back(def);
+  hipEvent_t a, b;
+  hip_or_throw(hipEventCreate(&a), "hipEventCreate");
+  hip_or_throw(hipEventCreate(&b), "hipEventCreate");
+  double best = 1e300;
+  int best_eb = el.tile;
+  for (int eb : cands) {
+    el.tile = eb;
+    compute_residual(0);
+    hip_or_throw(hipEventRecord(a, stream), "hipEventRecord");
+    for (int r = 0; r < reps; r++) compute_residual(0);
+    hip_or_throw(hipEventRecord(b, stream), "hipEventRecord");
+    hip_or_throw(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_or_throw(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    if (ms < best) { best = ms; best_eb = eb; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  el.tile = best_eb;
+  return best_eb;
+}
+
+// ---------------------------------------------------------------- PMGrid
+void PMGrid::setup(zefr_input_t *, FRSolver &) { raise("p-multigrid is not implemented in this build"); }
+void PMGrid::cycle(FRSolver &) { raise("p-multigrid is not implemented in this build"); }
+
+}  // namespace zfr
