@@ -1,0 +1,255 @@
+#!/usr/bin/env python
+"""Residual throughput of the MI355X DFR path on the 3D Navier-Stokes Taylor-Green case.
+
+Workload (one "step" = one FRSolver::compute_residual over the whole mesh, inputs
+resident in HBM): periodic 64^3 uniform hex mesh on [0, 2pi]^3, p = 3 (64 solution
+points per element, 16.8M points x 5 variables = 83.9M DoF), Re = 1600, Ma = 0.1,
+Pr = 0.71, Rusanov + LDG (beta = 0.5, tau = 0.1), FP64, Taylor-Green initial state.
+
+Prints one JSON line with the throughput in Gdof-residuals/s, the roofline of the
+dominant kernel (live HIP-event timing on the solver's stream), the whole-residual
+roofline against the per-element byte model, a stream-copy bandwidth measured on the
+same device, and a CPU baseline: the CPU oracle (oracle/, OpenMP) on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--n 64] [--order 3]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gdof-residuals/s FP64 hex p=3, % HBM roofline; VJP/residual time ratio"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--n", type=int, default=64, help="elements per direction")
+    p.add_argument("--order", type=int, default=3)
+    p.add_argument("--tile", type=int, default=-1, help="elements per workgroup (-1: autotune)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
+    p.add_argument("--cpu-steps", type=int, default=20)
+    p.add_argument("--workdir", default=None)
+    return p.parse_args()
+
+
+def write_deck(path, meshfile, order, dt=2e-4):
+    with open(path, "w") as f:
+        f.write(f"""nDims 3
+meshfile {meshfile}
+mesh_bound periodic periodic
+mesh_bound fluid fluid
+order {order}
+equation EulerNS
+viscous 1
+n_steps 200
+dt_scheme RK44
+dt {dt!r}
+dt_type 0
+write_freq 0
+report_freq 10
+res_type 2
+ic_type 1
+test_case 4
+fconv_type Rusanov
+fvisc_type LDG
+rus_k 0.0
+ldg_b 0.5
+ldg_tau 0.1
+gamma 1.4
+prandtl 0.71
+disable_nondim 1
+fix_vis 1
+mach_fs 0.1
+Re_fs 1600
+P_fs 1.0
+rho_fs 1.0
+L_fs 1.0
+output_prefix tgv
+""")
+    return path
+
+
+def kernel_bytes(D, N, viscous, beta_half):
+    """Algorithmic bytes per element of each kernel: every input read once, every output
+    written once (FP64 values, int32 partner map).  With LDG beta = +-1/2 one side's
+    gradient trace per face is produced and consumed."""
+    V = D + 2
+    nS = N ** D
+    nF = 2 * D * N ** (D - 1)
+    m = D * D + (1 if viscous else 0)
+    g = nF * V * D // (2 if beta_half else 1)
+    extrap = 8 * (nS * V + nF * V)
+    grad = 8 * (nS * V + nF * V + nS * (D * D + 1) + g) + 4 * nF
+    div = 8 * (nS * V + nF * V + nS * m + nF * (D + 1) + (g if viscous else 0) + nS * V) + 4 * nF
+    return {"extrapolate": extrap, "gradient": grad if viscous else 0, "divergence": div}
+
+
+def residual_model_bytes(D, N, viscous):
+    """Per-element algorithmic bytes of one residual (the benchmark's byte model):
+    8 [2 nS V + nS m + 2 nF V + (viscous ? 2 nF V D : 0)] + (nF/2)(D+2) 8 + nF 4."""
+    V = D + 2
+    nS = N ** D
+    nF = 2 * D * N ** (D - 1)
+    m = D * D + (1 if viscous else 0)
+    return 8 * (2 * nS * V + nS * m + 2 * nF * V + (2 * nF * V * D if viscous else 0)) \
+        + (nF // 2) * (D + 2) * 8 + nF * 4
+
+
+def stream_copy_gbs(L, nbytes=2 << 30, reps=20):
+    """Device copy bandwidth (read + write bytes / time) with the library's copy kernel."""
+    import torch
+    n = nbytes // 8
+    src = torch.empty(n, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    st = torch.cuda.current_stream()
+    sp = C.c_void_p(st.cuda_stream)
+    for _ in range(3):
+        L.zfr_stream_copy(dst.data_ptr(), src.data_ptr(), n, sp)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    for _ in range(reps):
+        L.zfr_stream_copy(dst.data_ptr(), src.data_ptr(), n, sp)
+    b.record(st)
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    ok = bool(torch.equal(dst[:1024], src[:1024]))
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9, ok
+
+
+def cpu_baseline(workdir, n, order, steps):
+    """The CPU oracle on an n^3 sample of the same workload (OpenMP, all host threads)."""
+    from oracle.oracle import Oracle, make_phys
+    from zefr_amd.mesh import write_box
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from common import tgv3d_state
+    mesh = write_box(os.path.join(workdir, f"cpu_hex{n}.msh"), (n, n, n), (0.0,) * 3, (2 * math.pi,) * 3)
+    U0 = 0.1 * math.sqrt(1.4)
+    mu = 1.0 * U0 * 1.0 / 1600.0
+    o = Oracle(mesh, 3, order, make_phys(viscous=True, gamma=1.4, prandtl=0.71, mu=mu, fix_vis=True,
+                                         rus_k=0.0, ldg_b=0.5, ldg_tau=0.1, dt=2e-4))
+    x = o.coord_spts()
+    o.set_U(tgv3d_state(x[:, 0, :], x[:, 1, :], x[:, 2, :]))
+    o.compute_residual(0)
+    t = time.perf_counter()
+    for _ in range(steps):
+        o.compute_residual(0)
+    dt = (time.perf_counter() - t) / steps
+    dof = o.nEles * o.nSpts * o.nVars
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    return {"value": dof / dt / 1e9, "unit": "Gdof-residuals/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (oracle/zefr_oracle.cpp, OpenMP) on the same TGV case at {n}^3 hexes "
+                      f"p={order} ({dof / 1e6:.2f}M DoF), median-free mean of {steps} residuals "
+                      f"({dt * 1e3:.1f} ms each)", "ms_per_residual": dt * 1e3}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo exchange is a design, see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    from zefr_amd import zefr
+    from zefr_amd._lib import lib
+    from zefr_amd.mesh import write_box
+
+    workdir = args.workdir or tempfile.mkdtemp(prefix="zefr_bench_")
+    n, order = args.n, args.order
+    D, N = 3, order + 1
+    t0 = time.perf_counter()
+    mesh = write_box(os.path.join(workdir, f"hex{n}.msh"), (n, n, n), (0.0,) * 3, (2 * math.pi,) * 3)
+    deck = write_deck(os.path.join(workdir, "tgv.deck"), mesh, order)
+    z = zefr.Zefr()
+    z.read_input(deck)
+    z.setup_solver()
+    setup_s = time.perf_counter() - t0
+    d = z.dims()
+    E, nS, V = d["nEles"], d["nSpts"], d["nVars"]
+    dof = E * nS * V
+
+    tile = z.autotune(3) if args.tile < 0 else (z.set_tile(args.tile) or args.tile)
+
+    for _ in range(args.warmup):
+        z.compute_residual(0)
+    z.synchronize()
+    torch.cuda.synchronize()
+    ms3 = (C.c_double * 3)()
+    t = time.perf_counter()
+    rc = lib().zefr_time_residual_kernels(z._h, args.steps, ms3)
+    z.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    if rc != 0:
+        raise RuntimeError(lib().zefr_last_error().decode())
+    ms_step = wall / args.steps * 1e3
+    value = dof / (wall / args.steps) / 1e9
+
+    kb = kernel_bytes(D, N, True, True)
+    names = ["extrapolate", "gradient", "divergence"]
+    kms = dict(zip(names, list(ms3)))
+    dom = max(names, key=lambda k: kms[k])
+    achieved = kb[dom] * E / (kms[dom] * 1e-3) / 1e9
+    per_kernel = {k: {"avg_ms": kms[k], "alg_bytes_per_element": kb[k],
+                      "achieved_GBs": kb[k] * E / (kms[k] * 1e-3) / 1e9 if kms[k] > 0 else None}
+                  for k in names}
+    model = residual_model_bytes(D, N, True)
+    res_gbs = model * E / (ms_step * 1e-3) / 1e9
+    copy_gbs, copy_ok = stream_copy_gbs(lib())
+
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gdof-residuals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: generated periodic hex mesh, Taylor-Green initial state",
+        "config": {"workload": f"3D Navier-Stokes Taylor-Green residual, {n}^3 hexes on [0,2pi]^3, p={order}",
+                   "elements": E, "spts_per_element": nS, "vars": V, "dof": dof, "Re": 1600, "Ma": 0.1,
+                   "Pr": 0.71, "flux": "Rusanov+LDG(beta=0.5,tau=0.1)", "tile_eles_per_block": tile,
+                   "setup_s": round(setup_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"k_divergence<3,{N},true>" if dom == "divergence" else dom,
+                     "alg_bytes_per_launch": kb[dom] * E, "avg_launch_ms": round(kms[dom], 4)},
+        "roofline_residual": {"bytes_per_element_model": model, "achieved": round(res_gbs, 1),
+                              "frac_of_peak": round(res_gbs / HBM_PEAK_GBS, 4),
+                              "stream_copy_GBs": round(copy_gbs, 1), "stream_copy_ok": copy_ok,
+                              "frac_of_stream_copy": round(res_gbs / copy_gbs, 4)},
+        "kernels": per_kernel,
+    }
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(workdir, args.cpu_n, order, args.cpu_steps)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

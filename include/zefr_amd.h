@@ -145,6 +145,11 @@ long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n);
  * candidates on the device (0 = keep the built-in default). Returns the chosen value. */
 int zefr_autotune(zefr_t *z, int reps);
 int zefr_set_tile(zefr_t *z, int eles_per_block);
+/* Run `reps` residuals (stage 0) with HIP events around each kernel on the solver's
+ * stream; ms_out[0..2] = average duration of the extrapolate / gradient / divergence
+ * launch (gradient is 0 for inviscid runs).  No counterpart in the reference: this is the
+ * timing hook of the benchmark. */
+int zefr_time_residual_kernels(zefr_t *z, int reps, double *ms_out);
 
 /* ---------------------------------------------------------------- singleton ------ */
 
@@ -204,6 +209,9 @@ int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const 
  * sum squares (res_type 0/1/2), reduced on the device; work holds >= 1024*V doubles. */
 int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work,
                       double *res_out_device, void *stream);
+/* Plain device copy dst[i] = src[i], i < n (bandwidth calibration for the benchmark; no
+ * counterpart in the reference). */
+int zfr_stream_copy(double *dst, const double *src, long n, void *stream);
 
 #ifdef __cplusplus
 }

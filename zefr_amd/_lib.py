@@ -97,6 +97,7 @@ SIGNATURES = {
     "zefr_get_host_array": (C.c_long, [_P, _CP, _DP, C.c_long]),
     "zefr_autotune": (_I, [_P, _I]),
     "zefr_set_tile": (_I, [_P, _I]),
+    "zefr_time_residual_kernels": (_I, [_P, _I, _DP]),
     "zefr_initialize": (_I, [_CP]),
     "zefr_get_zefr_object": (_P, []),
     "zefr_set_zefr_object": (None, [_P]),
@@ -113,6 +114,7 @@ SIGNATURES = {
     "zfr_rk_stage": (_I, [C.POINTER(EleDesc), _P, _P, _P, C.c_double, C.c_double, _P]),
     "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_residual_norm": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P, _P]),
+    "zfr_stream_copy": (_I, [_P, _P, C.c_long, _P]),
 }
 
 

@@ -320,14 +320,14 @@ long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n) {
     if (nm == "oppE" || nm == "oppD" || nm == "oppD_fpts" || nm == "oppDiv" || nm == "oppDiv_fpts") {
       tmp = e.re.dense_operator(nm);
       src = &tmp;
-    } else if (nm == "coord_spts") src = &e.coord_spts;
-    else if (nm == "coord_fpts") src = &e.coord_fpts;
-    else if (nm == "jaco_det_spts") src = &e.jaco_det_spts;
-    else if (nm == "inv_jaco_spts") src = &e.inv_jaco_spts;
-    else if (nm == "inv_jaco_fpts") src = &e.inv_jaco_fpts;
-    else if (nm == "norm") src = &s.faces->norm;
-    else if (nm == "dA") src = &s.faces->dA;
-    else if (nm == "weights_spts") src = &e.re.weights_spts;
+    } else if (nm == "coord_spts" || nm == "coord_fpts" || nm == "jaco_det_spts" || nm == "inv_jaco_spts" ||
+               nm == "inv_jaco_fpts") {
+      tmp = e.export_array(nm);
+      src = &tmp;
+    } else if (nm == "norm" || nm == "dA") {
+      tmp = s.faces->export_array(nm, e, s.conn);
+      src = &tmp;
+    } else if (nm == "weights_spts") src = &e.re.weights_spts;
     else if (nm == "ops1d") src = &e.re.ops1d;
     else if (nm == "U_init") src = &e.U_host;
     else zfr::raise("unknown host array " + nm);
@@ -339,6 +339,10 @@ long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n) {
 
 int zefr_autotune(zefr_t *z, int reps) {
   return zfr::guard_val<int>(-1, [&] { return z->s().autotune(reps < 1 ? 3 : reps); });
+}
+
+int zefr_time_residual_kernels(zefr_t *z, int reps, double *ms_out) {
+  return zfr::guard([&] { z->s().time_kernels(reps, ms_out); });
 }
 
 int zefr_set_tile(zefr_t *z, int eles_per_block) {

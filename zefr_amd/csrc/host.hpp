@@ -11,6 +11,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <array>
+#include <functional>
 #include <cstddef>
 #include <memory>
 #include <string>
@@ -71,7 +72,9 @@ struct Connectivity {
   std::vector<int> gf_ele, gf_fpt;   // [2][nGfpts]: element / element fpt of each side
 };
 
-Connectivity build_connectivity(const Mesh &m, const RefElement &re, const std::vector<double> &coord_fpts);
+// fpt_coord(e, f, x) writes the physical coordinates of flux point f of element e.
+Connectivity build_connectivity(const Mesh &m, const RefElement &re,
+                                const std::function<void(int, int, double *)> &fpt_coord);
 
 // ---------------------------------------------------------------- device buffers
 struct DevBuf {
@@ -89,41 +92,43 @@ struct DevBuf {
 // ---------------------------------------------------------------- solver classes
 class Faces;
 
-// One element-type block (include/elements.hpp:237-308).
+// One element-type block (include/elements.hpp:237-308).  Geometry is computed per
+// element straight into the element-major device layout; the reference-layout arrays
+// (coord_spts, inv_jaco_spts, ...) are produced only on request by export_array().
 class Elements {
  public:
   int etype = 0, nDims = 0, order = 0, nEles = 0, nSpts = 0, nFpts = 0, nVars = 0, nNodes = 0;
   RefElement re;
-  // host geometry in the reference layouts (src/elements.cpp:214-388, 845-996)
-  std::vector<double> nodes;          // [nNodes][nDims][nEles]
-  std::vector<double> coord_spts;     // [nSpts][nDims][nEles]
-  std::vector<double> coord_fpts;     // [nFpts][nDims][nEles]
-  std::vector<double> inv_jaco_spts;  // [nDims][nSpts][nDims][nEles] (adjugate)
-  std::vector<double> jaco_det_spts;  // [nSpts][nEles]
-  std::vector<double> inv_jaco_fpts;  // [nDims][nFpts][nDims][nEles]
+  std::vector<double> shp_s, dshp_s, shp_f, dshp_f;  // shape values [n][nNodes], derivatives [n][nDims][nNodes]
+  std::vector<double> X;                             // element nodes [E][nNodes][nDims]
+  std::vector<double> metric;                        // [E][D*D+1][nS]: adjugate(d1,d2), detJ
   // device data (element-major)
   DevBuf U_spts_d, U_ini_d, divF_d, dU_spts_d, metric_d, fgeo_d, nbr_d, ops_d, Utr_d, Gtr_d;
   std::vector<double> U_host, dU_host;  // host mirrors for get_u_spts / get_du_spts
   int tile = 0;                         // elements per workgroup (0 = kernel default)
 
   void setup_geometry(const Mesh &m);
-  void calc_transforms();
+  void point_coord(int e, bool fpt, int p, double *x) const;
+  void jacobian(int e, bool fpt, int p, double J[3][3]) const;  // J[r][d] = dx_d / dxi_r
+  // adjugate A (A[a][b] in the reference's inv_jaco(a, pt, b) order) and determinant
+  static double adjugate(int D, const double J[3][3], double A[3][3]);
   std::vector<double> initial_state(const zefr_input_t &in) const;  // [E][V][nS]
+  std::vector<double> export_array(const std::string &name) const;  // reference layouts
   zfr_ele_desc_t desc(const zefr_input_t &in) const;
   size_t state_size() const { return (size_t)nEles * nVars * nSpts; }
 };
 
-// Interface data (include/faces.hpp:147-167): normals, areas and the element-fpt
-// partner map that replaces the reference's per-gfpt pointer views.
+// Interface data (include/faces.hpp:147-167): per element flux point, the unit normal of
+// its global flux point (slot-0 side) and this side's signed area, plus the partner map
+// that replaces the reference's per-gfpt pointer views.
 class Faces {
  public:
   int nGfpts = 0;
-  std::vector<double> norm;  // [nDims][nGfpts] unit normal, slot-0 side
-  std::vector<double> dA;    // [2][nGfpts]
+  std::vector<double> fgeo;   // [E][D+1][nF]
+  std::vector<unsigned> nbr;  // [E][nF]: partner ele << 9 | own slot << 8 | partner fpt
   void setup(const Elements &e, const Connectivity &c);
-  // element-major device tables
-  std::vector<double> fgeo_host(const Elements &e, const Connectivity &c) const;
-  std::vector<unsigned> nbr_host(const Elements &e, const Connectivity &c) const;
+  // reference layouts: norm [nDims][nGfpts], dA [2][nGfpts]
+  std::vector<double> export_array(const std::string &name, const Elements &e, const Connectivity &c) const;
 };
 
 class FRSolver {
@@ -151,6 +156,7 @@ class FRSolver {
   void compute_gradient_export();  // fills dU_spts_d with the physical gradient of U
   void synchronize();
   int autotune(int reps);
+  void time_kernels(int reps, double *ms_out);
 
   zefr_input_t *input;
   int order;

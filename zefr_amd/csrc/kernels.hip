@@ -690,6 +690,19 @@ __global__ __launch_bounds__(kMaxThreads) void k_norm_partial(const double *__re
   }
 }
 
+__global__ __launch_bounds__(kMaxThreads) void k_copy(double *__restrict__ dst, const double *__restrict__ src,
+                                                      long n) {
+  const long stride = (long)gridDim.x * blockDim.x * 2;
+  for (long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < n; i += stride) {
+    if (i + 1 < n) {
+      const double2 v = *reinterpret_cast<const double2 *>(src + i);
+      *reinterpret_cast<double2 *>(dst + i) = v;
+    } else {
+      dst[i] = src[i];
+    }
+  }
+}
+
 __global__ void k_norm_final(const double *__restrict__ part, int nblk, int V, int type, double *__restrict__ out) {
   const int v = threadIdx.x;
   if (v >= V) return;
@@ -839,6 +852,13 @@ int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type,
                        (long)d->nEles, d->nSpts, res_type, work);
   hipLaunchKernelGGL(zk::k_norm_final, dim3(1), dim3(64), 0, st, work, (int)nblk, d->nVars, res_type, res_out);
   return zfr::check_hip(hipGetLastError(), "zfr_residual_norm");
+}
+
+int zfr_stream_copy(double *dst, const double *src, long n, void *stream) {
+  if (n <= 0) return 0;
+  if (((uintptr_t)dst | (uintptr_t)src) & 15) return zfr::fail("zfr_stream_copy: pointers must be 16-byte aligned");
+  hipLaunchKernelGGL(zk::k_copy, dim3(4096), dim3(zk::kMaxThreads), 0, (hipStream_t)stream, dst, src, n);
+  return zfr::check_hip(hipGetLastError(), "zfr_stream_copy");
 }
 
 }  // extern "C"

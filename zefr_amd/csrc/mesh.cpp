@@ -311,7 +311,8 @@ void fuzzy_rank(const double *pts, int n, int D, int *rank, double tol = 1e-6) {
 
 }  // namespace
 
-Connectivity build_connectivity(const Mesh &m, const RefElement &re, const std::vector<double> &coord_fpts) {
+Connectivity build_connectivity(const Mesh &m, const RefElement &re,
+                                const std::function<void(int, int, double *)> &fpt_coord) {
   const int D = m.nDims, E = m.nEles, nF = re.nFpts, n = re.nFptsPerFace;
   const auto &fn = face_nodes(D);
   const int nFaceEle = (int)fn.size();
@@ -371,21 +372,25 @@ Connectivity build_connectivity(const Mesh &m, const RefElement &re, const std::
       if (c.slot[k] == 0) rank0[k] = r++;
   }
   c.fpt2gfpt.assign((size_t)nF * E, -1);
-  std::vector<double> pl((size_t)n * D), pr((size_t)n * D);
-  std::vector<int> rl(n), rr(n);
-  auto cf = [&](int f, int d, int e) { return coord_fpts[((size_t)f * D + d) * E + e]; };
-  for (const auto &r : faces) {
-    const int fL = r.lfL * n, fR = r.lfR * n;
-    for (int i = 0; i < n; i++)
-      for (int d = 0; d < D; d++) {
-        pl[i * D + d] = cf(fL + i, d, r.eL);
-        pr[i * D + d] = cf(fR + (n - 1 - i), d, r.eR);  // second owner sees the face reversed
+  const long nFc = (long)faces.size();
+#pragma omp parallel
+  {
+    std::vector<double> pl((size_t)n * D), pr((size_t)n * D);
+    std::vector<int> rl(n), rr(n);
+#pragma omp for schedule(static)
+    for (long k = 0; k < nFc; k++) {
+      const FaceRec &r = faces[k];
+      const int fL = r.lfL * n, fR = r.lfR * n;
+      for (int i = 0; i < n; i++) {
+        fpt_coord(r.eL, fL + i, &pl[(size_t)i * D]);
+        fpt_coord(r.eR, fR + (n - 1 - i), &pr[(size_t)i * D]);  // second owner sees the face reversed
       }
-    fuzzy_rank(pl.data(), n, D, rl.data());
-    fuzzy_rank(pr.data(), n, D, rr.data());
-    for (int i = 0; i < n; i++) {
-      c.fpt2gfpt[(size_t)(fL + i) * E + r.eL] = rank0[(size_t)(fL + rl[i]) * E + r.eL];
-      c.fpt2gfpt[(size_t)(fR + (n - 1 - i)) * E + r.eR] = rank0[(size_t)(fL + rr[i]) * E + r.eL];
+      fuzzy_rank(pl.data(), n, D, rl.data());
+      fuzzy_rank(pr.data(), n, D, rr.data());
+      for (int i = 0; i < n; i++) {
+        c.fpt2gfpt[(size_t)(fL + i) * E + r.eL] = rank0[(size_t)(fL + rl[i]) * E + r.eL];
+        c.fpt2gfpt[(size_t)(fR + (n - 1 - i)) * E + r.eR] = rank0[(size_t)(fL + rr[i]) * E + r.eL];
+      }
     }
   }
   c.gf_ele.assign((size_t)2 * c.nGfpts, -1);

@@ -2,11 +2,15 @@
 //
 // Setup follows the reference's sequence (src/solver.cpp:55-180): read the mesh, build
 // connectivity, compute geometry and metrics, set the initial state, move everything to
-// the device.  The residual and the RK stages are kernel launches on the solver's own
-// stream; nothing on the hot path touches the host.
+// the device.  Geometry is computed element by element straight into the element-major
+// device layout (the reference keeps point-major [pt][dim][ele] arrays of several GB at
+// the benchmark size).  The residual and the RK stages are kernel launches on the
+// solver's own stream; nothing on the hot path touches the host.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -39,76 +43,133 @@ static void upload(DevBuf &b, const std::vector<T> &v) {
 }
 
 // ---------------------------------------------------------------- Elements
-// Physical coordinates and mapping derivatives at spts / fpts (src/elements.cpp:325-388,
-// 845-996): x = sum_n shape_n(xi) x_n, J(r,p) = dx_p / dxi_r, adjugate and determinant.
 void Elements::setup_geometry(const Mesh &m) {
-  const int D = nDims, nN = nNodes, E = nEles;
-  nodes.assign((size_t)nN * D * E, 0.0);
-  for (int e = 0; e < E; e++)
-    for (int n = 0; n < nN; n++)
-      for (int d = 0; d < D; d++)
-        nodes[((size_t)n * D + d) * E + e] = m.coord[(size_t)m.ele2nodes[(size_t)e * nN + n] * D + d];
-  auto eval = [&](const std::vector<double> &locs, int npts, std::vector<double> &coord,
-                  std::vector<double> *jac) {
-    coord.assign((size_t)npts * D * E, 0.0);
-    if (jac) jac->assign((size_t)D * npts * D * E, 0.0);
+  const int D = nDims, nN = nNodes, E = nEles, nS = nSpts, nM = D * D + 1;
+  // shape function tables at spts and fpts (src/elements.cpp:214-323)
+  auto tables = [&](const std::vector<double> &locs, int npts, std::vector<double> &shp, std::vector<double> &dshp) {
+    shp.assign((size_t)npts * nN, 0.0);
+    dshp.assign((size_t)npts * D * nN, 0.0);
     std::vector<double> val(nN), dval((size_t)nN * D);
     for (int p = 0; p < npts; p++) {
       re.shape(&locs[(size_t)p * D], val.data(), dval.data());
-      for (int d = 0; d < D; d++)
-        for (int e = 0; e < E; e++) {
-          double x = 0.0;
-          for (int n = 0; n < nN; n++) x += val[n] * nodes[((size_t)n * D + d) * E + e];
-          coord[((size_t)p * D + d) * E + e] = x;
-        }
-      if (jac)
-        for (int r = 0; r < D; r++)
-          for (int d = 0; d < D; d++)
-            for (int e = 0; e < E; e++) {
-              double j = 0.0;
-              for (int n = 0; n < nN; n++) j += dval[(size_t)n * D + r] * nodes[((size_t)n * D + d) * E + e];
-              (*jac)[(((size_t)r * npts + p) * D + d) * E + e] = j;
-            }
+      for (int n = 0; n < nN; n++) {
+        shp[(size_t)p * nN + n] = val[n];
+        for (int r = 0; r < D; r++) dshp[((size_t)p * D + r) * nN + n] = dval[(size_t)n * D + r];
+      }
     }
   };
-  std::vector<double> jaco_spts, jaco_fpts;
-  eval(re.loc_spts, nSpts, coord_spts, &jaco_spts);
-  eval(re.loc_fpts, nFpts, coord_fpts, &jaco_fpts);
-  auto invert = [&](const std::vector<double> &jac, int npts, std::vector<double> &inv, std::vector<double> *det) {
-    inv.assign(jac.size(), 0.0);
-    if (det) det->assign((size_t)npts * E, 0.0);
-    auto J = [&](int a, int p, int b, int e) { return jac[(((size_t)a * npts + p) * D + b) * E + e]; };
-    auto I = [&](int a, int p, int b, int e) -> double & { return inv[(((size_t)a * npts + p) * D + b) * E + e]; };
+  tables(re.loc_spts, nSpts, shp_s, dshp_s);
+  tables(re.loc_fpts, nFpts, shp_f, dshp_f);
+  X.assign((size_t)E * nN * D, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < E; e++)
+    for (int n = 0; n < nN; n++)
+      for (int d = 0; d < D; d++)
+        X[((size_t)e * nN + n) * D + d] = m.coord[(size_t)m.ele2nodes[(size_t)e * nN + n] * D + d];
+  // metric terms at spts: adjugate of the mapping Jacobian and its determinant
+  // (src/elements.cpp:845-875, 964-996)
+  metric.assign((size_t)E * nM * nS, 0.0);
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+  for (int e = 0; e < E; e++)
+    for (int s = 0; s < nS; s++) {
+      double J[3][3], A[3][3];
+      jacobian(e, false, s, J);
+      const double det = adjugate(D, J, A);
+      if (!(det > 0)) bad++;
+      for (int a = 0; a < D; a++)
+        for (int b = 0; b < D; b++) metric[((size_t)e * nM + a * D + b) * nS + s] = A[a][b];
+      metric[((size_t)e * nM + D * D) * nS + s] = det;
+    }
+  if (bad) raise("Negative Jacobian at solution points.");
+}
+
+// x = sum_n shape_n(xi) x_n  (src/elements.cpp:325-388)
+void Elements::point_coord(int e, bool fpt, int p, double *x) const {
+  const int D = nDims, nN = nNodes;
+  const double *shp = fpt ? &shp_f[(size_t)p * nN] : &shp_s[(size_t)p * nN];
+  const double *Xe = &X[(size_t)e * nN * D];
+  for (int d = 0; d < D; d++) {
+    double s = 0.0;
+    for (int n = 0; n < nN; n++) s += shp[n] * Xe[(size_t)n * D + d];
+    x[d] = s;
+  }
+}
+
+// J[r][d] = dx_d / dxi_r = sum_n dshape_n/dxi_r x_n,d  (src/elements.cpp:845-868)
+void Elements::jacobian(int e, bool fpt, int p, double J[3][3]) const {
+  const int D = nDims, nN = nNodes;
+  const double *dshp = fpt ? &dshp_f[(size_t)p * D * nN] : &dshp_s[(size_t)p * D * nN];
+  const double *Xe = &X[(size_t)e * nN * D];
+  for (int r = 0; r < D; r++)
+    for (int d = 0; d < D; d++) {
+      double s = 0.0;
+      for (int n = 0; n < nN; n++) s += dshp[(size_t)r * nN + n] * Xe[(size_t)n * D + d];
+      J[r][d] = s;
+    }
+}
+
+// Adjugate (det * J^-1, stored as the reference's inv_jaco(a, pt, b)) and determinant
+// (src/elements.cpp:964-996).
+double Elements::adjugate(int D, const double J[3][3], double A[3][3]) {
+  if (D == 2) {
+    A[0][0] = J[1][1];  A[0][1] = -J[1][0];
+    A[1][0] = -J[0][1]; A[1][1] = J[0][0];
+    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  }
+  const double xr = J[0][0], xs = J[1][0], xt = J[2][0];
+  const double yr = J[0][1], ys = J[1][1], yt = J[2][1];
+  const double zr = J[0][2], zs = J[1][2], zt = J[2][2];
+  A[0][0] = ys * zt - yt * zs; A[0][1] = xt * zs - xs * zt; A[0][2] = xs * yt - xt * ys;
+  A[1][0] = yt * zr - yr * zt; A[1][1] = xr * zt - xt * zr; A[1][2] = xt * yr - xr * yt;
+  A[2][0] = yr * zs - ys * zr; A[2][1] = xs * zr - xr * zs; A[2][2] = xr * ys - xs * yr;
+  return xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
+}
+
+// Reference-layout geometry for inspection (small meshes): coord_spts / coord_fpts
+// [n][D][E], jaco_det_spts [nS][E], inv_jaco_spts / inv_jaco_fpts [D][n][D][E].
+std::vector<double> Elements::export_array(const std::string &name) const {
+  const int D = nDims, E = nEles;
+  std::vector<double> out;
+  if (name == "coord_spts" || name == "coord_fpts") {
+    const bool f = name == "coord_fpts";
+    const int np = f ? nFpts : nSpts;
+    out.assign((size_t)np * D * E, 0.0);
     for (int e = 0; e < E; e++)
-      for (int p = 0; p < npts; p++) {
-        double dt;
-        if (D == 2) {
-          dt = J(0, p, 0, e) * J(1, p, 1, e) - J(0, p, 1, e) * J(1, p, 0, e);
-          I(0, p, 0, e) = J(1, p, 1, e);  I(0, p, 1, e) = -J(1, p, 0, e);
-          I(1, p, 0, e) = -J(0, p, 1, e); I(1, p, 1, e) = J(0, p, 0, e);
-        } else {
-          const double xr = J(0, p, 0, e), xs = J(1, p, 0, e), xt = J(2, p, 0, e);
-          const double yr = J(0, p, 1, e), ys = J(1, p, 1, e), yt = J(2, p, 1, e);
-          const double zr = J(0, p, 2, e), zs = J(1, p, 2, e), zt = J(2, p, 2, e);
-          dt = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
-          I(0, p, 0, e) = ys * zt - yt * zs; I(0, p, 1, e) = xt * zs - xs * zt; I(0, p, 2, e) = xs * yt - xt * ys;
-          I(1, p, 0, e) = yt * zr - yr * zt; I(1, p, 1, e) = xr * zt - xt * zr; I(1, p, 2, e) = xt * yr - xr * yt;
-          I(2, p, 0, e) = yr * zs - ys * zr; I(2, p, 1, e) = xs * zr - xr * zs; I(2, p, 2, e) = xr * ys - xs * yr;
-        }
-        if (det) {
-          if (!(dt > 0)) raise("Negative Jacobian at solution points.");
-          (*det)[(size_t)p * E + e] = dt;
-        }
+      for (int p = 0; p < np; p++) {
+        double x[3];
+        point_coord(e, f, p, x);
+        for (int d = 0; d < D; d++) out[((size_t)p * D + d) * E + e] = x[d];
       }
-  };
-  invert(jaco_spts, nSpts, inv_jaco_spts, &jaco_det_spts);
-  invert(jaco_fpts, nFpts, inv_jaco_fpts, nullptr);
+  } else if (name == "jaco_det_spts") {
+    out.assign((size_t)nSpts * E, 0.0);
+    for (int e = 0; e < E; e++)
+      for (int s = 0; s < nSpts; s++) out[(size_t)s * E + e] = metric[((size_t)e * (D * D + 1) + D * D) * nSpts + s];
+  } else if (name == "inv_jaco_spts" || name == "inv_jaco_fpts") {
+    const bool f = name == "inv_jaco_fpts";
+    const int np = f ? nFpts : nSpts;
+    out.assign((size_t)D * np * D * E, 0.0);
+    for (int e = 0; e < E; e++)
+      for (int p = 0; p < np; p++) {
+        double J[3][3], A[3][3];
+        jacobian(e, f, p, J);
+        adjugate(D, J, A);
+        for (int a = 0; a < D; a++)
+          for (int b = 0; b < D; b++) out[(((size_t)a * np + p) * D + b) * E + e] = A[a][b];
+      }
+  } else {
+    raise("unknown element array " + name);
+  }
+  return out;
 }
 
 // Initial solution (src/elements.cpp:998-1074, src/funcs.cpp:31-155), element-major.
 std::vector<double> Elements::initial_state(const zefr_input_t &in) const {
   const int D = nDims, V = nVars, E = nEles;
+  const bool tc_ok = in.ic_type == 0 || (in.ic_type == 1 && (in.test_case == 1 || (in.test_case == 4 && D == 3)));
+  if (!tc_ok) raise("unsupported initial condition (ic_type 0, or ic_type 1 with test_case 1 / 4 (3D))");
   std::vector<double> U((size_t)E * V * nSpts, 0.0);
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < E; e++)
     for (int s = 0; s < nSpts; s++) {
       double u[5] = {0, 0, 0, 0, 0};
@@ -117,10 +178,10 @@ std::vector<double> Elements::initial_state(const zefr_input_t &in) const {
         double Vsq = 0.0;
         for (int d = 0; d < D; d++) { u[d + 1] = in.rho_fs * in.V_fs[d]; Vsq += in.V_fs[d] * in.V_fs[d]; }
         u[D + 1] = in.P_fs / (in.gamma - 1.0) + 0.5 * in.rho_fs * Vsq;
-      } else if (in.ic_type == 1) {
-        const double x = coord_spts[((size_t)s * D + 0) * E + e];
-        const double y = coord_spts[((size_t)s * D + 1) * E + e];
-        const double z = (D == 3) ? coord_spts[((size_t)s * D + 2) * E + e] : 0.0;
+      } else {
+        double xyz[3] = {0, 0, 0};
+        point_coord(e, false, s, xyz);
+        const double x = xyz[0], y = xyz[1], z = xyz[2];
         if (in.test_case == 1) {  // isentropic vortex, src/funcs.cpp:63-75
           const double G = 5.0, R = 1.;
           const double f = (1.0 - x * x - y * y) / R;
@@ -131,8 +192,7 @@ std::vector<double> Elements::initial_state(const zefr_input_t &in) const {
           const double P = std::pow(rho, in.gamma);
           u[0] = rho; u[1] = rho * Vx; u[2] = rho * Vy;
           u[D + 1] = P / (in.gamma - 1.0) + 0.5 * rho * (Vx * Vx + Vy * Vy);
-        } else if (in.test_case == 4) {  // Taylor-Green vortex, src/funcs.cpp:125-155
-          if (D != 3) raise("Taylor-Green Vortex test case for 3D cases only");
+        } else {  // Taylor-Green vortex, src/funcs.cpp:125-155
           const double L = in.L_fs, U0 = in.v_mag_fs, Rho0 = in.rho_fs, P0 = in.P_fs;
           const double uu = U0 * std::sin(x / L) * std::cos(y / L) * std::cos(z / L);
           const double vv = -U0 * std::cos(x / L) * std::sin(y / L) * std::cos(z / L);
@@ -140,11 +200,7 @@ std::vector<double> Elements::initial_state(const zefr_input_t &in) const {
           const double rho = p / (in.R * in.T_fs);
           u[0] = rho; u[1] = rho * uu; u[2] = rho * vv; u[3] = 0.0;
           u[4] = p / (in.gamma - 1.0) + 0.5 * rho * (uu * uu + vv * vv);
-        } else {
-          raise("test_case " + std::to_string(in.test_case) + " is not supported (1, 4)");
         }
-      } else {
-        raise("ic_type not recognized!");
       }
       for (int v = 0; v < V; v++) U[((size_t)e * V + v) * nSpts + s] = u[v];
     }
@@ -170,53 +226,60 @@ zfr_ele_desc_t Elements::desc(const zefr_input_t &in) const {
 }
 
 // ---------------------------------------------------------------- Faces
-// Unit normal and area of every global flux point (src/elements.cpp:876-941):
-// n = adj(J)^T tnorm at the element flux point, dA(slot) = |n|, unit normal from slot 0.
+// Normal and area at every element flux point (src/elements.cpp:876-941):
+// n = adj(J)^T tnorm, dA(slot) = |n|; the unit normal of a global flux point is the one
+// seen from its slot-0 element.  Pass 1 stores each side's own normal, pass 2 replaces
+// slot-1 normals by the partner's.
 void Faces::setup(const Elements &el, const Connectivity &c) {
-  const int D = el.nDims, E = el.nEles, nF = el.nFpts;
+  const int D = el.nDims, E = el.nEles, nF = el.nFpts, R = D + 1;
+  if ((long)E >= (1L << 23)) raise("element count exceeds the 23-bit partner index");
   nGfpts = c.nGfpts;
-  norm.assign((size_t)D * nGfpts, 0.0);
-  dA.assign((size_t)2 * nGfpts, 0.0);
+  fgeo.assign((size_t)E * R * nF, 0.0);
+  nbr.assign((size_t)E * nF, 0u);
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < E; e++)
     for (int f = 0; f < nF; f++) {
-      const int g = c.fpt2gfpt[(size_t)f * E + e], s = c.slot[(size_t)f * E + e];
+      double J[3][3], A[3][3];
+      el.jacobian(e, true, f, J);
+      Elements::adjugate(D, J, A);
       double n[3] = {0, 0, 0};
       for (int d1 = 0; d1 < D; d1++)
-        for (int d2 = 0; d2 < D; d2++)
-          n[d1] += el.inv_jaco_fpts[(((size_t)d2 * nF + f) * D + d1) * E + e] * el.re.tnorm[(size_t)f * D + d2];
+        for (int d2 = 0; d2 < D; d2++) n[d1] += A[d2][d1] * el.re.tnorm[(size_t)f * D + d2];
       double a = 0.0;
       for (int d = 0; d < D; d++) a += n[d] * n[d];
       a = std::sqrt(a);
-      dA[(size_t)s * nGfpts + g] = a;
-      if (s == 0)
-        for (int d = 0; d < D; d++) norm[(size_t)d * nGfpts + g] = n[d] / a;
-    }
-}
-
-// Per element flux point: the gfpt's unit normal (D) and this side's signed area.
-std::vector<double> Faces::fgeo_host(const Elements &el, const Connectivity &c) const {
-  const int D = el.nDims, E = el.nEles, nF = el.nFpts;
-  std::vector<double> out((size_t)E * (D + 1) * nF, 0.0);
-  for (int e = 0; e < E; e++)
-    for (int f = 0; f < nF; f++) {
-      const int g = c.fpt2gfpt[(size_t)f * E + e], s = c.slot[(size_t)f * E + e];
-      for (int d = 0; d < D; d++) out[((size_t)e * (D + 1) + d) * nF + f] = norm[(size_t)d * nGfpts + g];
-      const double a = dA[(size_t)s * nGfpts + g];
-      out[((size_t)e * (D + 1) + D) * nF + f] = s ? -a : a;
-    }
-  return out;
-}
-
-// Per element flux point: partner element, slot of this side, partner flux point.
-std::vector<unsigned> Faces::nbr_host(const Elements &el, const Connectivity &c) const {
-  const int E = el.nEles, nF = el.nFpts;
-  if ((long)E >= (1L << 23)) raise("element count exceeds the 23-bit partner index");
-  std::vector<unsigned> out((size_t)E * nF, 0u);
-  for (int e = 0; e < E; e++)
-    for (int f = 0; f < nF; f++) {
-      const int g = c.fpt2gfpt[(size_t)f * E + e], s = c.slot[(size_t)f * E + e];
+      const int s = c.slot[(size_t)f * E + e];
+      for (int d = 0; d < D; d++) fgeo[((size_t)e * R + d) * nF + f] = n[d] / a;
+      fgeo[((size_t)e * R + D) * nF + f] = s ? -a : a;
+      const int g = c.fpt2gfpt[(size_t)f * E + e];
       const int pe = c.gf_ele[(size_t)(1 - s) * c.nGfpts + g], pf = c.gf_fpt[(size_t)(1 - s) * c.nGfpts + g];
-      out[(size_t)e * nF + f] = ((unsigned)pe << 9) | ((unsigned)s << 8) | (unsigned)pf;
+      nbr[(size_t)e * nF + f] = ((unsigned)pe << 9) | ((unsigned)s << 8) | (unsigned)pf;
+    }
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < E; e++)
+    for (int f = 0; f < nF; f++) {
+      const unsigned p = nbr[(size_t)e * nF + f];
+      if (((p >> 8) & 1u) == 0) continue;
+      const size_t pe = p >> 9, pf = p & 255u;
+      for (int d = 0; d < D; d++) fgeo[((size_t)e * R + d) * nF + f] = fgeo[(pe * R + d) * nF + pf];
+    }
+}
+
+std::vector<double> Faces::export_array(const std::string &name, const Elements &el, const Connectivity &c) const {
+  const int D = el.nDims, E = el.nEles, nF = el.nFpts, R = D + 1;
+  std::vector<double> out;
+  if (name == "norm") out.assign((size_t)D * nGfpts, 0.0);
+  else if (name == "dA") out.assign((size_t)2 * nGfpts, 0.0);
+  else raise("unknown face array " + name);
+  for (int e = 0; e < E; e++)
+    for (int f = 0; f < nF; f++) {
+      const int g = c.fpt2gfpt[(size_t)f * E + e], s = c.slot[(size_t)f * E + e];
+      if (name == "norm") {
+        if (s == 0)
+          for (int d = 0; d < D; d++) out[(size_t)d * nGfpts + g] = fgeo[((size_t)e * R + d) * nF + f];
+      } else {
+        out[(size_t)s * nGfpts + g] = std::abs(fgeo[((size_t)e * R + D) * nF + f]);
+      }
     }
   return out;
 }
@@ -228,38 +291,52 @@ FRSolver::~FRSolver() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+namespace {
+// Setup stage timing, printed when ZEFR_VERBOSE is set (the reference prints its setup
+// stages unconditionally, src/solver.cpp:62-180).
+struct StageClock {
+  bool on = std::getenv("ZEFR_VERBOSE") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void lap(const char *what) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[zefr] %-28s %8.3f s\n", what, std::chrono::duration<double>(now - t).count());
+    t = now;
+  }
+};
+}  // namespace
+
 void FRSolver::setup_host() {
   if (input->equation != 1) raise("only the EulerNS equation set is supported");
   if (input->nStages != 1 && input->nStages != 4) raise("nStages must be 1 (Euler) or 4 (RK44)");
+  StageClock clk;
   mesh = read_gmsh(input->meshfile, *input);
+  clk.lap("read mesh + periodic pairs");
   eles = std::make_shared<Elements>();
   Elements &el = *eles;
   el.re.build((int)input->nDims, order);
   el.etype = mesh.etype; el.nDims = mesh.nDims; el.order = order; el.nEles = mesh.nEles;
   el.nSpts = el.re.nSpts; el.nFpts = el.re.nFpts; el.nVars = mesh.nDims + 2; el.nNodes = mesh.nNodesPerEle;
   el.setup_geometry(mesh);
-  conn = build_connectivity(mesh, el.re, el.coord_fpts);
+  clk.lap("shape tables + metrics");
+  conn = build_connectivity(mesh, el.re, [&el](int e, int f, double *x) { el.point_coord(e, true, f, x); });
+  clk.lap("connectivity");
   faces = std::make_shared<Faces>();
   faces->setup(el, conn);
+  clk.lap("face normals + partner map");
   el.U_host = el.initial_state(*input);
+  clk.lap("initial state");
 }
 
 void FRSolver::setup_device() {
   Elements &el = *eles;
-  const int D = el.nDims, E = el.nEles, nS = el.nSpts, nF = el.nFpts, V = el.nVars, nM = D * D + 1;
+  const int D = el.nDims, E = el.nEles, nF = el.nFpts, V = el.nVars;
+  StageClock clk;
   if (!stream) hip_or_throw(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   upload(el.ops_d, el.re.ops1d);
-  std::vector<double> metric((size_t)E * nM * nS);
-  for (int e = 0; e < E; e++)
-    for (int s = 0; s < nS; s++) {
-      for (int a = 0; a < D; a++)
-        for (int b = 0; b < D; b++)
-          metric[((size_t)e * nM + a * D + b) * nS + s] = el.inv_jaco_spts[(((size_t)a * nS + s) * D + b) * E + e];
-      metric[((size_t)e * nM + D * D) * nS + s] = el.jaco_det_spts[(size_t)s * E + e];
-    }
-  upload(el.metric_d, metric);
-  upload(el.fgeo_d, faces->fgeo_host(el, conn));
-  upload(el.nbr_d, faces->nbr_host(el, conn));
+  upload(el.metric_d, el.metric);
+  upload(el.fgeo_d, faces->fgeo);
+  upload(el.nbr_d, faces->nbr);
   const size_t st = el.state_size();
   upload(el.U_spts_d, el.U_host);
   el.U_ini_d.alloc(st * sizeof(double));
@@ -270,6 +347,7 @@ void FRSolver::setup_device() {
   norm_work_d.alloc(1024 * 8 * sizeof(double));
   norm_out_d.alloc(8 * sizeof(double));
   device_ready = true;
+  clk.lap("device upload");
 }
 
 void FRSolver::synchronize() { hip_or_throw(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
@@ -439,6 +517,33 @@ int FRSolver::autotune(int reps) {
   (void)hipEventDestroy(b);
   el.tile = best_eb;
   return best_eb;
+}
+
+// Residuals with an event pair around every launch (benchmark instrumentation).
+void FRSolver::time_kernels(int reps, double *ms_out) {
+  need_device(*this);
+  Elements &el = *eles;
+  const auto d = el.desc(*input);
+  hipEvent_t ev[4];
+  for (auto &e : ev) hip_or_throw(hipEventCreate(&e), "hipEventCreate");
+  double acc[3] = {0, 0, 0};
+  for (int r = 0; r < reps; r++) {
+    hip_or_throw(hipEventRecord(ev[0], stream), "hipEventRecord");
+    status_or_throw(zfr_extrapolate(&d, el.U_spts_d.as<double>(), stream));
+    hip_or_throw(hipEventRecord(ev[1], stream), "hipEventRecord");
+    if (input->viscous) status_or_throw(zfr_gradient(&d, el.U_spts_d.as<double>(), nullptr, stream));
+    hip_or_throw(hipEventRecord(ev[2], stream), "hipEventRecord");
+    status_or_throw(zfr_divergence(&d, el.U_spts_d.as<double>(), el.divF_d.as<double>(), stream));
+    hip_or_throw(hipEventRecord(ev[3], stream), "hipEventRecord");
+    hip_or_throw(hipEventSynchronize(ev[3]), "hipEventSynchronize");
+    for (int k = 0; k < 3; k++) {
+      float ms = 0.f;
+      hip_or_throw(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "hipEventElapsedTime");
+      acc[k] += ms;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 3; k++) ms_out[k] = acc[k] / std::max(reps, 1);
 }
 
 // ---------------------------------------------------------------- PMGrid
