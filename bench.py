@@ -84,19 +84,27 @@ output_prefix tgv
     return path
 
 
-def kernel_bytes(D, N, viscous, beta_half):
-    """Algorithmic bytes per element of each kernel: every input read once, every output
-    written once (FP64 values, int32 partner map).  With LDG beta = +-1/2 one side's
-    gradient trace per face is produced and consumed."""
+def kernel_bytes(D, N):
+    """Algorithmic bytes per element of the three launches of one viscous residual with
+    LDG beta = +-1/2 (every input read once, every output written once; FP64 values,
+    uint32 partner map).  Each face has exactly one owner side, so an element owns nF/2
+    flux points on average.
+      extrapolate   : U (nS V) -> Utr (nF V)
+      gradient_flux : U, partner Utr at owned fpts (nF/2 V), adjugate + detJ (nS (D^2+1)),
+                      owned normals (nF/2 D), nbr (nF) -> Ftr at owned fpts (nF/2 V), divF (nS V)
+      face_correction: divF (nS V) in/out, owner Ftr (nF V), signed dA (nF), nbr (nF)"""
     V = D + 2
     nS = N ** D
     nF = 2 * D * N ** (D - 1)
-    m = D * D + (1 if viscous else 0)
-    g = nF * V * D // (2 if beta_half else 1)
+    h = nF // 2
     extrap = 8 * (nS * V + nF * V)
-    grad = 8 * (nS * V + nF * V + nS * (D * D + 1) + g) + 4 * nF
-    div = 8 * (nS * V + nF * V + nS * m + nF * (D + 1) + (g if viscous else 0) + nS * V) + 4 * nF
-    return {"extrapolate": extrap, "gradient": grad if viscous else 0, "divergence": div}
+    gflux = 8 * (nS * V + h * V + nS * (D * D + 1) + h * D + h * V + nS * V) + 4 * nF
+    fcorr = 8 * (2 * nS * V + nF * V + nF) + 4 * nF
+    return {"extrapolate": extrap, "gradient_flux": gflux, "face_correction": fcorr}
+
+
+KERNEL_NAMES = {"extrapolate": "k_extrapolate<3,{N}>", "gradient_flux": "k_visc_onesided<3,{N},1>",
+                "face_correction": "k_face_correction<3,{N},1>"}
 
 
 def residual_model_bytes(D, N, viscous):
@@ -207,12 +215,12 @@ def main():
     ms_step = wall / args.steps * 1e3
     value = dof / (wall / args.steps) / 1e9
 
-    kb = kernel_bytes(D, N, True, True)
-    names = ["extrapolate", "gradient", "divergence"]
+    kb = kernel_bytes(D, N)
+    names = ["extrapolate", "gradient_flux", "face_correction"]
     kms = dict(zip(names, list(ms3)))
     dom = max(names, key=lambda k: kms[k])
     achieved = kb[dom] * E / (kms[dom] * 1e-3) / 1e9
-    per_kernel = {k: {"avg_ms": kms[k], "alg_bytes_per_element": kb[k],
+    per_kernel = {KERNEL_NAMES[k].format(N=N): {"avg_ms": kms[k], "alg_bytes_per_element": kb[k],
                       "achieved_GBs": kb[k] * E / (kms[k] * 1e-3) / 1e9 if kms[k] > 0 else None}
                   for k in names}
     model = residual_model_bytes(D, N, True)
@@ -238,7 +246,7 @@ def main():
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_divergence<3,{N},true>" if dom == "divergence" else dom,
+                     "kernel": KERNEL_NAMES[dom].format(N=N),
                      "alg_bytes_per_launch": kb[dom] * E, "avg_launch_ms": round(kms[dom], 4)},
         "roofline_residual": {"bytes_per_element_model": model, "achieved": round(res_gbs, 1),
                               "frac_of_peak": round(res_gbs / HBM_PEAK_GBS, 4),

@@ -185,6 +185,8 @@ typedef struct zfr_ele_desc {
   const unsigned int *nbr;         /* [E][nF]: partner (ele << 9 | slot << 8 | fpt)      */
   double *Utr;                     /* [E][V][nF] solution trace (scratch)                */
   double *Gtr;                     /* [E][D][V][nF] gradient trace (scratch, viscous)    */
+  double *Ftr;                     /* [E][V][nF] common normal flux at the flux points
+                                      whose side owns it (scratch, LDG beta = +-1/2)     */
 } zfr_ele_desc_t;
 
 /* Elements::extrapolate_U (src/elements.cpp:1337-1360): U[E][V][nS] -> Utr trace. */
@@ -198,6 +200,15 @@ int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void 
  * (src/elements.cpp:1449-1501, 1588-1699; src/faces.cpp:1244-1364, 1476-1676):
  * divF[E][V][nS] from U and the traces. */
 int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream);
+/* Navier-Stokes with LDG ldg_b = +-1/2, where one side's gradient enters each common
+ * flux: corrected gradient + compute_F + compute_divF_spts (volume part of divF) and the
+ * complete common normal flux (Rusanov + LDG, src/faces.cpp:1244-1614) at the flux
+ * points whose gradient this element owns, written unscaled to Ftr.  Replaces
+ * zfr_gradient + the volume half of zfr_divergence on this path (src/solver.cpp:1552-1610). */
+int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream);
+/* Elements::compute_divF_fpts (src/elements.cpp:1476-1501) on that path:
+ * divF += oppDiv_fpts Fcomm with Fcomm = owner's Ftr * signed dA of this side. */
+int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream);
 /* RK stage update (src/solver.cpp:2897-2941): U = U_ini - (alpha*dt/detJ) * divF */
 int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
                  double alpha, double dt, void *stream);

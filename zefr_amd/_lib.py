@@ -52,7 +52,7 @@ class EleDesc(C.Structure):
         ("gamma", C.c_double), ("prandtl", C.c_double), ("mu", C.c_double), ("rt", C.c_double),
         ("c_sth", C.c_double), ("fix_vis", C.c_int), ("rus_k", C.c_double), ("ldg_b", C.c_double),
         ("ldg_tau", C.c_double), ("ops1d", C.c_void_p), ("metric", C.c_void_p), ("fgeo", C.c_void_p),
-        ("nbr", C.c_void_p), ("Utr", C.c_void_p), ("Gtr", C.c_void_p),
+        ("nbr", C.c_void_p), ("Utr", C.c_void_p), ("Gtr", C.c_void_p), ("Ftr", C.c_void_p),
     ]
 
 
@@ -111,6 +111,8 @@ SIGNATURES = {
     "zfr_extrapolate": (_I, [C.POINTER(EleDesc), _P, _P]),
     "zfr_gradient": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_divergence": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
+    "zfr_gradient_flux": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
+    "zfr_face_correction": (_I, [C.POINTER(EleDesc), _P, _P]),
     "zfr_rk_stage": (_I, [C.POINTER(EleDesc), _P, _P, _P, C.c_double, C.c_double, _P]),
     "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_residual_norm": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P, _P]),

@@ -103,7 +103,7 @@ class Elements {
   std::vector<double> X;                             // element nodes [E][nNodes][nDims]
   std::vector<double> metric;                        // [E][D*D+1][nS]: adjugate(d1,d2), detJ
   // device data (element-major)
-  DevBuf U_spts_d, U_ini_d, divF_d, dU_spts_d, metric_d, fgeo_d, nbr_d, ops_d, Utr_d, Gtr_d;
+  DevBuf U_spts_d, U_ini_d, divF_d, dU_spts_d, metric_d, fgeo_d, nbr_d, ops_d, Utr_d, Gtr_d, Ftr_d;
   std::vector<double> U_host, dU_host;  // host mirrors for get_u_spts / get_du_spts
   int tile = 0;                         // elements per workgroup (0 = kernel default)
 
@@ -156,6 +156,7 @@ class FRSolver {
   void compute_gradient_export();  // fills dU_spts_d with the physical gradient of U
   void synchronize();
   int autotune(int reps);
+  bool onesided() const;  // viscous with LDG beta = +-1/2: one-sided gradient path
   void time_kernels(int reps, double *ms_out);
 
   zefr_input_t *input;

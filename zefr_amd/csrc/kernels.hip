@@ -63,6 +63,7 @@ struct KArgs {
   const unsigned int *nbr;
   double *Utr;
   double *Gtr;
+  double *Ftr;
 };
 
 // Flux point f of the reference ordering (src/quads.cpp:86-123, src/hexas.cpp:133-194):
@@ -109,38 +110,72 @@ __device__ __forceinline__ int fpt_end(int a, int b, int c, int dim, int side) {
 }
 
 // ---------------------------------------------------------------- pointwise physics
-// include/flux.hpp:11-63 (zero grid velocity)
+// The reference forms the full flux tensor F[var][dim] (include/flux.hpp:1-202) and then
+// contracts it with a vector: the face normal (src/faces.cpp:1313-1322, 1580-1590) or a
+// row of the adjugate metric (src/elements.cpp:1676-1687).  Here the contraction is done
+// analytically: Fn = F . n is assembled from primitive quantities without materialising
+// F, which keeps register pressure low.  Terms are the same; only FP64 rounding order
+// differs.
+
 template <int D>
-__device__ __forceinline__ void conv_flux(const double *U, double F[][D], double &P, double gamma) {
+struct Prim {
+  double invrho, P, H;
+  double vel[D];
+};
+
+// include/flux.hpp:11-63: invrho, pressure, total enthalpy, velocity
+template <int D>
+__device__ __forceinline__ Prim<D> primitives(const double *U, double gamma) {
   constexpr int V = D + 2;
-  const double invrho = 1. / U[0];
+  Prim<D> p;
+  p.invrho = 1. / U[0];
   double momF = U[1] * U[1];
 #pragma unroll
   for (int d = 1; d < D; d++) momF += U[d + 1] * U[d + 1];
-  momF *= invrho;
-  P = (gamma - 1.0) * (U[V - 1] - 0.5 * momF);
-  const double H = (U[V - 1] + P) * invrho;
+  momF *= p.invrho;
+  p.P = (gamma - 1.0) * (U[V - 1] - 0.5 * momF);
+  p.H = (U[V - 1] + p.P) * p.invrho;
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    F[0][d] = U[d + 1];
-#pragma unroll
-    for (int m = 0; m < D; m++) F[m + 1][d] = U[d + 1] * U[m + 1] * invrho;
-    F[d + 1][d] += P;
-    F[V - 1][d] = U[d + 1] * H;
-  }
+  for (int d = 0; d < D; d++) p.vel[d] = U[d + 1] * p.invrho;
+  return p;
 }
 
-// include/flux.hpp:75-202: F -= viscous flux(U, grad U)
+// Inviscid flux contracted with n: Fn[0] = m.n, Fn[1+k] = m_k (m.n)/rho + P n_k,
+// Fn[V-1] = H (m.n), with m the momentum.
 template <int D>
-__device__ __forceinline__ void visc_flux_sub(const double *U, const double dU[][D], double F[][D],
-                                              const KArgs &a) {
+__device__ __forceinline__ void conv_proj(const double *U, const Prim<D> &p, const double *n, double *Fn) {
   constexpr int V = D + 2;
-  const double invrho = 1.0 / U[0];
-  double vel[D];
-  double ke2 = 0.0;
+  double mn = U[1] * n[0];
 #pragma unroll
-  for (int d = 0; d < D; d++) { vel[d] = U[d + 1] * invrho; ke2 += vel[d] * vel[d]; }
-  const double e_int = U[V - 1] * invrho - 0.5 * ke2;
+  for (int d = 1; d < D; d++) mn += U[d + 1] * n[d];
+  Fn[0] = mn;
+  const double un = mn * p.invrho;
+#pragma unroll
+  for (int k = 0; k < D; k++) Fn[k + 1] = U[k + 1] * un + p.P * n[k];
+  Fn[V - 1] = p.H * mn;
+}
+
+// Viscous stress tau and heat + work flux q of include/flux.hpp:75-202:
+// F_visc[1+i][j] = -tau[i][j], F_visc[V-1][j] = -q[j].
+template <int D>
+struct Visc {
+  double tau[D][D];
+  double q[D];
+};
+
+// The gradient enters only through its derivative directions one at a time: direction j
+// needs dU[0..V-1][j] and yields dvel[:][j] (velocity gradient) and de[j] (internal energy
+// gradient).  fetch(j, dUj) supplies dU[.][j]; a compiler memory barrier after each
+// direction keeps the next direction's loads from being hoisted, so only one direction's
+// V values are live (the gradient is often gathered straight from LDS or global memory).
+template <int D, typename Fetch>
+__device__ __forceinline__ Visc<D> viscous_dir(const double *U, const Prim<D> &p, Fetch &&fetch, const KArgs &a) {
+  constexpr int V = D + 2;
+  Visc<D> s;
+  double ke2 = p.vel[0] * p.vel[0];
+#pragma unroll
+  for (int d = 1; d < D; d++) ke2 += p.vel[d] * p.vel[d];
+  const double e_int = U[V - 1] * p.invrho - 0.5 * ke2;
   double mu;
   if (a.fix_vis) {
     mu = a.mu;
@@ -148,82 +183,108 @@ __device__ __forceinline__ void visc_flux_sub(const double *U, const double dU[]
     const double rt_ratio = (a.gamma - 1.0) * e_int / a.rt;
     mu = a.mu * pow(rt_ratio, 1.5) * (1. + a.c_sth) / (rt_ratio + a.c_sth);
   }
-  double dvel[D][D];
+  double dvel[D][D], de[D];
+#pragma unroll
+  for (int j = 0; j < D; j++) {
+    double dUj[V];
+    fetch(j, dUj);
+#pragma unroll
+    for (int i = 0; i < D; i++) dvel[i][j] = (dUj[i + 1] - dUj[0] * p.vel[i]) * p.invrho;
+    double w = p.vel[0] * dvel[0][j];
+#pragma unroll
+    for (int i = 1; i < D; i++) w += p.vel[i] * dvel[i][j];
+    const double dke = 0.5 * ke2 * dUj[0] + U[0] * w;
+    de[j] = (dUj[V - 1] - dke - dUj[0] * e_int) * p.invrho;
+    asm volatile("" ::: "memory");
+  }
+  double div = dvel[0][0];
+#pragma unroll
+  for (int i = 1; i < D; i++) div += dvel[i][i];
+  const double diag = div / 3.0;
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
-    for (int j = 0; j < D; j++) dvel[i][j] = (dU[i + 1][j] - dU[0][j] * vel[i]) * invrho;
-  double de[D];
-#pragma unroll
-  for (int j = 0; j < D; j++) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < D; i++) s += vel[i] * dvel[i][j];
-    const double dke = 0.5 * ke2 * dU[0][j] + U[0] * s;
-    de[j] = (dU[V - 1][j] - dke - dU[0][j] * e_int) * invrho;
-  }
-  double div = 0.0;
-#pragma unroll
-  for (int i = 0; i < D; i++) div += dvel[i][i];
-  const double diag = div / 3.0;
+    for (int j = 0; j < D; j++)
+      s.tau[i][j] = (i == j) ? 2.0 * mu * (dvel[i][i] - diag) : mu * (dvel[i][j] + dvel[j][i]);
   const double kc = (mu / a.prandtl) * a.gamma;
 #pragma unroll
   for (int j = 0; j < D; j++) {
-    double w = 0.0;
+    double work = p.vel[0] * s.tau[0][j];
 #pragma unroll
-    for (int i = 0; i < D; i++) {
-      const double tau = (i == j) ? 2.0 * mu * (dvel[i][i] - diag) : mu * (dvel[i][j] + dvel[j][i]);
-      F[i + 1][j] -= tau;
-      w += vel[i] * tau;
-    }
-    F[V - 1][j] -= (w + kc * de[j]);
+    for (int i = 1; i < D; i++) work += p.vel[i] * s.tau[i][j];
+    s.q[j] = work + kc * de[j];
   }
+  return s;
 }
 
-// Common normal flux at one flux point for this element's side: Rusanov
-// (src/faces.cpp:1244-1364) plus, when viscous, LDG (src/faces.cpp:1476-1614); the
-// result is scaled by the signed area of this side (Fcomm(0) = F dA0, Fcomm(1) = -F dA1).
 template <int D>
-__device__ __forceinline__ void common_flux(const double *UL, const double *UR, const double dUL[][D],
-                                            const double dUR[][D], bool hasL, bool hasR,
-                                            const double *n, double dAs, const KArgs &a,
-                                            bool viscous, double *Fout) {
-  constexpr int V = D + 2;
-  double FL[V][D], FR[V][D], PL, PR;
-  conv_flux<D>(UL, FL, PL, a.gamma);
-  conv_flux<D>(UR, FR, PR, a.gamma);
-  const double aL = sqrt(a.gamma * PL / UL[0]);
-  const double aR = sqrt(a.gamma * PR / UR[0]);
-  double VnL = 0.0, VnR = 0.0;
+__device__ __forceinline__ Visc<D> viscous(const double *U, const Prim<D> &p, const double dU[][D], const KArgs &a) {
+  return viscous_dir<D>(U, p, [&](int j, double *dUj) {
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    VnL += UL[d + 1] / UL[0] * n[d];
-    VnR += UR[d + 1] / UR[0] * n[d];
+    for (int v = 0; v < D + 2; v++) dUj[v] = dU[v][j];
+  }, a);
+}
+
+// Fn -= F_visc . n
+template <int D>
+__device__ __forceinline__ void visc_proj_sub(const Visc<D> &s, const double *n, double *Fn) {
+  constexpr int V = D + 2;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    double t = s.tau[i][0] * n[0];
+#pragma unroll
+    for (int j = 1; j < D; j++) t += s.tau[i][j] * n[j];
+    Fn[i + 1] -= t;
   }
-  const double eig = fmax(fabs(VnL) + aL, fabs(VnR) + aR);
+  double qn = s.q[0] * n[0];
+#pragma unroll
+  for (int j = 1; j < D; j++) qn += s.q[j] * n[j];
+  Fn[V - 1] -= qn;
+}
+
+// LDG beta branches of src/faces.cpp:1520-1545, fixed at compile time: BM = 0 any beta
+// (both sides' gradients), 1 beta = +1/2 (left side only), 2 beta = -1/2 (right only).
+// Common normal flux for this element's side of a flux point: Rusanov
+// (src/faces.cpp:1244-1364) plus LDG when viscous (src/faces.cpp:1476-1614), scaled by
+// the signed area of the side (Fcomm(0) = F dA0, Fcomm(1) = -F dA1).  For BM = 1 / 2 the
+// one gradient used (left / right side's) is passed in dG1; dG2 is the right side's for
+// BM = 0.
+// Viscous flux of one side contracted with n (zero-initialised output, as the reference's
+// FL / FR in src/faces.cpp:1555-1560).
+template <int D, typename Fetch>
+__device__ __forceinline__ void visc_normal(const double *U, Fetch &&fetch, const double *n, const KArgs &a,
+                                            double *G) {
+  constexpr int V = D + 2;
+#pragma unroll
+  for (int v = 0; v < V; v++) G[v] = 0.0;
+  const Prim<D> p = primitives<D>(U, a.gamma);
+  visc_proj_sub<D>(viscous_dir<D>(U, p, fetch, a), n, G);
+}
+
+// G1 / G2: viscous normal fluxes already formed (G1 = left for BM 0 / 1, right for BM 2;
+// G2 = right for BM 0).
+template <int D, bool VISC, int BM>
+__device__ __forceinline__ void common_flux(const double *UL, const double *UR, const double *G1,
+                                            const double *G2, const double *n, double dAs, const KArgs &a,
+                                            double *Fout) {
+  constexpr int V = D + 2;
+  const Prim<D> pL = primitives<D>(UL, a.gamma), pR = primitives<D>(UR, a.gamma);
+  double FnL[V], FnR[V];
+  conv_proj<D>(UL, pL, n, FnL);
+  conv_proj<D>(UR, pR, n, FnR);
+  const double aL = sqrt(a.gamma * pL.P * pL.invrho);
+  const double aR = sqrt(a.gamma * pR.P * pR.invrho);
+  const double eig = fmax(fabs(FnL[0] * pL.invrho) + aL, fabs(FnR[0] * pR.invrho) + aR);
   const double diss = 0.5 * eig * (1.0 - a.rus_k);
 #pragma unroll
-  for (int v = 0; v < V; v++) {
-    double FnL = 0.0, FnR = 0.0;
-#pragma unroll
-    for (int d = 0; d < D; d++) { FnL += FL[v][d] * n[d]; FnR += FR[v][d] * n[d]; }
-    Fout[v] = (0.5 * (FnR + FnL) - diss * (UR[v] - UL[v])) * dAs;
-  }
-  if (viscous) {
-    double GL[V][D], GR[V][D];
-#pragma unroll
-    for (int v = 0; v < V; v++)
-#pragma unroll
-      for (int d = 0; d < D; d++) { GL[v][d] = 0.0; GR[v][d] = 0.0; }
-    if (hasL) visc_flux_sub<D>(UL, dUL, GL, a);
-    if (hasR) visc_flux_sub<D>(UR, dUR, GR, a);
+  for (int v = 0; v < V; v++) Fout[v] = (0.5 * (FnR[v] + FnL[v]) - diss * (UR[v] - UL[v])) * dAs;
+  if (VISC) {
     const double beta = a.ldg_b, tau = a.ldg_tau;
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      double FnL = 0.0, FnR = 0.0;
-#pragma unroll
-      for (int d = 0; d < D; d++) { FnL += GL[v][d] * n[d]; FnR += GR[v][d] * n[d]; }
-      const double Fc = 0.5 * (FnL + FnR) + tau * (UL[v] - UR[v]) + beta * (FnL - FnR);
+      const double GL = (BM == 2) ? 0.0 : G1[v];
+      const double GR = (BM == 0) ? G2[v] : ((BM == 2) ? G1[v] : 0.0);
+      const double Fc = 0.5 * (GL + GR) + tau * (UL[v] - UR[v]) + beta * (GL - GR);
       Fout[v] += Fc * dAs;
     }
   }
@@ -239,6 +300,29 @@ __device__ __forceinline__ void unpack_nbr(unsigned int p, long &ep, int &slot, 
 __device__ __forceinline__ bool need_left_grad(double beta) { return beta != -0.5; }
 __device__ __forceinline__ bool need_right_grad(double beta) { return beta != 0.5; }
 
+// Stage EB consecutive elements' U blocks and the 1D tables into LDS.
+template <int D, int N>
+__device__ __forceinline__ void stage_U(const double *__restrict__ U, const KArgs &a, double *lops, double *lU,
+                                        long e0, long neb, int Ustride) {
+  using S = Shape<D, N>;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
+  const double *Ub = U + e0 * S::V * S::nS;
+  for (long i = tid; i < neb * S::V * S::nS; i += NT) {
+    const long el = i / (S::V * S::nS), r = i - el * (S::V * S::nS);
+    lU[el * Ustride + r] = Ub[i];
+  }
+}
+
+// Value of var v along the line (base, str) of a [var][spt] block, extrapolated to one end.
+template <int N>
+__device__ __forceinline__ double line_extrap(const double *lE, const double *blk, int base, int str) {
+  double s = 0.0;
+#pragma unroll
+  for (int m = 0; m < N; m++) s += lE[m] * blk[base + m * str];
+  return s;
+}
+
 // ---------------------------------------------------------------- kernel 1: traces
 // Elements::extrapolate_U (src/elements.cpp:1337-1360): U_fpts = oppE U_spts.
 template <int D, int N>
@@ -248,14 +332,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__res
   extern __shared__ double lds[];
   double *lops = lds;
   double *lU = lds + S::NOPS_PAD;
-  const int tid = threadIdx.x, NT = a.EB * nS;
   const long e0 = (long)blockIdx.x * a.EB;
   const long neb = min((long)a.EB, (long)a.E - e0);
-  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const double *Ub = U + e0 * V * nS;
-  for (long i = tid; i < neb * V * nS; i += NT) lU[i] = Ub[i];
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
   __syncthreads();
-  const int el = tid / nS, t = tid - el * nS;
+  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   if (el >= neb) return;
   const double *u = lU + el * V * nS;
@@ -267,57 +348,36 @@ __global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__res
       fpt_info<D, N>(f, dim, side, base, str);
       const double *lE = tab_lE<N>(lops, side);
 #pragma unroll
-      for (int v = 0; v < V; v++) {
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < N; m++) s += lE[m] * u[v * nS + base + m * str];
-        a.Utr[(e * V + v) * nF + f] = s;
-      }
+      for (int v = 0; v < V; v++) a.Utr[(e * V + v) * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
     }
   }
 }
 
-// Shared by the gradient and divergence kernels: own + partner U at this thread's flux
-// points, and (viscous) the LDG common solution written to lC (src/faces.cpp:1366-1474).
+// LDG common solution at this thread's flux points (src/faces.cpp:1366-1474):
+// Ucomm = 1/2 (UL + UR) - beta (UL - UR), with the beta = +-1/2 one-sided branches.
 template <int D, int N>
-__device__ __forceinline__ void face_states(const KArgs &a, const double *lops, const double *u, double *lC,
-                                            long e, int t, bool valid, bool viscous,
-                                            double UL[][D + 2], double UR[][D + 2], int *slot) {
+__device__ __forceinline__ void common_solution(const KArgs &a, const double *lops, const double *u, double *lC,
+                                                long e, int t) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  const double beta = a.ldg_b;
 #pragma unroll
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
-    slot[k] = 0;
-    if (!valid || f >= nF) continue;
+    if (f >= nF) continue;
     int dim, side, base, str;
     fpt_info<D, N>(f, dim, side, base, str);
     const double *lE = tab_lE<N>(lops, side);
     long ep; int sl, fp;
     unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-    slot[k] = sl;
-    double own[V], par[V];
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; m++) s += lE[m] * u[v * nS + base + m * str];
-      own[v] = s;
-      par[v] = a.Utr[(ep * V + v) * nF + fp];
-    }
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      UL[k][v] = sl ? par[v] : own[v];
-      UR[k][v] = sl ? own[v] : par[v];
-    }
-    if (viscous) {
-      const double beta = a.ldg_b;
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        const double l = (beta == 0.5) ? 0.0 : UL[k][v];
-        const double r = (beta == -0.5) ? 0.0 : UR[k][v];
-        lC[v * nF + f] = 0.5 * (l + r) - beta * (l - r);
-      }
+      const double own = line_extrap<N>(lE, u + v * nS, base, str);
+      const double par = a.Utr[(ep * V + v) * nF + fp];
+      const double uL = sl ? par : own, uR = sl ? own : par;
+      const double l = (beta == 0.5) ? 0.0 : uL;
+      const double r = (beta == -0.5) ? 0.0 : uR;
+      lC[v * nF + f] = 0.5 * (l + r) - beta * (l - r);
     }
   }
 }
@@ -374,50 +434,47 @@ __device__ __forceinline__ void load_metric(const KArgs &a, long e, int t, int n
 }
 
 // ---------------------------------------------------------------- kernel 2: gradient
-// Common solution + corrected gradient + physical transform + gradient extrapolation.
+// Common solution + corrected gradient + physical transform + gradient extrapolation to
+// the flux points whose gradient the partner element needs.  LDS per element holds U and
+// Ucomm, then (aliased) the physical gradient.
 template <int D, int N>
 __global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restrict__ U, double *__restrict__ dU_out,
                                                           KArgs a) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  constexpr int W = (V * nS + V * nF > D * V * nS) ? V * nS + V * nF : D * V * nS;
   extern __shared__ double lds[];
-  const int EB = a.EB;
   double *lops = lds;
-  double *lU = lops + S::NOPS_PAD;
-  double *lC = lU + EB * V * nS;
-  double *lG = lC + EB * V * nF;
-  const int tid = threadIdx.x, NT = EB * nS;
-  const long e0 = (long)blockIdx.x * EB;
-  const long neb = min((long)EB, (long)a.E - e0);
-  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const double *Ub = U + e0 * V * nS;
-  for (long i = tid; i < neb * V * nS; i += NT) lU[i] = Ub[i];
+  double *lW = lds + S::NOPS_PAD;
+  const long e0 = (long)blockIdx.x * a.EB;
+  const long neb = min((long)a.EB, (long)a.E - e0);
+  stage_U<D, N>(U, a, lops, lW, e0, neb, W);
   __syncthreads();
-  const int el = tid / nS, t = tid - el * nS;
+  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   const bool valid = el < neb;
-  const double *u = lU + el * V * nS;
-  double *c = lC + el * V * nF;
-  double *g_l = lG + el * D * V * nS;
-  double UL[S::FPT][V], UR[S::FPT][V];
-  int slot[S::FPT];
-  face_states<D, N>(a, lops, u, c, e, t, valid, true, UL, UR, slot);
+  double *u = lW + el * W;
+  double *c = u + V * nS;
+  if (valid) common_solution<D, N>(a, lops, u, c, e, t);
   __syncthreads();
+  double g[V][D];
   if (valid) {
     double ij[D][D], detJ;
     load_metric<D>(a, e, t, nS, ij, detJ, true);
-    double g[V][D];
     spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
     if (dU_out) {
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
         for (int v = 0; v < V; v++) dU_out[((e * D + d) * V + v) * nS + t] = g[v][d];
     }
+  }
+  __syncthreads();  // U and Ucomm are dead: the region now holds the gradient
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int v = 0; v < V; v++) u[(d * V + v) * nS + t] = g[v][d];
   }
   __syncthreads();
   if (!valid) return;
@@ -426,8 +483,9 @@ __global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restri
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
     if (f >= nF) continue;
+    const int sl = (int)((a.nbr[e * nF + f] >> 8) & 1u);
     // the partner needs this side's gradient iff this side is L and dUL is used, or R and dUR is used
-    const bool write = slot[k] ? need_right_grad(beta) : need_left_grad(beta);
+    const bool write = sl ? need_right_grad(beta) : need_left_grad(beta);
     if (!write) continue;
     int dim, side, base, str;
     fpt_info<D, N>(f, dim, side, base, str);
@@ -435,18 +493,16 @@ __global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restri
 #pragma unroll
     for (int d = 0; d < D; d++)
 #pragma unroll
-      for (int v = 0; v < V; v++) {
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < N; m++) s += lE[m] * g_l[(d * V + v) * nS + base + m * str];
-        a.Gtr[((e * D + d) * V + v) * nF + f] = s;
-      }
+      for (int v = 0; v < V; v++)
+        a.Gtr[((e * D + d) * V + v) * nF + f] = line_extrap<N>(lE, u + (d * V + v) * nS, base, str);
   }
 }
 
 // ---------------------------------------------------------------- kernel 3: divergence
 // compute_F + compute_divF_spts + compute_common_F + compute_divF_fpts.
-template <int D, int N, bool VISC>
+// Phases (viscous): U -> LDS | Ucomm -> lC | gradient -> lG | common flux -> lC |
+// transformed flux -> lG | divergence.  Inviscid: U -> LDS | flux + common flux | divergence.
+template <int D, int N, bool VISC, int BM>
 __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__restrict__ U, double *__restrict__ divF,
                                                             KArgs a) {
   using S = Shape<D, N>;
@@ -457,122 +513,121 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
   double *lU = lops + S::NOPS_PAD;
   double *lC = lU + EB * V * nS;
   double *lG = lC + EB * V * nF;  // gradient (viscous), then the transformed flux
-  const int tid = threadIdx.x, NT = EB * nS;
   const long e0 = (long)blockIdx.x * EB;
   const long neb = min((long)EB, (long)a.E - e0);
-  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const double *Ub = U + e0 * V * nS;
-  for (long i = tid; i < neb * V * nS; i += NT) lU[i] = Ub[i];
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
   __syncthreads();
-  const int el = tid / nS, t = tid - el * nS;
+  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   const bool valid = el < neb;
   const double *u = lU + el * V * nS;
   double *c = lC + el * V * nF;
   double *g_l = lG + el * D * V * nS;
-  double UL[S::FPT][V], UR[S::FPT][V];
-  int slot[S::FPT];
-  face_states<D, N>(a, lops, u, c, e, t, valid, VISC, UL, UR, slot);
-  if (VISC) __syncthreads();
-
-  // pointwise flux at the spt, transformed to reference space (src/elements.cpp:1588-1699)
-  double tF[V][D];
-  if (valid) {
-    double ij[D][D], detJ;
-    load_metric<D>(a, e, t, nS, ij, detJ, VISC);
-    double Us[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
-    double F[V][D], P;
-    conv_flux<D>(Us, F, P, a.gamma);
-    if (VISC) {
-      double g[V][D];
+  if (VISC) {
+    if (valid) common_solution<D, N>(a, lops, u, c, e, t);
+    __syncthreads();
+    if (valid) {
+      double ij[D][D], detJ, g[V][D];
+      load_metric<D>(a, e, t, nS, ij, detJ, true);
       spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
         for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
-      visc_flux_sub<D>(Us, g, F, a);
     }
-#pragma unroll
-    for (int v = 0; v < V; v++)
-#pragma unroll
-      for (int d1 = 0; d1 < D; d1++) {
-        double s = 0.0;
-#pragma unroll
-        for (int d2 = 0; d2 < D; d2++) s += F[v][d2] * ij[d1][d2];
-        tF[v][d1] = s;
-      }
-    if (!VISC) {
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[v][d];
-    }
+    __syncthreads();
   }
-  if (VISC) __syncthreads();
 
-  // common flux at this thread's flux points -> lC (Ucomm is no longer needed)
+  // common flux at this thread's flux points -> lC (Ucomm has been consumed)
   if (valid) {
-    const double beta = a.ldg_b;
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k < S::FPT; k++) {
       const int f = t + k * nS;
       if (f >= nF) continue;
+      int dim, side, base, str;
+      fpt_info<D, N>(f, dim, side, base, str);
+      const double *lE = tab_lE<N>(lops, side);
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
       const double *fg = a.fgeo + e * (D + 1) * nF + f;
       double n[D];
 #pragma unroll
       for (int d = 0; d < D; d++) n[d] = fg[d * nF];
       const double dAs = fg[D * nF];
-      double dUL[V][D], dUR[V][D];
-      bool hasL = false, hasR = false;
+      // own side's state / gradient from LDS, the partner's from its traces
+      auto own_state = [&](double *Us) {
+#pragma unroll
+        for (int v = 0; v < V; v++) Us[v] = line_extrap<N>(lE, u + v * nS, base, str);
+      };
+      auto par_state = [&](double *Us) {
+#pragma unroll
+        for (int v = 0; v < V; v++) Us[v] = a.Utr[(ep * V + v) * nF + fp];
+      };
+      // viscous normal fluxes first (one side's state + gradient live at a time):
+      // G1 = left (BM 0, 1) or right (BM 2) side, G2 = right side (BM 0)
+      double G1[VISC ? V : 1], G2[(VISC && BM == 0) ? V : 1];
       if (VISC) {
-        int dim, side, base, str;
-        fpt_info<D, N>(f, dim, side, base, str);
-        const double *lE = tab_lE<N>(lops, side);
-        long ep; int sl, fp;
-        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-        hasL = need_left_grad(beta);
-        hasR = need_right_grad(beta);
-        const bool ownNeed = sl ? hasR : hasL;
-        const bool parNeed = sl ? hasL : hasR;
-        double og[V][D], pg[V][D];
+        auto side_visc = [&](bool own, double *G) {
+          double Us[V];
+          if (own) {
+            own_state(Us);
+            visc_normal<D>(Us, [&](int j, double *dUj) {
 #pragma unroll
-        for (int d = 0; d < D; d++)
+              for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+            }, n, a, G);
+          } else {
+            par_state(Us);
+            visc_normal<D>(Us, [&](int j, double *dUj) {
 #pragma unroll
-          for (int v = 0; v < V; v++) {
-            double s = 0.0;
-            if (ownNeed) {
-#pragma unroll
-              for (int m = 0; m < N; m++) s += lE[m] * g_l[(d * V + v) * nS + base + m * str];
-            }
-            og[v][d] = s;
-            pg[v][d] = parNeed ? a.Gtr[((ep * D + d) * V + v) * nF + fp] : 0.0;
+              for (int v = 0; v < V; v++) dUj[v] = a.Gtr[((ep * D + j) * V + v) * nF + fp];
+            }, n, a, G);
           }
-#pragma unroll
-        for (int v = 0; v < V; v++)
-#pragma unroll
-          for (int d = 0; d < D; d++) {
-            dUL[v][d] = sl ? pg[v][d] : og[v][d];
-            dUR[v][d] = sl ? og[v][d] : pg[v][d];
-          }
+        };
+        if (BM == 0) {
+          side_visc(sl == 0, G1);
+          side_visc(sl == 1, G2);
+        } else {
+          side_visc((BM == 1) ? (sl == 0) : (sl == 1), G1);
+        }
       }
+      double UL[V], UR[V];
+      if (sl) { par_state(UL); own_state(UR); } else { own_state(UL); par_state(UR); }
       double Fo[V];
-      common_flux<D>(UL[k], UR[k], dUL, dUR, hasL, hasR, n, dAs, a, VISC, Fo);
+      common_flux<D, VISC, BM>(UL, UR, G1, G2, n, dAs, a, Fo);
 #pragma unroll
       for (int v = 0; v < V; v++) c[v * nF + f] = Fo[v];
     }
   }
-  __syncthreads();
-  if (VISC) {
-    if (valid) {
+  if (VISC) __syncthreads();  // the gradient field in lG is consumed
+
+  // pointwise flux at the spt, contracted with the adjugate rows (src/elements.cpp:1588-1699);
+  // each thread reads and overwrites only its own spt column of lG
+  if (valid) {
+    double ij[D][D], detJ;
+    load_metric<D>(a, e, t, nS, ij, detJ, false);
+    double Us[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
+    const Prim<D> p = primitives<D>(Us, a.gamma);
+    double tF[D][V];
+#pragma unroll
+    for (int d1 = 0; d1 < D; d1++) conv_proj<D>(Us, p, ij[d1], tF[d1]);
+    if (VISC) {
+      double g[V][D];
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[v][d];
+        for (int v = 0; v < V; v++) g[v][d] = g_l[(d * V + v) * nS + t];
+      const Visc<D> vs = viscous<D>(Us, p, g, a);
+#pragma unroll
+      for (int d1 = 0; d1 < D; d1++) visc_proj_sub<D>(vs, ij[d1], tF[d1]);
     }
-    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[d][v];
   }
+  __syncthreads();
   if (!valid) return;
 
   // divergence: oppDiv tF + oppDiv_fpts Fcomm (src/elements.cpp:1449-1501)
@@ -594,6 +649,208 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
       acc[v] = s;
     }
   }
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+    const double *Df = tab_Dsf<N>(lops, id);
+    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      acc[v] += (-Df[0]) * c[v * nF + f0];
+      acc[v] += Df[1] * c[v * nF + f1];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+}
+
+// ---------------------------------------------------------------- one-sided LDG path
+// With LDG beta = +1/2 (BM 1) the common viscous flux uses only the left (slot-0)
+// side's gradient, with beta = -1/2 (BM 2) only the right side's (src/faces.cpp:1520-1545),
+// and the common solution is the other side's state.  The element on that "owner" side
+// therefore has everything for the complete common flux (Rusanov + LDG) right after its
+// own corrected gradient: it writes the unscaled common normal flux to Ftr at its owned
+// flux points, and no gradient trace is needed.  A residual is then
+//   k_extrapolate -> k_visc_onesided (gradient, volume flux divergence, owned common
+//   fluxes) -> k_face_correction (oppDiv_fpts Fcomm, both sides).
+
+template <int D, int N, int BM>
+__global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__restrict__ U,
+                                                               double *__restrict__ divF, KArgs a) {
+  using S = Shape<D, N>;
+  constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  constexpr int owner = (BM == 1) ? 0 : 1;  // slot whose gradient enters the common flux
+  extern __shared__ double lds[];
+  const int EB = a.EB;
+  double *lops = lds;
+  double *lU = lops + S::NOPS_PAD;
+  double *lC = lU + EB * V * nS;
+  double *lG = lC + EB * V * nF;  // gradient, then the transformed flux
+  int *lcnt = reinterpret_cast<int *>(lG + EB * D * V * nS);
+  int *llist = lcnt + EB;
+  const long e0 = (long)blockIdx.x * EB;
+  const long neb = min((long)EB, (long)a.E - e0);
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  if (threadIdx.x < EB) lcnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
+  const long e = e0 + el;
+  const bool valid = el < neb;
+  const double *u = lU + el * V * nS;
+  double *c = lC + el * V * nF;
+  double *g_l = lG + el * D * V * nS;
+  int *list = llist + el * nF;
+  // common solution (= the non-owner side's state) and the list of owned flux points
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++) {
+      const int f = t + k * nS;
+      if (f >= nF) continue;
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      if (sl == owner) {
+#pragma unroll
+        for (int v = 0; v < V; v++) c[v * nF + f] = a.Utr[(ep * V + v) * nF + fp];
+        list[atomicAdd(&lcnt[el], 1)] = f;
+      } else {
+        int dim, side, base, str;
+        fpt_info<D, N>(f, dim, side, base, str);
+        const double *lE = tab_lE<N>(lops, side);
+#pragma unroll
+        for (int v = 0; v < V; v++) c[v * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
+      }
+    }
+  }
+  __syncthreads();
+  if (valid) {
+    double ij[D][D], detJ, g[V][D];
+    load_metric<D>(a, e, t, nS, ij, detJ, true);
+    spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
+  }
+  __syncthreads();
+  // common normal flux (unscaled) at the owned flux points -> Ftr
+  if (valid) {
+    const int cnt = lcnt[el];
+#pragma unroll 1
+    for (int k = t; k < cnt; k += nS) {
+      const int f = list[k];
+      int dim, side, base, str;
+      fpt_info<D, N>(f, dim, side, base, str);
+      const double *lE = tab_lE<N>(lops, side);
+      const double *fg = a.fgeo + e * (D + 1) * nF + f;
+      double n[D];
+#pragma unroll
+      for (int d = 0; d < D; d++) n[d] = fg[d * nF];
+      double Uo[V], Up[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
+        Up[v] = c[v * nF + f];  // the partner's state, staged as the common solution
+      }
+      double G[V];
+      visc_normal<D>(Uo, [&](int j, double *dUj) {
+#pragma unroll
+        for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+      }, n, a, G);
+      double Fo[V];
+      if (owner == 0) common_flux<D, true, BM>(Uo, Up, G, G, n, 1.0, a, Fo);
+      else common_flux<D, true, BM>(Up, Uo, G, G, n, 1.0, a, Fo);
+#pragma unroll
+      for (int v = 0; v < V; v++) a.Ftr[(e * V + v) * nF + f] = Fo[v];
+    }
+  }
+  __syncthreads();  // the gradient field is consumed
+  // pointwise flux contracted with the adjugate rows, over the own spt column of lG
+  if (valid) {
+    double ij[D][D], detJ;
+    load_metric<D>(a, e, t, nS, ij, detJ, false);
+    double Us[V], g[V][D];
+#pragma unroll
+    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int v = 0; v < V; v++) g[v][d] = g_l[(d * V + v) * nS + t];
+    const Prim<D> p = primitives<D>(Us, a.gamma);
+    const Visc<D> vs = viscous<D>(Us, p, g, a);
+    double tF[D][V];
+#pragma unroll
+    for (int d1 = 0; d1 < D; d1++) {
+      conv_proj<D>(Us, p, ij[d1], tF[d1]);
+      visc_proj_sub<D>(vs, ij[d1], tF[d1]);
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[d][v];
+  }
+  __syncthreads();
+  if (!valid) return;
+  // volume part of the divergence: oppDiv tF (src/elements.cpp:1449-1474)
+  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
+  double acc[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) acc[v] = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+    const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
+    const int base = t - id * str;
+    const double *Dr = tab_Dss<N>(lops, id);
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double s = acc[v];
+#pragma unroll
+      for (int m = 0; m < N; m++) s += Dr[m] * g_l[(d * V + v) * nS + base + m * str];
+      acc[v] = s;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+}
+
+// divF += oppDiv_fpts Fcomm (src/elements.cpp:1476-1501) with Fcomm = Ftr(owner) * signed
+// dA of this side; the owner's Ftr is this element's own or its partner's.
+template <int D, int N, int BM>
+__global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restrict__ divF, KArgs a) {
+  using S = Shape<D, N>;
+  constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  constexpr int owner = (BM == 1) ? 0 : 1;
+  extern __shared__ double lds[];
+  const int EB = a.EB;
+  double *lops = lds;
+  double *lF = lops + S::NOPS_PAD;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
+  const long e0 = (long)blockIdx.x * EB;
+  const long neb = min((long)EB, (long)a.E - e0);
+  const int el = tid / nS, t = tid - el * nS;
+  const long e = e0 + el;
+  const bool valid = el < neb;
+  double *c = lF + el * V * nF;
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++) {
+      const int f = t + k * nS;
+      if (f >= nF) continue;
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      const double dAs = a.fgeo[(e * (D + 1) + D) * nF + f];
+      const double *src = (sl == owner) ? a.Ftr + e * V * nF + f : a.Ftr + ep * V * nF + fp;
+#pragma unroll
+      for (int v = 0; v < V; v++) c[v * nF + f] = src[v * nF] * dAs;
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
+  double acc[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) acc[v] = divF[(e * V + v) * nS + t];
 #pragma unroll
   for (int d = 0; d < D; d++) {
     const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
@@ -729,10 +986,11 @@ inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
   a.gamma = d->gamma; a.prandtl = d->prandtl; a.mu = d->mu; a.rt = d->rt; a.c_sth = d->c_sth;
   a.rus_k = d->rus_k; a.ldg_b = d->ldg_b; a.ldg_tau = d->ldg_tau;
   a.ops = d->ops1d; a.metric = d->metric; a.fgeo = d->fgeo; a.nbr = d->nbr; a.Utr = d->Utr; a.Gtr = d->Gtr;
+  a.Ftr = d->Ftr;
   return a;
 }
 
-enum class Op { Extrap, Grad, Div };
+enum class Op { Extrap, Grad, Div, OneSided, FaceCorr };
 
 template <int D, int N>
 int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st) {
@@ -740,8 +998,18 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   if (d->nSpts != S::nS || d->nFpts != S::nF || d->nVars != S::V)
     return zfr::fail("kernel launch: element descriptor does not match (nDims, order)");
   if (d->nEles <= 0) return 0;
-  const size_t per_ele = (op == Op::Extrap) ? (size_t)S::V * S::nS
-                                            : (size_t)(S::V * S::nS + S::V * S::nF + D * S::V * S::nS);
+  constexpr size_t kUC = (size_t)S::V * S::nS + S::V * S::nF, kG = (size_t)D * S::V * S::nS;
+  // LDS doubles per element (the owned-point list of the one-sided kernel is nF + 1 ints)
+  size_t per_ele = 0;
+  switch (op) {
+    case Op::Extrap: per_ele = (size_t)S::V * S::nS; break;
+    case Op::Grad: per_ele = kUC > kG ? kUC : kG; break;
+    case Op::Div: per_ele = kUC + kG; break;
+    case Op::OneSided: per_ele = kUC + kG + (S::nF + 2) / 2; break;
+    case Op::FaceCorr: per_ele = (size_t)S::V * S::nF; break;
+  }
+  if ((op == Op::OneSided || op == Op::FaceCorr) && d->ldg_b != 0.5 && d->ldg_b != -0.5)
+    return zfr::fail("one-sided LDG kernels need ldg_b = +-0.5");
   const int EB = eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele);
   const KArgs a = make_args(d, EB);
   const dim3 grid((unsigned)((d->nEles + EB - 1) / EB)), block((unsigned)(EB * S::nS));
@@ -754,8 +1022,18 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       hipLaunchKernelGGL((k_gradient<D, N>), grid, block, lds, st, U, out, a);
       break;
     case Op::Div:
-      if (d->viscous) hipLaunchKernelGGL((k_divergence<D, N, true>), grid, block, lds, st, U, out, a);
-      else hipLaunchKernelGGL((k_divergence<D, N, false>), grid, block, lds, st, U, out, a);
+      if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0>), grid, block, lds, st, U, out, a);
+      else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 1>), grid, block, lds, st, U, out, a);
+      else if (d->ldg_b == -0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 2>), grid, block, lds, st, U, out, a);
+      else hipLaunchKernelGGL((k_divergence<D, N, true, 0>), grid, block, lds, st, U, out, a);
+      break;
+    case Op::OneSided:
+      if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_visc_onesided<D, N, 1>), grid, block, lds, st, U, out, a);
+      else hipLaunchKernelGGL((k_visc_onesided<D, N, 2>), grid, block, lds, st, U, out, a);
+      break;
+    case Op::FaceCorr:
+      if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_face_correction<D, N, 1>), grid, block, lds, st, out, a);
+      else hipLaunchKernelGGL((k_face_correction<D, N, 2>), grid, block, lds, st, out, a);
       break;
   }
   return zfr::check_hip(hipGetLastError(), "kernel launch");
@@ -806,6 +1084,16 @@ int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void 
 
 int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
   return zk::dispatch(zk::Op::Div, d, U, divF, (hipStream_t)stream);
+}
+
+int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
+  if (!d->viscous) return zfr::fail("zfr_gradient_flux: descriptor is inviscid");
+  return zk::dispatch(zk::Op::OneSided, d, U, divF, (hipStream_t)stream);
+}
+
+int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream) {
+  if (!d->viscous) return zfr::fail("zfr_face_correction: descriptor is inviscid");
+  return zk::dispatch(zk::Op::FaceCorr, d, nullptr, divF, (hipStream_t)stream);
 }
 
 int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF, double alpha,

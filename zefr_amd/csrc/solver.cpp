@@ -222,6 +222,7 @@ zfr_ele_desc_t Elements::desc(const zefr_input_t &in) const {
   d.nbr = nbr_d.as<unsigned>();
   d.Utr = Utr_d.as<double>();
   d.Gtr = Gtr_d.as<double>();
+  d.Ftr = Ftr_d.as<double>();
   return d;
 }
 
@@ -343,7 +344,8 @@ void FRSolver::setup_device() {
   el.divF_d.alloc((size_t)input->nStages * st * sizeof(double));
   hip_or_throw(hipMemset(el.divF_d.p, 0, el.divF_d.bytes), "hipMemset");
   el.Utr_d.alloc((size_t)E * V * nF * sizeof(double));
-  if (input->viscous) el.Gtr_d.alloc((size_t)E * D * V * nF * sizeof(double));
+  if (onesided()) el.Ftr_d.alloc((size_t)E * V * nF * sizeof(double));
+  else if (input->viscous) el.Gtr_d.alloc((size_t)E * D * V * nF * sizeof(double));
   norm_work_d.alloc(1024 * 8 * sizeof(double));
   norm_out_d.alloc(8 * sizeof(double));
   device_ready = true;
@@ -356,8 +358,16 @@ static void need_device(const FRSolver &s) {
   if (!s.device_ready) raise("solver is not set up on the device (call setup_solver)");
 }
 
-// compute_residual (src/solver.cpp:1494-1645): start = traces, mid = gradient, finish =
-// divergence of the common flux.
+bool FRSolver::onesided() const {
+  return input->viscous && (input->ldg_b == 0.5 || input->ldg_b == -0.5);
+}
+
+// compute_residual (src/solver.cpp:1494-1645) and its split (:1647-1840):
+//   start  = U traces (extrapolate_U);
+//   mid    = one-sided LDG: gradient, volume divergence and owned common fluxes;
+//            general LDG beta: gradient and its traces;
+//   finish = one-sided LDG: face correction of divF;
+//            otherwise: common flux + full divergence.
 void FRSolver::compute_residual_start(unsigned, int color) {
   need_device(*this);
   if (color > -1) raise("multi-colour (MCGS) residuals are not supported");
@@ -365,17 +375,24 @@ void FRSolver::compute_residual_start(unsigned, int color) {
   status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
 }
 
-void FRSolver::compute_residual_mid(unsigned, int) {
+void FRSolver::compute_residual_mid(unsigned stage, int) {
   if (!input->viscous) return;
+  if (stage >= input->nStages) raise("stage out of range");
   const auto d = eles->desc(*input);
-  status_or_throw(zfr_gradient(&d, eles->U_spts_d.as<double>(), nullptr, stream));
+  if (onesided()) {
+    double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
+    status_or_throw(zfr_gradient_flux(&d, eles->U_spts_d.as<double>(), divF, stream));
+  } else {
+    status_or_throw(zfr_gradient(&d, eles->U_spts_d.as<double>(), nullptr, stream));
+  }
 }
 
 void FRSolver::compute_residual_finish(unsigned stage, int) {
   if (stage >= input->nStages) raise("stage out of range");
   const auto d = eles->desc(*input);
   double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
-  status_or_throw(zfr_divergence(&d, eles->U_spts_d.as<double>(), divF, stream));
+  if (onesided()) status_or_throw(zfr_face_correction(&d, divF, stream));
+  else status_or_throw(zfr_divergence(&d, eles->U_spts_d.as<double>(), divF, stream));
 }
 
 void FRSolver::compute_residual(unsigned stage, int color) {
@@ -390,6 +407,7 @@ void FRSolver::compute_gradient_export() {
   if (!input->viscous) raise("the gradient is only formed for viscous runs");
   Elements &el = *eles;
   if (!el.dU_spts_d.p) el.dU_spts_d.alloc((size_t)el.nDims * el.state_size() * sizeof(double));
+  if (!el.Gtr_d.p) el.Gtr_d.alloc((size_t)el.nEles * el.nDims * el.nVars * el.nFpts * sizeof(double));
   const auto d = el.desc(*input);
   status_or_throw(zfr_extrapolate(&d, el.U_spts_d.as<double>(), stream));
   status_or_throw(zfr_gradient(&d, el.U_spts_d.as<double>(), el.dU_spts_d.as<double>(), stream));
@@ -524,26 +542,31 @@ void FRSolver::time_kernels(int reps, double *ms_out) {
   need_device(*this);
   Elements &el = *eles;
   const auto d = el.desc(*input);
-  hipEvent_t ev[4];
+  reps = std::max(reps, 1);
+  // one event before each launch and one after the last: residuals run back to back
+  std::vector<hipEvent_t> ev(3 * reps + 1);
   for (auto &e : ev) hip_or_throw(hipEventCreate(&e), "hipEventCreate");
-  double acc[3] = {0, 0, 0};
+  double *U = el.U_spts_d.as<double>(), *divF = el.divF_d.as<double>();
   for (int r = 0; r < reps; r++) {
-    hip_or_throw(hipEventRecord(ev[0], stream), "hipEventRecord");
-    status_or_throw(zfr_extrapolate(&d, el.U_spts_d.as<double>(), stream));
-    hip_or_throw(hipEventRecord(ev[1], stream), "hipEventRecord");
-    if (input->viscous) status_or_throw(zfr_gradient(&d, el.U_spts_d.as<double>(), nullptr, stream));
-    hip_or_throw(hipEventRecord(ev[2], stream), "hipEventRecord");
-    status_or_throw(zfr_divergence(&d, el.U_spts_d.as<double>(), el.divF_d.as<double>(), stream));
-    hip_or_throw(hipEventRecord(ev[3], stream), "hipEventRecord");
-    hip_or_throw(hipEventSynchronize(ev[3]), "hipEventSynchronize");
-    for (int k = 0; k < 3; k++) {
-      float ms = 0.f;
-      hip_or_throw(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "hipEventElapsedTime");
-      acc[k] += ms;
-    }
+    hip_or_throw(hipEventRecord(ev[3 * r], stream), "hipEventRecord");
+    status_or_throw(zfr_extrapolate(&d, U, stream));
+    hip_or_throw(hipEventRecord(ev[3 * r + 1], stream), "hipEventRecord");
+    if (onesided()) status_or_throw(zfr_gradient_flux(&d, U, divF, stream));
+    else if (input->viscous) status_or_throw(zfr_gradient(&d, U, nullptr, stream));
+    hip_or_throw(hipEventRecord(ev[3 * r + 2], stream), "hipEventRecord");
+    if (onesided()) status_or_throw(zfr_face_correction(&d, divF, stream));
+    else status_or_throw(zfr_divergence(&d, U, divF, stream));
+  }
+  hip_or_throw(hipEventRecord(ev[3 * reps], stream), "hipEventRecord");
+  hip_or_throw(hipEventSynchronize(ev[3 * reps]), "hipEventSynchronize");
+  double acc[3] = {0, 0, 0};
+  for (int i = 0; i < 3 * reps; i++) {
+    float ms = 0.f;
+    hip_or_throw(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "hipEventElapsedTime");
+    acc[i % 3] += ms;
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
-  for (int k = 0; k < 3; k++) ms_out[k] = acc[k] / std::max(reps, 1);
+  for (int k = 0; k < 3; k++) ms_out[k] = acc[k] / reps;
 }
 
 // ---------------------------------------------------------------- PMGrid
