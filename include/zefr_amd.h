@@ -139,7 +139,9 @@ int zefr_copy_state(zefr_t *z, int which, int stage, double *host, long n, int t
 int zefr_get_fpt_maps(zefr_t *z, int *fpt2gfpt, signed char *slot);
 /* Host arrays for inspection, in the reference layouts: "oppE", "oppD", "oppD_fpts",
  * "oppDiv", "oppDiv_fpts" (dense, from the 1D tables the kernels use), "coord_spts",
- * "jaco_det_spts", "inv_jaco_spts", "norm", "dA", "weights_spts". Returns the count. */
+ * "coord_fpts", "jaco_det_spts", "inv_jaco_spts", "inv_jaco_fpts", "norm", "dA",
+ * "weights_spts", "ops1d", "U_init".  Returns the count; with out == NULL only the
+ * count is returned. */
 long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n);
 /* Pick the elements-per-workgroup tile for this solver's kernels by timing the
  * candidates on the device (0 = keep the built-in default). Returns the chosen value. */

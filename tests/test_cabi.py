@@ -1,0 +1,89 @@
+"""The C ABI library: it loads without a GPU and exports every function that
+include/zefr_amd.h declares, and the ctypes table covers exactly that set.  No compute
+entry point is called here (that needs a HIP device)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from zefr_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "zefr_amd.h")
+
+
+def declared_functions():
+    with open(HEADER) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", " ", txt, flags=re.S)
+    txt = re.sub(r"//[^\n]*", " ", txt)
+    names = re.findall(r"\b((?:zefr|zfr)_[A-Za-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ["zefr_compute_residual", "zefr_compute_residual_start", "zefr_compute_residual_mid",
+                 "zefr_compute_residual_finish", "zefr_do_step", "zefr_do_rk_stage", "zefr_write_residual",
+                 "zefr_get_q_spts_d", "zefr_get_dq_spts_d", "zfr_extrapolate", "zfr_gradient_flux",
+                 "zfr_face_correction", "zfr_rk_stage", "zfr_residual_norm", "zefr_last_error"]:
+        assert must in names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = _lib.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header():
+    assert set(_lib.SIGNATURES) == set(declared_functions())
+
+
+def test_exported_dynamic_symbols_are_c_names():
+    """nm -D lists the declared names unmangled (extern "C")."""
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    syms = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    for n in declared_functions():
+        assert n in syms, n
+
+
+def test_struct_sizes_match_header_layout():
+    """zefr_input_t / zfr_ele_desc_t as seen by ctypes vs. a C compile of the header."""
+    code =('#include "zefr_amd.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+            'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(zefr_input_t), sizeof(zfr_ele_desc_t),'
+            ' offsetof(zefr_input_t, meshBoundBC), offsetof(zfr_ele_desc_t, Ftr)); return 0;}\n')
+    import tempfile
+    d = tempfile.mkdtemp(prefix="zefr_sizes_")
+    c = os.path.join(d, "sizes.c")
+    exe = os.path.join(d, "sizes")
+    with open(c, "w") as f:
+        f.write(code)
+    r = subprocess.run(["gcc", "-std=c11", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+                        "-I", "/opt/rocm/include", c, "-o", exe], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("the header needs hip_runtime_api.h from /opt/rocm/include: " + r.stderr[-200:])
+    got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [C.sizeof(_lib.ZefrInput), C.sizeof(_lib.EleDesc),
+                   _lib.ZefrInput.meshBoundBC.offset, _lib.EleDesc.Ftr.offset]
+
+
+def test_errors_without_a_device_are_reported_not_raised_through_c():
+    """Host-side validation fails with a status and a message, not a crash."""
+    L = _lib.lib()
+    z = L.zefr_create()
+    assert z
+    try:
+        rc = L.zefr_read_input(z, b"/nonexistent/deck")
+        assert rc != 0
+        assert _lib.last_error()
+        assert L.zefr_compute_residual(z, 0) != 0  # no solver set up yet
+        assert _lib.last_error()
+    finally:
+        L.zefr_destroy(z)

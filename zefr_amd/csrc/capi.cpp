@@ -331,6 +331,7 @@ long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n) {
     else if (nm == "ops1d") src = &e.re.ops1d;
     else if (nm == "U_init") src = &e.U_host;
     else zfr::raise("unknown host array " + nm);
+    if (!out) return (long)src->size();
     if ((long)src->size() > n) zfr::raise("get_host_array: buffer too small");
     std::memcpy(out, src->data(), src->size() * sizeof(double));
     return (long)src->size();

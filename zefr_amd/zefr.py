@@ -244,16 +244,13 @@ class Zefr:
         return g, s
 
     def host_array(self, name):
-        n = 1 << 26
-        while True:
-            buf = np.zeros(n)
-            cnt = lib().zefr_get_host_array(self._h, name.encode(), buf.ctypes.data_as(C.POINTER(C.c_double)), n)
-            if cnt >= 0:
-                return buf[:cnt].copy()
-            msg = last_error()
-            if "too small" not in msg:
-                raise RuntimeError(msg)
-            n *= 4
+        n = lib().zefr_get_host_array(self._h, name.encode(), None, 0)
+        if n < 0:
+            raise RuntimeError(last_error())
+        buf = np.zeros(n)
+        if lib().zefr_get_host_array(self._h, name.encode(), buf.ctypes.data_as(C.POINTER(C.c_double)), n) != n:
+            raise RuntimeError(last_error())
+        return buf
 
 
 # ---------------------------------------------------------------- module-level (zefr namespace)
