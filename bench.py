@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--tile", type=int, default=-1, help="elements per workgroup (-1: autotune)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
-    p.add_argument("--cpu-steps", type=int, default=20)
+    p.add_argument("--cpu-steps", type=int, default=150)
     p.add_argument("--workdir", default=None)
     return p.parse_args()
 
@@ -118,6 +118,22 @@ def residual_model_bytes(D, N, viscous):
         + (nF // 2) * (D + 2) * 8 + nF * 4
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "residual_kernels_tgv_hex64_p3_hbm_traffic.md")
+
+
+def profiled_traffic(kernel, n, order):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (copy-kernel calibrated), which covers the default workload (64^3, p=3) only."""
+    if (n, order) != (64, 3) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
 def stream_copy_gbs(L, nbytes=2 << 30, reps=20):
     """Device copy bandwidth (read + write bytes / time) with the library's copy kernel."""
     import torch
@@ -166,8 +182,8 @@ def cpu_baseline(workdir, n, order, steps):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     return {"value": dof / dt / 1e9, "unit": "Gdof-residuals/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle (oracle/zefr_oracle.cpp, OpenMP) on the same TGV case at {n}^3 hexes "
-                      f"p={order} ({dof / 1e6:.2f}M DoF), median-free mean of {steps} residuals "
-                      f"({dt * 1e3:.1f} ms each)", "ms_per_residual": dt * 1e3}
+                      f"p={order} ({dof / 1e6:.2f}M DoF), mean of {steps} back-to-back residuals "
+                      f"({dt * 1e3:.1f} ms each, {dt * steps:.1f} s total)", "ms_per_residual": dt * 1e3}
 
 
 def main():
@@ -245,7 +261,8 @@ def main():
                    "Pr": 0.71, "flux": "Rusanov+LDG(beta=0.5,tau=0.1)", "tile_eles_per_block": tile,
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(KERNEL_NAMES[dom].format(N=N), n, order),
                      "kernel": KERNEL_NAMES[dom].format(N=N),
                      "alg_bytes_per_launch": kb[dom] * E, "avg_launch_ms": round(kms[dom], 4)},
         "roofline_residual": {"bytes_per_element_model": model, "achieved": round(res_gbs, 1),

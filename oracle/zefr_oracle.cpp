@@ -27,6 +27,18 @@
 // mesh, periodic or fluid physical groups only (any other boundary raises), Euler /
 // Navier-Stokes with Rusanov + LDG, RK44 or forward Euler with a fixed dt.
 //
+// What pins each part (tests/test_oracle.py): operators against the reference's baked
+// GiMMiK hex p=3/p=4 constants and polynomial exactness; Gauss-Legendre rules against
+// numpy; metrics against affine closed forms and cell volumes; connectivity by brute
+// force (each global flux point has one slot-0 and one slot-1 point at the same
+// physical location); fluxes against textbook Euler/NS from analytic gradients;
+// residuals by freestream preservation and discrete conservation; RK44 and the norms
+// by their textbook definitions.
+// Parity unpinned: the exact global flux-point index permutation and which side of a
+// face is slot 0 (setup_global_fpts + orient_fpts).  No reference-held output records
+// them; they are restated from the cited code, and the brute-force checks above only
+// prove that the numbering is a valid pairing.
+//
 // gimmik_mm_cpu: its generated source (src/gimmik_cpu.c) is absent from the reference.
 // GiMMiK bakes the nonzero entries of the constant operator into an unrolled dot product,
 // so op_mm restates it as C = alpha*A*B + beta*C over the nonzeros of A, summed in

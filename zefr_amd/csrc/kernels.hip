@@ -12,12 +12,17 @@
 //    integer map (no pointer tables), and computes the common solution / common flux for
 //    its own side of the face.  Both sides evaluate the same formula on the same L/R
 //    states, so the interface flux stays single-valued.
-//  * Navier-Stokes needs two grid-wide dependencies (U traces, then gradient traces), so
-//    a residual is three launches: extrapolate -> gradient -> divergence.  The divergence
-//    kernel recomputes the corrected gradient instead of storing it: re-reading U and the
-//    U trace costs less HBM traffic than writing and re-reading the gradient.
-//  * With LDG beta = +-1/2 only one side's gradient enters each common flux, so only that
-//    side's gradient trace is written and read.
+//  * Navier-Stokes with a general LDG beta needs two grid-wide dependencies (U traces,
+//    then gradient traces), so a residual is three launches: extrapolate -> gradient ->
+//    divergence.  The divergence kernel recomputes the corrected gradient instead of
+//    storing it: re-reading U and the U trace costs less HBM traffic than writing and
+//    re-reading the gradient.
+//  * With LDG beta = +-1/2 (the usual setting) only one side's gradient enters each
+//    common flux and the common solution is the other side's trace, so the "owner" side
+//    can finish the whole common flux right after its own gradient: extrapolate ->
+//    gradient_flux (gradient, volume divergence, owned common fluxes) -> face_correction.
+//    No gradient trace is stored at all.
+//  * Euler: extrapolate -> divergence.
 //
 // All arithmetic is FP64.  Reference call sites are cited at each kernel.
 
@@ -947,16 +952,22 @@ __global__ __launch_bounds__(kMaxThreads) void k_norm_partial(const double *__re
   }
 }
 
-__global__ __launch_bounds__(kMaxThreads) void k_copy(double *__restrict__ dst, const double *__restrict__ src,
-                                                      long n) {
-  const long stride = (long)gridDim.x * blockDim.x * 2;
-  for (long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < n; i += stride) {
-    if (i + 1 < n) {
-      const double2 v = *reinterpret_cast<const double2 *>(src + i);
-      *reinterpret_cast<double2 *>(dst + i) = v;
-    } else {
-      dst[i] = src[i];
-    }
+// Stream copy used as the device's measured bandwidth reference: each thread moves
+// kCopyUnroll 16-byte vectors, all loads issued before the stores.
+constexpr int kCopyUnroll = 4;
+__global__ __launch_bounds__(kMaxThreads) void k_copy(double2 *__restrict__ dst, const double2 *__restrict__ src,
+                                                      long n2) {
+  const long base = (long)blockIdx.x * blockDim.x * kCopyUnroll + threadIdx.x;
+  double2 v[kCopyUnroll];
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; u++) {
+    const long i = base + (long)u * blockDim.x;
+    if (i < n2) v[u] = src[i];
+  }
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; u++) {
+    const long i = base + (long)u * blockDim.x;
+    if (i < n2) dst[i] = v[u];
   }
 }
 
@@ -1145,7 +1156,15 @@ int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type,
 int zfr_stream_copy(double *dst, const double *src, long n, void *stream) {
   if (n <= 0) return 0;
   if (((uintptr_t)dst | (uintptr_t)src) & 15) return zfr::fail("zfr_stream_copy: pointers must be 16-byte aligned");
-  hipLaunchKernelGGL(zk::k_copy, dim3(4096), dim3(zk::kMaxThreads), 0, (hipStream_t)stream, dst, src, n);
+  const long n2 = n / 2, per_block = (long)zk::kMaxThreads * zk::kCopyUnroll;
+  if (n2 > 0)
+    hipLaunchKernelGGL(zk::k_copy, dim3((unsigned)((n2 + per_block - 1) / per_block)), dim3(zk::kMaxThreads), 0,
+                       (hipStream_t)stream, reinterpret_cast<double2 *>(dst), reinterpret_cast<const double2 *>(src),
+                       n2);
+  if (n & 1) {
+    hipError_t e = hipMemcpyAsync(dst + n - 1, src + n - 1, sizeof(double), hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    if (e != hipSuccess) return zfr::check_hip(e, "zfr_stream_copy");
+  }
   return zfr::check_hip(hipGetLastError(), "zfr_stream_copy");
 }
 
