@@ -226,6 +226,32 @@ int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type,
  * counterpart in the reference). */
 int zfr_stream_copy(double *dst, const double *src, long n, void *stream);
 
+/* ---------------------------------------------------------------- host physics --- */
+
+/* Physical parameters of the pointwise fluxes (InputStruct fields of the same names). */
+typedef struct zfr_phys {
+  double gamma, prandtl, mu, rt, c_sth;
+  int fix_vis;
+  double rus_k, ldg_b, ldg_tau;
+} zfr_phys_t;
+
+/* Host builds of the pointwise physics the kernels run, for checking on the CPU.
+ * zfr_host_point_flux: F[V][D] = Fconv(U) - Fvisc(U, dU) (include/flux.hpp:15-202;
+ * dU[V][D] may be NULL for the inviscid flux).  zfr_host_common_flux: the common normal
+ * flux of one side, Rusanov + LDG (src/faces.cpp:1244-1614), with each side's viscous
+ * normal flux formed from its own gradient dUL / dUR [V][D] (the LDG beta in `a` decides
+ * which are used).  The *_vjp variants return the reverse-mode derivatives (new; no
+ * reference counterpart): for the cotangent Fb of F they write Ub = (dF/dU)^T Fb and
+ * dUb = (dF/d dU)^T Fb. */
+int zfr_host_point_flux(int nDims, const zfr_phys_t *a, const double *U, const double *dU, double *F);
+int zfr_host_point_flux_vjp(int nDims, const zfr_phys_t *a, const double *U, const double *dU, const double *Fb,
+                            double *Ub, double *dUb);
+int zfr_host_common_flux(int nDims, const zfr_phys_t *a, int viscous, const double *UL, const double *UR,
+                         const double *dUL, const double *dUR, const double *n, double dAs, double *F);
+int zfr_host_common_flux_vjp(int nDims, const zfr_phys_t *a, int viscous, const double *UL, const double *UR,
+                             const double *dUL, const double *dUR, const double *n, double dAs, const double *Fb,
+                             double *ULb, double *URb, double *dULb, double *dURb);
+
 #ifdef __cplusplus
 }
 #endif

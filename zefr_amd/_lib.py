@@ -56,6 +56,14 @@ class EleDesc(C.Structure):
     ]
 
 
+class ZfrPhys(C.Structure):
+    """``zfr_phys_t``: physical parameters of the pointwise fluxes."""
+
+    _fields_ = [("gamma", C.c_double), ("prandtl", C.c_double), ("mu", C.c_double), ("rt", C.c_double),
+                ("c_sth", C.c_double), ("fix_vis", C.c_int), ("rus_k", C.c_double), ("ldg_b", C.c_double),
+                ("ldg_tau", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/zefr_amd.h
 _P = C.c_void_p
 _I = C.c_int
@@ -117,6 +125,11 @@ SIGNATURES = {
     "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_residual_norm": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P, _P]),
     "zfr_stream_copy": (_I, [_P, _P, C.c_long, _P]),
+    "zfr_host_point_flux": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP]),
+    "zfr_host_point_flux_vjp": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP, _DP, _DP]),
+    "zfr_host_common_flux": (_I, [_I, C.POINTER(ZfrPhys), _I, _DP, _DP, _DP, _DP, _DP, C.c_double, _DP]),
+    "zfr_host_common_flux_vjp": (_I, [_I, C.POINTER(ZfrPhys), _I, _DP, _DP, _DP, _DP, _DP, C.c_double, _DP,
+                                      _DP, _DP, _DP, _DP]),
 }
 
 

@@ -25,27 +25,6 @@ thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 const char *last_error() { return g_err.c_str(); }
 
-template <typename Fn>
-int guard(Fn &&fn) {
-  try {
-    fn();
-    return 0;
-  } catch (const std::exception &ex) {
-    set_error(ex.what());
-    return 1;
-  }
-}
-
-template <typename T, typename Fn>
-T guard_val(T fail_value, Fn &&fn) {
-  try {
-    return fn();
-  } catch (const std::exception &ex) {
-    set_error(ex.what());
-    return fail_value;
-  }
-}
-
 }  // namespace zfr
 
 struct zefr_s {

@@ -43,4 +43,26 @@ inline void status_or_throw(int rc) {
   if (rc != 0) throw Error(last_error());
 }
 
+// Run fn at a C entry point: exceptions become a stored message and status 1.
+template <typename Fn>
+int guard(Fn &&fn) {
+  try {
+    fn();
+    return 0;
+  } catch (const std::exception &ex) {
+    set_error(ex.what());
+    return 1;
+  }
+}
+
+template <typename T, typename Fn>
+T guard_val(T fail_value, Fn &&fn) {
+  try {
+    return fn();
+  } catch (const std::exception &ex) {
+    set_error(ex.what());
+    return fail_value;
+  }
+}
+
 }  // namespace zfr

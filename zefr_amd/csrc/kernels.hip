@@ -34,6 +34,7 @@
 
 #include "zefr_amd.h"
 #include "common.hpp"
+#include "physics.hpp"
 
 namespace zk {
 
@@ -115,185 +116,9 @@ __device__ __forceinline__ int fpt_end(int a, int b, int c, int dim, int side) {
 }
 
 // ---------------------------------------------------------------- pointwise physics
-// The reference forms the full flux tensor F[var][dim] (include/flux.hpp:1-202) and then
-// contracts it with a vector: the face normal (src/faces.cpp:1313-1322, 1580-1590) or a
-// row of the adjugate metric (src/elements.cpp:1676-1687).  Here the contraction is done
-// analytically: Fn = F . n is assembled from primitive quantities without materialising
-// F, which keeps register pressure low.  Terms are the same; only FP64 rounding order
-// differs.
-
-template <int D>
-struct Prim {
-  double invrho, P, H;
-  double vel[D];
-};
-
-// include/flux.hpp:11-63: invrho, pressure, total enthalpy, velocity
-template <int D>
-__device__ __forceinline__ Prim<D> primitives(const double *U, double gamma) {
-  constexpr int V = D + 2;
-  Prim<D> p;
-  p.invrho = 1. / U[0];
-  double momF = U[1] * U[1];
-#pragma unroll
-  for (int d = 1; d < D; d++) momF += U[d + 1] * U[d + 1];
-  momF *= p.invrho;
-  p.P = (gamma - 1.0) * (U[V - 1] - 0.5 * momF);
-  p.H = (U[V - 1] + p.P) * p.invrho;
-#pragma unroll
-  for (int d = 0; d < D; d++) p.vel[d] = U[d + 1] * p.invrho;
-  return p;
-}
-
-// Inviscid flux contracted with n: Fn[0] = m.n, Fn[1+k] = m_k (m.n)/rho + P n_k,
-// Fn[V-1] = H (m.n), with m the momentum.
-template <int D>
-__device__ __forceinline__ void conv_proj(const double *U, const Prim<D> &p, const double *n, double *Fn) {
-  constexpr int V = D + 2;
-  double mn = U[1] * n[0];
-#pragma unroll
-  for (int d = 1; d < D; d++) mn += U[d + 1] * n[d];
-  Fn[0] = mn;
-  const double un = mn * p.invrho;
-#pragma unroll
-  for (int k = 0; k < D; k++) Fn[k + 1] = U[k + 1] * un + p.P * n[k];
-  Fn[V - 1] = p.H * mn;
-}
-
-// Viscous stress tau and heat + work flux q of include/flux.hpp:75-202:
-// F_visc[1+i][j] = -tau[i][j], F_visc[V-1][j] = -q[j].
-template <int D>
-struct Visc {
-  double tau[D][D];
-  double q[D];
-};
-
-// The gradient enters only through its derivative directions one at a time: direction j
-// needs dU[0..V-1][j] and yields dvel[:][j] (velocity gradient) and de[j] (internal energy
-// gradient).  fetch(j, dUj) supplies dU[.][j]; a compiler memory barrier after each
-// direction keeps the next direction's loads from being hoisted, so only one direction's
-// V values are live (the gradient is often gathered straight from LDS or global memory).
-template <int D, typename Fetch>
-__device__ __forceinline__ Visc<D> viscous_dir(const double *U, const Prim<D> &p, Fetch &&fetch, const KArgs &a) {
-  constexpr int V = D + 2;
-  Visc<D> s;
-  double ke2 = p.vel[0] * p.vel[0];
-#pragma unroll
-  for (int d = 1; d < D; d++) ke2 += p.vel[d] * p.vel[d];
-  const double e_int = U[V - 1] * p.invrho - 0.5 * ke2;
-  double mu;
-  if (a.fix_vis) {
-    mu = a.mu;
-  } else {
-    const double rt_ratio = (a.gamma - 1.0) * e_int / a.rt;
-    mu = a.mu * pow(rt_ratio, 1.5) * (1. + a.c_sth) / (rt_ratio + a.c_sth);
-  }
-  double dvel[D][D], de[D];
-#pragma unroll
-  for (int j = 0; j < D; j++) {
-    double dUj[V];
-    fetch(j, dUj);
-#pragma unroll
-    for (int i = 0; i < D; i++) dvel[i][j] = (dUj[i + 1] - dUj[0] * p.vel[i]) * p.invrho;
-    double w = p.vel[0] * dvel[0][j];
-#pragma unroll
-    for (int i = 1; i < D; i++) w += p.vel[i] * dvel[i][j];
-    const double dke = 0.5 * ke2 * dUj[0] + U[0] * w;
-    de[j] = (dUj[V - 1] - dke - dUj[0] * e_int) * p.invrho;
-    asm volatile("" ::: "memory");
-  }
-  double div = dvel[0][0];
-#pragma unroll
-  for (int i = 1; i < D; i++) div += dvel[i][i];
-  const double diag = div / 3.0;
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++)
-      s.tau[i][j] = (i == j) ? 2.0 * mu * (dvel[i][i] - diag) : mu * (dvel[i][j] + dvel[j][i]);
-  const double kc = (mu / a.prandtl) * a.gamma;
-#pragma unroll
-  for (int j = 0; j < D; j++) {
-    double work = p.vel[0] * s.tau[0][j];
-#pragma unroll
-    for (int i = 1; i < D; i++) work += p.vel[i] * s.tau[i][j];
-    s.q[j] = work + kc * de[j];
-  }
-  return s;
-}
-
-template <int D>
-__device__ __forceinline__ Visc<D> viscous(const double *U, const Prim<D> &p, const double dU[][D], const KArgs &a) {
-  return viscous_dir<D>(U, p, [&](int j, double *dUj) {
-#pragma unroll
-    for (int v = 0; v < D + 2; v++) dUj[v] = dU[v][j];
-  }, a);
-}
-
-// Fn -= F_visc . n
-template <int D>
-__device__ __forceinline__ void visc_proj_sub(const Visc<D> &s, const double *n, double *Fn) {
-  constexpr int V = D + 2;
-#pragma unroll
-  for (int i = 0; i < D; i++) {
-    double t = s.tau[i][0] * n[0];
-#pragma unroll
-    for (int j = 1; j < D; j++) t += s.tau[i][j] * n[j];
-    Fn[i + 1] -= t;
-  }
-  double qn = s.q[0] * n[0];
-#pragma unroll
-  for (int j = 1; j < D; j++) qn += s.q[j] * n[j];
-  Fn[V - 1] -= qn;
-}
-
-// LDG beta branches of src/faces.cpp:1520-1545, fixed at compile time: BM = 0 any beta
-// (both sides' gradients), 1 beta = +1/2 (left side only), 2 beta = -1/2 (right only).
-// Common normal flux for this element's side of a flux point: Rusanov
-// (src/faces.cpp:1244-1364) plus LDG when viscous (src/faces.cpp:1476-1614), scaled by
-// the signed area of the side (Fcomm(0) = F dA0, Fcomm(1) = -F dA1).  For BM = 1 / 2 the
-// one gradient used (left / right side's) is passed in dG1; dG2 is the right side's for
-// BM = 0.
-// Viscous flux of one side contracted with n (zero-initialised output, as the reference's
-// FL / FR in src/faces.cpp:1555-1560).
-template <int D, typename Fetch>
-__device__ __forceinline__ void visc_normal(const double *U, Fetch &&fetch, const double *n, const KArgs &a,
-                                            double *G) {
-  constexpr int V = D + 2;
-#pragma unroll
-  for (int v = 0; v < V; v++) G[v] = 0.0;
-  const Prim<D> p = primitives<D>(U, a.gamma);
-  visc_proj_sub<D>(viscous_dir<D>(U, p, fetch, a), n, G);
-}
-
-// G1 / G2: viscous normal fluxes already formed (G1 = left for BM 0 / 1, right for BM 2;
-// G2 = right for BM 0).
-template <int D, bool VISC, int BM>
-__device__ __forceinline__ void common_flux(const double *UL, const double *UR, const double *G1,
-                                            const double *G2, const double *n, double dAs, const KArgs &a,
-                                            double *Fout) {
-  constexpr int V = D + 2;
-  const Prim<D> pL = primitives<D>(UL, a.gamma), pR = primitives<D>(UR, a.gamma);
-  double FnL[V], FnR[V];
-  conv_proj<D>(UL, pL, n, FnL);
-  conv_proj<D>(UR, pR, n, FnR);
-  const double aL = sqrt(a.gamma * pL.P * pL.invrho);
-  const double aR = sqrt(a.gamma * pR.P * pR.invrho);
-  const double eig = fmax(fabs(FnL[0] * pL.invrho) + aL, fabs(FnR[0] * pR.invrho) + aR);
-  const double diss = 0.5 * eig * (1.0 - a.rus_k);
-#pragma unroll
-  for (int v = 0; v < V; v++) Fout[v] = (0.5 * (FnR[v] + FnL[v]) - diss * (UR[v] - UL[v])) * dAs;
-  if (VISC) {
-    const double beta = a.ldg_b, tau = a.ldg_tau;
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      const double GL = (BM == 2) ? 0.0 : G1[v];
-      const double GR = (BM == 0) ? G2[v] : ((BM == 2) ? G1[v] : 0.0);
-      const double Fc = 0.5 * (GL + GR) + tau * (UL[v] - UR[v]) + beta * (GL - GR);
-      Fout[v] += Fc * dAs;
-    }
-  }
-}
+// primitives, conv_proj, viscous_dir, visc_proj_sub, visc_normal, common_flux and their
+// reverse-mode counterparts live in physics.hpp (shared with a host build).
+using namespace zphys;
 
 __device__ __forceinline__ void unpack_nbr(unsigned int p, long &ep, int &slot, int &fp) {
   ep = (long)(p >> 9);
