@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--order", type=int, default=3)
     p.add_argument("--tile", type=int, default=-1, help="elements per workgroup (-1: autotune)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-vjp", action="store_true", help="skip the VJP timing")
     p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
     p.add_argument("--cpu-steps", type=int, default=150)
     p.add_argument("--workdir", default=None)
@@ -159,6 +160,38 @@ def stream_copy_gbs(L, nbytes=2 << 30, reps=20):
     return 2 * n * 8 / (ms * 1e-3) / 1e9, ok
 
 
+def vjp_timing(z, reps):
+    """The residual's vector-Jacobian product on the same mesh and state: mean time of
+    `reps` back-to-back calls (HIP events on the torch stream both run on) against the
+    mean time of as many forward residual_at calls on that stream."""
+    import torch
+    U = z.state_tensor()
+    w = torch.randn_like(U)
+    v = torch.empty_like(U)
+    R = torch.empty_like(U)
+    st = torch.cuda.current_stream()
+    for _ in range(2):
+        z.residual_at(U, R)
+        z.residual_vjp(U, w, v)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(st)
+    for _ in range(reps):
+        z.residual_at(U, R)
+    ev[1].record(st)
+    for _ in range(reps):
+        z.residual_vjp(U, w, v)
+    ev[2].record(st)
+    ev[2].synchronize()
+    fwd = ev[0].elapsed_time(ev[1]) / reps
+    adj = ev[1].elapsed_time(ev[2]) / reps
+    ok = bool(torch.isfinite(v).all())
+    del U, w, v, R
+    torch.cuda.empty_cache()
+    return {"forward_ms": round(fwd, 4), "vjp_ms": round(adj, 4), "vjp_residual_ratio": round(adj / fwd, 3),
+            "finite": ok}
+
+
 def cpu_baseline(workdir, n, order, steps):
     """The CPU oracle on an n^3 sample of the same workload (OpenMP, all host threads)."""
     from oracle.oracle import Oracle, make_phys
@@ -242,6 +275,7 @@ def main():
     model = residual_model_bytes(D, N, True)
     res_gbs = model * E / (ms_step * 1e-3) / 1e9
     copy_gbs, copy_ok = stream_copy_gbs(lib())
+    vjp = None if args.no_vjp else vjp_timing(z, args.steps)
 
     rec = {
         "metric": METRIC,
@@ -270,6 +304,7 @@ def main():
                               "stream_copy_GBs": round(copy_gbs, 1), "stream_copy_ok": copy_ok,
                               "frac_of_stream_copy": round(res_gbs / copy_gbs, 4)},
         "kernels": per_kernel,
+        "vjp": vjp,
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(workdir, args.cpu_n, order, args.cpu_steps)

@@ -152,6 +152,16 @@ int zefr_set_tile(zefr_t *z, int eles_per_block);
  * launch (gradient is 0 for inviscid runs).  No counterpart in the reference: this is the
  * timing hook of the benchmark. */
 int zefr_time_residual_kernels(zefr_t *z, int reps, double *ms_out);
+/* divF = R(U) for an arbitrary device state U (device layout [E][V][nS], as
+ * get_u_spts_d) on the caller's `stream` (NULL: the null stream), with the kernels of
+ * compute_residual (src/solver.cpp:1494-1645) but without touching the solver's state.
+ * It uses the solver's trace scratch: do not overlap it with other work of the solver
+ * (its own stream is non-blocking, so synchronise in between). */
+int zefr_residual_at(zefr_t *z, const double *U, double *divF, void *stream);
+/* v = (dR/dU)^T w at U (all device pointers, layout [E][V][nS]) on `stream` (NULL: the
+ * null stream): the residual's vector-Jacobian product (new; no reference counterpart).
+ * Euler, or Navier-Stokes with LDG beta = +-1/2.  Same scratch rule as residual_at. */
+int zefr_residual_vjp(zefr_t *z, const double *U, const double *w, double *v, void *stream);
 
 /* ---------------------------------------------------------------- singleton ------ */
 
@@ -222,6 +232,12 @@ int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const 
  * sum squares (res_type 0/1/2), reduced on the device; work holds >= 1024*V doubles. */
 int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work,
                       double *res_out_device, void *stream);
+/* Vector-Jacobian product of one residual (new; no reference counterpart): for the state
+ * U [E][V][nS] and a cotangent w of divF (same layout) writes v = (d divF / dU)^T w.
+ * Euler, or Navier-Stokes with LDG beta = +-1/2.  Overwrites d->Utr with the trace of U;
+ * Wtr and Xtr are [E][V][nF] scratch buffers. */
+int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, double *v, double *Wtr,
+                     double *Xtr, void *stream);
 /* Plain device copy dst[i] = src[i], i < n (bandwidth calibration for the benchmark; no
  * counterpart in the reference). */
 int zfr_stream_copy(double *dst, const double *src, long n, void *stream);

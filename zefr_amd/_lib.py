@@ -106,6 +106,8 @@ SIGNATURES = {
     "zefr_autotune": (_I, [_P, _I]),
     "zefr_set_tile": (_I, [_P, _I]),
     "zefr_time_residual_kernels": (_I, [_P, _I, _DP]),
+    "zefr_residual_at": (_I, [_P, _P, _P, _P]),
+    "zefr_residual_vjp": (_I, [_P, _P, _P, _P, _P]),
     "zefr_initialize": (_I, [_CP]),
     "zefr_get_zefr_object": (_P, []),
     "zefr_set_zefr_object": (None, [_P]),
@@ -124,6 +126,7 @@ SIGNATURES = {
     "zfr_rk_stage": (_I, [C.POINTER(EleDesc), _P, _P, _P, C.c_double, C.c_double, _P]),
     "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_residual_norm": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P, _P]),
+    "zfr_residual_vjp": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, _P, _P]),
     "zfr_stream_copy": (_I, [_P, _P, C.c_long, _P]),
     "zfr_host_point_flux": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP]),
     "zfr_host_point_flux_vjp": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP, _DP, _DP]),
@@ -143,10 +146,23 @@ def build(verbose=False):
 _lib = None
 
 
+def _init_torch_first():
+    """PyTorch-ROCm ships its own HIP runtime (torch/lib/libamdhip64.so, loaded into the
+    global symbol scope).  Imported first, it also serves this library: the library's HIP
+    calls bind to it, so there is one runtime in the process and torch's streams, events
+    and allocations are valid here.  Loaded the other way round, two runtimes open the
+    device and torch then reports no GPU."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load the HIP library; raises if it has not been built."""
     global _lib
     if _lib is None:
+        _init_torch_first()
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 "libzefr_amd.so is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "

@@ -325,6 +325,14 @@ int zefr_time_residual_kernels(zefr_t *z, int reps, double *ms_out) {
   return zfr::guard([&] { z->s().time_kernels(reps, ms_out); });
 }
 
+int zefr_residual_at(zefr_t *z, const double *U, double *divF, void *stream) {
+  return zfr::guard([&] { z->s().residual_at(U, divF, (hipStream_t)stream); });
+}
+
+int zefr_residual_vjp(zefr_t *z, const double *U, const double *w, double *v, void *stream) {
+  return zfr::guard([&] { z->s().residual_vjp(U, w, v, (hipStream_t)stream); });
+}
+
 int zefr_set_tile(zefr_t *z, int eles_per_block) {
   return zfr::guard([&] {
     if (eles_per_block < 0) zfr::raise("tile must be >= 0");

@@ -158,6 +158,12 @@ class FRSolver {
   int autotune(int reps);
   bool onesided() const;  // viscous with LDG beta = +-1/2: one-sided gradient path
   void time_kernels(int reps, double *ms_out);
+  // The residual of an arbitrary device state U [E][V][nS] into divF (same layout) on the
+  // caller's stream st (nullptr: the null stream), with the kernels of compute_residual;
+  // and its vector-Jacobian product v = (d divF / dU)^T w.  Both use the solver's trace
+  // scratch, so they must not overlap other work of this solver.
+  void residual_at(const double *U, double *divF, hipStream_t st);
+  void residual_vjp(const double *U, const double *w, double *v, hipStream_t st);
 
   zefr_input_t *input;
   int order;
@@ -170,6 +176,7 @@ class FRSolver {
   int current_iter = 0;
   bool device_ready = false;
   DevBuf norm_work_d, norm_out_d;
+  DevBuf Wtr_d, Xtr_d;  // reverse-mode scratch [E][V][nF] each, allocated on first use
 
  private:
   void rk_stage_update(int stage);

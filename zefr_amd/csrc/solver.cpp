@@ -402,6 +402,30 @@ void FRSolver::compute_residual(unsigned stage, int color) {
   compute_residual_finish(stage, color);
 }
 
+void FRSolver::residual_at(const double *U, double *divF, hipStream_t st) {
+  need_device(*this);
+  const auto d = eles->desc(*input);
+  status_or_throw(zfr_extrapolate(&d, U, st));
+  if (onesided()) {
+    status_or_throw(zfr_gradient_flux(&d, U, divF, st));
+    status_or_throw(zfr_face_correction(&d, divF, st));
+  } else {
+    if (input->viscous) status_or_throw(zfr_gradient(&d, U, nullptr, st));
+    status_or_throw(zfr_divergence(&d, U, divF, st));
+  }
+}
+
+void FRSolver::residual_vjp(const double *U, const double *w, double *v, hipStream_t st) {
+  need_device(*this);
+  if (input->viscous && !onesided()) raise("the residual VJP needs LDG beta = +-0.5 (or an inviscid run)");
+  Elements &el = *eles;
+  const size_t tr = (size_t)el.nEles * el.nVars * el.nFpts * sizeof(double);
+  if (!Wtr_d.p) Wtr_d.alloc(tr);
+  if (!Xtr_d.p) Xtr_d.alloc(tr);
+  const auto d = el.desc(*input);
+  status_or_throw(zfr_residual_vjp(&d, U, w, v, Wtr_d.as<double>(), Xtr_d.as<double>(), st));
+}
+
 void FRSolver::compute_gradient_export() {
   need_device(*this);
   if (!input->viscous) raise("the gradient is only formed for viscous runs");

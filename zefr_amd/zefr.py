@@ -169,6 +169,45 @@ class Zefr:
     def set_tile(self, eles_per_block):
         check(lib().zefr_set_tile(self._h, eles_per_block))
 
+    # -- functional residual and its vector-Jacobian product on device tensors
+    def state_tensor(self):
+        """A device tensor copy of the current state in the device layout."""
+        import torch
+        return torch.from_numpy(self.state("U", layout="device")).to("cuda")
+
+    def _tensor_ok(self, t):
+        d = self.dims()
+        shape = (d["nEles"], d["nVars"], d["nSpts"])
+        if tuple(t.shape) != shape or not t.is_contiguous() or str(t.dtype) != "torch.float64" or not t.is_cuda:
+            raise ValueError(f"expected a contiguous float64 device tensor of shape {shape} "
+                             "(the device layout [nEles][nVars][nSpts])")
+
+    # Both run on torch's current stream (torch and the library share one HIP runtime, see
+    # _lib._init_torch_first), ordered with the surrounding torch work.
+    def residual_at(self, U, out=None):
+        """R(U) = divF for the device state U (torch float64 [nEles][nVars][nSpts]),
+        without touching the solver's own state."""
+        import torch
+        self._tensor_ok(U)
+        out = torch.empty_like(U) if out is None else out
+        self._tensor_ok(out)
+        self.synchronize()  # the solver's own stream is non-blocking; it shares the scratch
+        st = torch.cuda.current_stream().cuda_stream
+        check(lib().zefr_residual_at(self._h, U.data_ptr(), out.data_ptr(), st))
+        return out
+
+    def residual_vjp(self, U, w, out=None):
+        """v = (dR/dU)^T w at U (device tensors in the device layout)."""
+        import torch
+        self._tensor_ok(U)
+        self._tensor_ok(w)
+        out = torch.empty_like(U) if out is None else out
+        self._tensor_ok(out)
+        self.synchronize()  # the solver's own stream is non-blocking; it shares the scratch
+        st = torch.cuda.current_stream().cuda_stream
+        check(lib().zefr_residual_vjp(self._h, U.data_ptr(), w.data_ptr(), out.data_ptr(), st))
+        return out
+
     # -- data access (src/zefr.cpp:560-650)
     def _strides(self, fn, with_dim, etype):
         s = [C.c_int() for _ in range(4 if with_dim else 3)]
