@@ -1,0 +1,389 @@
+// adjoint.hpp -- reverse-mode kernels of the residual (included by kernels.hip, inside
+// namespace zk, after the forward kernels and their helpers).
+//
+// Vector-Jacobian product of one residual: v = (d divF / dU)^T w at the state U, for the
+// one-sided LDG path (beta = +-1/2) and for Euler (new; no reference counterpart).
+// The forward path, read backwards, is
+//   divF = Dss^T-lines(tF) + signed Dsf (Fcomm),  Fcomm = Ftr(owner) dA_side,
+//   Ftr  = RusanovLDG(U_L, U_R, G_owner . n),   G = viscous flux of (Utr, E grad),
+//   tF   = F(U, grad) . adj,  grad = adj^T (Dss U + Dsf Ucomm) / detJ,
+//   Ucomm = partner trace at owned points, own trace elsewhere, Utr = E U.
+// The owner side of each face (slot 0 for beta = +1/2 and Euler, slot 1 for -1/2) does all
+// the face work, as in the forward path: it combines both sides' weights of its common
+// flux, and hands the partner's share of the trace cotangent back through Xtr.
+//   k_adj_prep   : Utr = E U and Wtr = dA_side * (signed Dsf^T w) at every flux point;
+//   k_adj_main   : forward recompute, volume and face adjoints, gradient adjoint; writes
+//                  v (all of this element's own contributions) and Xtr at owned points;
+//   k_adj_gather : v += E^T (partner Xtr) at the points the partner owns.
+#pragma once
+
+// Stage EB consecutive [V][nS] blocks of x into LDS (stride V*nS).
+template <int D, int N>
+__device__ __forceinline__ void stage_block(const double *__restrict__ x, double *lx, long e0, long neb) {
+  using S = Shape<D, N>;
+  const double *xb = x + e0 * S::V * S::nS;
+  for (long i = threadIdx.x; i < neb * S::V * S::nS; i += blockDim.x) lx[i] = xb[i];
+}
+
+// Transposed line operator: sum_i Dss[i][m] x[base + i*str].
+template <int N>
+__device__ __forceinline__ double line_T_Dss(const double *lops, int m, const double *blk, int base, int str) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; i++) s += tab_Dss<N>(lops, i)[m] * blk[base + i * str];
+  return s;
+}
+
+// sum over the spts t of a line of Dsf[m_t][side] x(t)
+template <int N>
+__device__ __forceinline__ double line_Dsf(const double *lops, int side, const double *blk, int base, int str) {
+  double s = 0.0;
+#pragma unroll
+  for (int m = 0; m < N; m++) s += tab_Dsf<N>(lops, m)[side] * blk[base + m * str];
+  return s;
+}
+
+template <int D, int N>
+__global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restrict__ U, const double *__restrict__ w,
+                                                          double *__restrict__ Wtr, KArgs a) {
+  using S = Shape<D, N>;
+  constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  extern __shared__ double lds[];
+  double *lops = lds;
+  double *lU = lops + S::NOPS_PAD;
+  double *lW = lU + a.EB * V * nS;
+  const long e0 = (long)blockIdx.x * a.EB;
+  const long neb = min((long)a.EB, (long)a.E - e0);
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  stage_block<D, N>(w, lW, e0, neb);
+  __syncthreads();
+  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
+  const long e = e0 + el;
+  if (el >= neb) return;
+  const double *u = lU + el * V * nS, *wl = lW + el * V * nS;
+#pragma unroll
+  for (int k = 0; k < S::FPT; k++) {
+    const int f = t + k * nS;
+    if (f >= nF) continue;
+    int dim, side, base, str;
+    fpt_info<D, N>(f, dim, side, base, str);
+    const double *lE = tab_lE<N>(lops, side);
+    const double dAs = a.fgeo[(e * (D + 1) + D) * nF + f];
+    const double sg = side ? dAs : -dAs;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      a.Utr[(e * V + v) * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
+      Wtr[(e * V + v) * nF + f] = sg * line_Dsf<N>(lops, side, wl + v * nS, base, str);
+    }
+  }
+}
+
+// LDS per element: U [V][nS]; one region that holds the common solution, then the forward
+// gradient [D][V][nS]; the gradient cotangent [D][V][nS] (viscous); the own trace
+// cotangent [V][nF]; the owned-point list.  w is read from global memory (each value by
+// the few threads of its lines, from cache).  The partner's share of the trace cotangent
+// goes straight to Xtr; __syncthreads orders this block's global accesses too.
+template <int D, int N, bool VISC>
+struct AdjLds {
+  using S = Shape<D, N>;
+  static constexpr int R = VISC ? ((S::V * S::nF > D * S::V * S::nS) ? S::V * S::nF : D * S::V * S::nS) : 0;
+  static constexpr int GB = VISC ? D * S::V * S::nS : 0;
+  static constexpr int per_ele = S::V * S::nS + R + GB + S::V * S::nF + (S::nF + 2) / 2;  // doubles
+};
+
+template <int D, int N, bool VISC, int BM>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
+                double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
+  using S = Shape<D, N>;
+  using L = AdjLds<D, N, VISC>;
+  constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  constexpr int owner = (BM == 2) ? 1 : 0;
+  extern __shared__ double lds[];
+  const int EB = a.EB;
+  double *lops = lds;
+  double *lU = lops + S::NOPS_PAD;
+  double *lR = lU + EB * V * nS;
+  double *lGB = lR + EB * L::R;
+  double *lTo = lGB + EB * L::GB;
+  int *lcnt = reinterpret_cast<int *>(lTo + EB * V * nF);
+  int *llist = lcnt + EB;
+  const long e0 = (long)blockIdx.x * EB;
+  const long neb = min((long)EB, (long)a.E - e0);
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  if (threadIdx.x < EB) lcnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
+  const long e = e0 + el;
+  const bool valid = el < neb;
+  const double *u = lU + el * V * nS;
+  const double *we = w + e * V * nS;
+  double *c = lR + el * L::R, *g_l = lR + el * L::R;
+  double *gb = lGB + el * L::GB, *to = lTo + el * V * nF;
+  int *list = llist + el * nF;
+  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
+  // 1. owned list, zeroed own-trace cotangent, common solution (viscous)
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++) {
+      const int f = t + k * nS;
+      if (f >= nF) continue;
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      if (sl == owner) list[atomicAdd(&lcnt[el], 1)] = f;
+      if (VISC) {
+        if (sl == owner) {
+#pragma unroll
+          for (int v = 0; v < V; v++) c[v * nF + f] = a.Utr[(ep * V + v) * nF + fp];
+        } else {
+          int dim, side, base, str;
+          fpt_info<D, N>(f, dim, side, base, str);
+          const double *lE = tab_lE<N>(lops, side);
+#pragma unroll
+          for (int v = 0; v < V; v++) c[v * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) to[v * nF + f] = 0.0;
+    }
+  }
+  __syncthreads();
+  double ij[D][D], detJ = 1.0;
+  if (valid) load_metric<D>(a, e, t, nS, ij, detJ, VISC);
+  // 2. forward physical gradient at every spt (viscous), into the region of c
+  if (VISC) {
+    double g[V][D];
+    if (valid) spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
+    __syncthreads();  // the common solution is consumed
+    if (valid) {
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
+    }
+    __syncthreads();
+  }
+  // 3. volume adjoint: tFb[d1] = Dss^T w along the d1 lines; F(U, grad) reversed
+  double Ub[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) Ub[v] = 0.0;
+  if (valid) {
+    double Us[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
+    const Prim<D> p = primitives<D>(Us, a.gamma);
+    double tFb[D][V];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+      const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
+      const int base = t - id * str;
+#pragma unroll
+      for (int v = 0; v < V; v++) tFb[d][v] = line_T_Dss<N>(lops, id, we + v * nS, base, str);
+      conv_proj_vjp<D>(Us, p, ij[d], tFb[d], a.gamma, Ub);
+    }
+    if (VISC) {
+      // tF[d1] -= T . adj[d1]: cotangent on T[v][j] is -sum_d1 tFb[d1][v] adj[d1][j]
+      viscous_vjp<D>(
+          Us, p,
+          [&](int j, double *dUj) {
+#pragma unroll
+            for (int v = 0; v < V; v++) dUj[v] = g_l[(j * V + v) * nS + t];
+          },
+          [&](int v, int j) {
+            double s = 0.0;
+#pragma unroll
+            for (int d1 = 0; d1 < D; d1++) s += tFb[d1][v] * ij[d1][j];
+            return -s;
+          },
+          [&](int j, const double *gbj) {
+#pragma unroll
+            for (int v = 0; v < V; v++) gb[(j * V + v) * nS + t] = gbj[v];
+          },
+          a, Ub);
+    }
+  }
+  __syncthreads();
+  // 4. face adjoint at owned points; the partner's trace cotangent goes to Xtr, the
+  //    gradient cotangent spreads face by face (lines of one face never meet, so the
+  //    accumulation order is fixed)
+  const int cnt = valid ? lcnt[el] : 0;
+#pragma unroll
+  for (int kk = 0; kk < S::FPT; kk++) {
+    const int k = t + kk * nS;
+    const bool has = valid && k < cnt;
+    int f = 0, dim = 0, side = 0, base = 0, str = 0;
+    double dGb[V][D];
+    if (has) {
+      f = list[k];
+      fpt_info<D, N>(f, dim, side, base, str);
+      const double *lE = tab_lE<N>(lops, side);
+      const double *fg = a.fgeo + e * (D + 1) * nF + f;
+      double n[D];
+#pragma unroll
+      for (int d = 0; d < D; d++) n[d] = fg[d * nF];
+      double Uo[V], Uob[V], Gb[V];
+      {
+        long ep; int sl, fp;
+        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+        double Up[V], Fb[V], Upb[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
+          Up[v] = a.Utr[(ep * V + v) * nF + fp];
+          Fb[v] = Wtr[(e * V + v) * nF + f] + Wtr[(ep * V + v) * nF + fp];
+          Uob[v] = Upb[v] = Gb[v] = 0.0;
+        }
+        if (owner == 0) common_flux_vjp<D, VISC, BM>(Uo, Up, n, 1.0, a, Fb, Uob, Upb, Gb, nullptr);
+        else common_flux_vjp<D, VISC, BM>(Up, Uo, n, 1.0, a, Fb, Upb, Uob, Gb, nullptr);
+#pragma unroll
+        for (int v = 0; v < V; v++) Xtr[(e * V + v) * nF + f] = Upb[v];
+      }
+      asm volatile("" ::: "memory");
+      if (VISC) {
+        // G = -T(Uo, grad_o) . n
+        const Prim<D> po = primitives<D>(Uo, a.gamma);
+        viscous_vjp<D>(
+            Uo, po,
+            [&](int j, double *dUj) {
+#pragma unroll
+              for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+            },
+            [&](int v, int j) { return -Gb[v] * n[j]; },
+            [&](int j, const double *gbj) {
+#pragma unroll
+              for (int v = 0; v < V; v++) dGb[v][j] = gbj[v];
+            },
+            a, Uob);
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) to[v * nF + f] = Uob[v];
+    }
+    if (VISC) {
+      const int face = has ? f / S::nFF : -1;
+      const double *lE = tab_lE<N>(lops, side);
+#pragma unroll 1
+      for (int q = 0; q < 2 * D; q++) {
+        __syncthreads();
+        if (face == q) {
+#pragma unroll
+          for (int m = 0; m < N; m++)
+#pragma unroll
+            for (int j = 0; j < D; j++)
+#pragma unroll
+              for (int v = 0; v < V; v++) gb[(j * V + v) * nS + base + m * str] += lE[m] * dGb[v][j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (VISC) {
+    // 5. reference-space gradient cotangent: grb[d2][v] = sum_d1 adj[d2][d1] gb[v][d1] / detJ
+    if (valid) {
+      double gv[V][D];
+#pragma unroll
+      for (int j = 0; j < D; j++)
+#pragma unroll
+        for (int v = 0; v < V; v++) gv[v][j] = gb[(j * V + v) * nS + t];
+      const double inv_det = 1.0 / detJ;
+#pragma unroll
+      for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          double s = 0.0;
+#pragma unroll
+          for (int d1 = 0; d1 < D; d1++) s += ij[d2][d1] * gv[v][d1];
+          gb[(d2 * V + v) * nS + t] = s * inv_det;
+        }
+    }
+    __syncthreads();
+    // 6. Dss^T into U, Dsf^T into the common solution's cotangent: the partner's trace at
+    //    owned points (Xtr), this element's trace elsewhere
+    if (valid) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+        const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
+        const int base = t - id * str;
+#pragma unroll
+        for (int v = 0; v < V; v++) Ub[v] += line_T_Dss<N>(lops, id, gb + (d * V + v) * nS, base, str);
+      }
+#pragma unroll
+      for (int k = 0; k < S::FPT; k++) {
+        const int f = t + k * nS;
+        if (f >= nF) continue;
+        int dim, side, base, str;
+        fpt_info<D, N>(f, dim, side, base, str);
+        long ep; int sl, fp;
+        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          const double cb = line_Dsf<N>(lops, side, gb + (dim * V + v) * nS, base, str);
+          if (sl == owner) Xtr[(e * V + v) * nF + f] += cb;
+          else to[v * nF + f] += cb;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!valid) return;
+  // 7. own trace cotangent E^T into v
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+    const double l0 = tab_lE<N>(lops, 0)[id], l1 = tab_lE<N>(lops, 1)[id];
+    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
+#pragma unroll
+    for (int v = 0; v < V; v++) Ub[v] += l0 * to[v * nF + f0] + l1 * to[v * nF + f1];
+  }
+#pragma unroll
+  for (int v = 0; v < V; v++) vout[(e * V + v) * nS + t] = Ub[v];
+}
+
+// v += E^T (the owner's cotangent of this element's trace) at the points the partner owns.
+template <int D, int N, int BM>
+__global__ __launch_bounds__(kMaxThreads) void k_adj_gather(const double *__restrict__ Xtr, double *__restrict__ vout,
+                                                            KArgs a) {
+  using S = Shape<D, N>;
+  constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  constexpr int owner = (BM == 2) ? 1 : 0;
+  extern __shared__ double lds[];
+  const int EB = a.EB;
+  double *lops = lds;
+  double *lT = lops + S::NOPS_PAD;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
+  const long e0 = (long)blockIdx.x * EB;
+  const long neb = min((long)EB, (long)a.E - e0);
+  const int el = tid / nS, t = tid - el * nS;
+  const long e = e0 + el;
+  const bool valid = el < neb;
+  double *T = lT + el * V * nF;
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++) {
+      const int f = t + k * nS;
+      if (f >= nF) continue;
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+#pragma unroll
+      for (int v = 0; v < V; v++) T[v * nF + f] = (sl == owner) ? 0.0 : Xtr[(ep * V + v) * nF + fp];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
+  double acc[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) acc[v] = vout[(e * V + v) * nS + t];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+    const double l0 = tab_lE<N>(lops, 0)[id], l1 = tab_lE<N>(lops, 1)[id];
+    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
+#pragma unroll
+    for (int v = 0; v < V; v++) acc[v] += l0 * T[v * nF + f0] + l1 * T[v * nF + f1];
+  }
+#pragma unroll
+  for (int v = 0; v < V; v++) vout[(e * V + v) * nS + t] = acc[v];
+}

@@ -510,13 +510,15 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   constexpr int owner = (BM == 1) ? 0 : 1;  // slot whose gradient enters the common flux
+  // one LDS region per element holds the common solution, then (after a barrier) the
+  // gradient, then the transformed flux
+  constexpr int W = (V * nF > D * V * nS) ? V * nF : D * V * nS;
   extern __shared__ double lds[];
   const int EB = a.EB;
   double *lops = lds;
   double *lU = lops + S::NOPS_PAD;
-  double *lC = lU + EB * V * nS;
-  double *lG = lC + EB * V * nF;  // gradient, then the transformed flux
-  int *lcnt = reinterpret_cast<int *>(lG + EB * D * V * nS);
+  double *lW = lU + EB * V * nS;
+  int *lcnt = reinterpret_cast<int *>(lW + EB * W);
   int *llist = lcnt + EB;
   const long e0 = (long)blockIdx.x * EB;
   const long neb = min((long)EB, (long)a.E - e0);
@@ -527,8 +529,8 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
   const long e = e0 + el;
   const bool valid = el < neb;
   const double *u = lU + el * V * nS;
-  double *c = lC + el * V * nF;
-  double *g_l = lG + el * D * V * nS;
+  double *c = lW + el * W;
+  double *g_l = lW + el * W;
   int *list = llist + el * nF;
   // common solution (= the non-owner side's state) and the list of owned flux points
   if (valid) {
@@ -552,14 +554,20 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
     }
   }
   __syncthreads();
-  if (valid) {
-    double ij[D][D], detJ, g[V][D];
-    load_metric<D>(a, e, t, nS, ij, detJ, true);
-    spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
+  {
+    double g[V][D];
+    if (valid) {
+      double ij[D][D], detJ;
+      load_metric<D>(a, e, t, nS, ij, detJ, true);
+      spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
+    }
+    __syncthreads();  // the common solution is consumed
+    if (valid) {
 #pragma unroll
-    for (int d = 0; d < D; d++)
+      for (int d = 0; d < D; d++)
 #pragma unroll
-      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
+        for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
+    }
   }
   __syncthreads();
   // common normal flux (unscaled) at the owned flux points -> Ftr
@@ -577,15 +585,19 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
       for (int d = 0; d < D; d++) n[d] = fg[d * nF];
       double Uo[V], Up[V];
 #pragma unroll
-      for (int v = 0; v < V; v++) {
-        Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
-        Up[v] = c[v * nF + f];  // the partner's state, staged as the common solution
-      }
+      for (int v = 0; v < V; v++) Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
       double G[V];
       visc_normal<D>(Uo, [&](int j, double *dUj) {
 #pragma unroll
         for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
       }, n, a, G);
+      asm volatile("" ::: "memory");  // load the partner's state only now: fewer live registers
+      {
+        long ep; int sl, fp;
+        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+#pragma unroll
+        for (int v = 0; v < V; v++) Up[v] = a.Utr[(ep * V + v) * nF + fp];  // cached since phase 1
+      }
       double Fo[V];
       if (owner == 0) common_flux<D, true, BM>(Uo, Up, G, G, n, 1.0, a, Fo);
       else common_flux<D, true, BM>(Up, Uo, G, G, n, 1.0, a, Fo);
@@ -697,381 +709,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restr
 }
 
 // ---------------------------------------------------------------- reverse mode
-// Vector-Jacobian product of one residual: v = (d divF / dU)^T w at the state U, for the
-// one-sided LDG path (beta = +-1/2) and for Euler (new; no reference counterpart).
-// The forward path, read backwards, is
-//   divF = Dss^T-lines(tF) + signed Dsf (Fcomm),  Fcomm = Ftr(owner) dA_side,
-//   Ftr  = RusanovLDG(U_L, U_R, G_owner . n),   G = viscous flux of (Utr, E grad),
-//   tF   = F(U, grad) . adj,  grad = adj^T (Dss U + Dsf Ucomm) / detJ,
-//   Ucomm = partner trace at owned points, own trace elsewhere, Utr = E U.
-// The owner side of each face (slot 0 for beta = +1/2 and Euler, slot 1 for -1/2) does all
-// the face work, as in the forward path: it combines both sides' weights of its common
-// flux, and hands the partner's share of the trace cotangent back through Xtr.
-//   k_adj_prep   : Utr = E U and Wtr = dA_side * (signed Dsf^T w) at every flux point;
-//   k_adj_main   : forward recompute, volume and face adjoints, gradient adjoint; writes
-//                  v (all of this element's own contributions) and Xtr at owned points;
-//   k_adj_gather : v += E^T (partner Xtr) at the points the partner owns.
-
-// Stage EB consecutive [V][nS] blocks of x into LDS (stride V*nS).
-template <int D, int N>
-__device__ __forceinline__ void stage_block(const double *__restrict__ x, double *lx, long e0, long neb) {
-  using S = Shape<D, N>;
-  const double *xb = x + e0 * S::V * S::nS;
-  for (long i = threadIdx.x; i < neb * S::V * S::nS; i += blockDim.x) lx[i] = xb[i];
-}
-
-// Transposed line operator: sum_i T[i][m] x[base + i*str] with T[i] rows of a 1D table
-// (Dss: T[i][m] = tab_Dss(i)[m]).
-template <int N>
-__device__ __forceinline__ double line_T_Dss(const double *lops, int m, const double *blk, int base, int str) {
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; i++) s += tab_Dss<N>(lops, i)[m] * blk[base + i * str];
-  return s;
-}
-
-// sum over the spts t of a line of Dsf[m_t][side] x(t)
-template <int N>
-__device__ __forceinline__ double line_Dsf(const double *lops, int side, const double *blk, int base, int str) {
-  double s = 0.0;
-#pragma unroll
-  for (int m = 0; m < N; m++) s += tab_Dsf<N>(lops, m)[side] * blk[base + m * str];
-  return s;
-}
-
-template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restrict__ U, const double *__restrict__ w,
-                                                          double *__restrict__ Wtr, KArgs a) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  extern __shared__ double lds[];
-  double *lops = lds;
-  double *lU = lops + S::NOPS_PAD;
-  double *lW = lU + a.EB * V * nS;
-  const long e0 = (long)blockIdx.x * a.EB;
-  const long neb = min((long)a.EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  stage_block<D, N>(w, lW, e0, neb);
-  __syncthreads();
-  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  if (el >= neb) return;
-  const double *u = lU + el * V * nS, *wl = lW + el * V * nS;
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    if (f >= nF) continue;
-    int dim, side, base, str;
-    fpt_info<D, N>(f, dim, side, base, str);
-    const double *lE = tab_lE<N>(lops, side);
-    const double dAs = a.fgeo[(e * (D + 1) + D) * nF + f];
-    const double sg = side ? dAs : -dAs;
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      a.Utr[(e * V + v) * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
-      Wtr[(e * V + v) * nF + f] = sg * line_Dsf<N>(lops, side, wl + v * nS, base, str);
-    }
-  }
-}
-
-template <int D, int N, bool VISC, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_adj_main(const double *__restrict__ U, const double *__restrict__ w,
-                                                          const double *__restrict__ Wtr, double *__restrict__ Xtr,
-                                                          double *__restrict__ vout, KArgs a) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  constexpr int owner = (BM == 2) ? 1 : 0;
-  constexpr int PE = VISC ? (2 * V * nS + 2 * V * nF + 2 * D * V * nS) : (2 * V * nS + 2 * V * nF);
-  extern __shared__ double lds[];
-  const int EB = a.EB;
-  double *lops = lds;
-  double *lU = lops + S::NOPS_PAD;
-  double *lW = lU + EB * V * nS;
-  double *lC = lW + EB * V * nS;   // forward common solution; then the partner's trace cotangent
-  double *lTo = lC + EB * V * nF;  // cotangent of this element's own trace
-  double *lG = lTo + EB * V * nF;  // forward physical gradient [D][V][nS]
-  double *lGB = VISC ? lG + EB * D * V * nS : lG;  // gradient cotangent, then its reference-space form
-  int *lcnt = reinterpret_cast<int *>(lU + EB * PE);
-  int *llist = lcnt + EB;
-  const long e0 = (long)blockIdx.x * EB;
-  const long neb = min((long)EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  stage_block<D, N>(w, lW, e0, neb);
-  if (threadIdx.x < EB) lcnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  const bool valid = el < neb;
-  const double *u = lU + el * V * nS, *wl = lW + el * V * nS;
-  double *c = lC + el * V * nF, *to = lTo + el * V * nF;
-  double *g_l = lG + el * D * V * nS, *gb = lGB + el * D * V * nS;
-  int *list = llist + el * nF;
-  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-  // 1. common solution (viscous), owned list, zeroed trace cotangents
-  if (valid) {
-#pragma unroll
-    for (int k = 0; k < S::FPT; k++) {
-      const int f = t + k * nS;
-      if (f >= nF) continue;
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      if (sl == owner) {
-        list[atomicAdd(&lcnt[el], 1)] = f;
-#pragma unroll
-        for (int v = 0; v < V; v++) c[v * nF + f] = a.Utr[(ep * V + v) * nF + fp];
-      } else if (VISC) {
-        int dim, side, base, str;
-        fpt_info<D, N>(f, dim, side, base, str);
-        const double *lE = tab_lE<N>(lops, side);
-#pragma unroll
-        for (int v = 0; v < V; v++) c[v * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
-      }
-#pragma unroll
-      for (int v = 0; v < V; v++) to[v * nF + f] = 0.0;
-    }
-  }
-  __syncthreads();
-  double ij[D][D], detJ = 1.0;
-  if (valid) load_metric<D>(a, e, t, nS, ij, detJ, VISC);
-  // 2. forward physical gradient at every spt (viscous)
-  if (VISC && valid) {
-    double g[V][D];
-    spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
-  }
-  if (VISC) __syncthreads();
-  // 3. volume adjoint: tFb[d1] = Dss^T w along the d1 lines; F(U, grad) reversed
-  double Ub[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) Ub[v] = 0.0;
-  if (valid) {
-    double Us[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
-    const Prim<D> p = primitives<D>(Us, a.gamma);
-    double tFb[D][V];
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-      const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
-      const int base = t - id * str;
-#pragma unroll
-      for (int v = 0; v < V; v++) tFb[d][v] = line_T_Dss<N>(lops, id, wl + v * nS, base, str);
-      conv_proj_vjp<D>(Us, p, ij[d], tFb[d], a.gamma, Ub);
-    }
-    if (VISC) {
-      // tF[d1] -= T . adj[d1]: cotangent on T[v][j] is -sum_d1 tFb[d1][v] adj[d1][j]
-      viscous_vjp<D>(
-          Us, p,
-          [&](int j, double *dUj) {
-#pragma unroll
-            for (int v = 0; v < V; v++) dUj[v] = g_l[(j * V + v) * nS + t];
-          },
-          [&](int v, int j) {
-            double s = 0.0;
-#pragma unroll
-            for (int d1 = 0; d1 < D; d1++) s += tFb[d1][v] * ij[d1][j];
-            return -s;
-          },
-          [&](int j, const double *gbj) {
-#pragma unroll
-            for (int v = 0; v < V; v++) gb[(j * V + v) * nS + t] = gbj[v];
-          },
-          a, Ub);
-    }
-  }
-  __syncthreads();
-  // 4. face adjoint at owned points; the gradient cotangent spreads face by face so that
-  //    lines of one face never meet (deterministic accumulation order)
-  const int cnt = valid ? lcnt[el] : 0;
-#pragma unroll
-  for (int kk = 0; kk < S::FPT; kk++) {
-    const int k = t + kk * nS;
-    const bool has = valid && k < cnt;
-    int f = 0, dim = 0, side = 0, base = 0, str = 0;
-    double dGb[V][D];
-    if (has) {
-      f = list[k];
-      fpt_info<D, N>(f, dim, side, base, str);
-      const double *lE = tab_lE<N>(lops, side);
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      const double *fg = a.fgeo + e * (D + 1) * nF + f;
-      double n[D];
-#pragma unroll
-      for (int d = 0; d < D; d++) n[d] = fg[d * nF];
-      double Uo[V], Up[V], Fb[V];
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
-        Up[v] = c[v * nF + f];
-        Fb[v] = Wtr[(e * V + v) * nF + f] + Wtr[(ep * V + v) * nF + fp];
-      }
-      double Uob[V], Upb[V], Gb[V];
-#pragma unroll
-      for (int v = 0; v < V; v++) Uob[v] = Upb[v] = Gb[v] = 0.0;
-      if (owner == 0) common_flux_vjp<D, VISC, BM>(Uo, Up, n, 1.0, a, Fb, Uob, Upb, Gb, nullptr);
-      else common_flux_vjp<D, VISC, BM>(Up, Uo, n, 1.0, a, Fb, Upb, Uob, Gb, nullptr);
-      if (VISC) {
-        // G = -T(Uo, grad_o) . n
-        const Prim<D> po = primitives<D>(Uo, a.gamma);
-        viscous_vjp<D>(
-            Uo, po,
-            [&](int j, double *dUj) {
-#pragma unroll
-              for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
-            },
-            [&](int v, int j) { return -Gb[v] * n[j]; },
-            [&](int j, const double *gbj) {
-#pragma unroll
-              for (int v = 0; v < V; v++) dGb[v][j] = gbj[v];
-            },
-            a, Uob);
-      }
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        to[v * nF + f] = Uob[v];
-        c[v * nF + f] = Upb[v];  // only this thread touches point f in this phase
-      }
-    }
-    if (VISC) {
-      const int face = has ? f / S::nFF : -1;
-      const double *lE = tab_lE<N>(lops, side);
-#pragma unroll 1
-      for (int q = 0; q < 2 * D; q++) {
-        __syncthreads();
-        if (face == q) {
-#pragma unroll
-          for (int m = 0; m < N; m++)
-#pragma unroll
-            for (int j = 0; j < D; j++)
-#pragma unroll
-              for (int v = 0; v < V; v++) gb[(j * V + v) * nS + base + m * str] += lE[m] * dGb[v][j];
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if (VISC) {
-    // 5. reference-space gradient cotangent: grb[d2][v] = sum_d1 adj[d2][d1] gb[v][d1] / detJ
-    if (valid) {
-      double gv[V][D];
-#pragma unroll
-      for (int j = 0; j < D; j++)
-#pragma unroll
-        for (int v = 0; v < V; v++) gv[v][j] = gb[(j * V + v) * nS + t];
-      const double inv_det = 1.0 / detJ;
-#pragma unroll
-      for (int d2 = 0; d2 < D; d2++)
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-          double s = 0.0;
-#pragma unroll
-          for (int d1 = 0; d1 < D; d1++) s += ij[d2][d1] * gv[v][d1];
-          gb[(d2 * V + v) * nS + t] = s * inv_det;
-        }
-    }
-    __syncthreads();
-    // 6. Dss^T into U, Dsf^T into the common solution's cotangent
-    if (valid) {
-#pragma unroll
-      for (int d = 0; d < D; d++) {
-        const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-        const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
-        const int base = t - id * str;
-#pragma unroll
-        for (int v = 0; v < V; v++) Ub[v] += line_T_Dss<N>(lops, id, gb + (d * V + v) * nS, base, str);
-      }
-#pragma unroll
-      for (int k = 0; k < S::FPT; k++) {
-        const int f = t + k * nS;
-        if (f >= nF) continue;
-        int dim, side, base, str;
-        fpt_info<D, N>(f, dim, side, base, str);
-        long ep; int sl, fp;
-        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-        double *dst = (sl == owner) ? c : to;
-#pragma unroll
-        for (int v = 0; v < V; v++) dst[v * nF + f] += line_Dsf<N>(lops, side, gb + (dim * V + v) * nS, base, str);
-      }
-    }
-    __syncthreads();
-  }
-  if (!valid) return;
-  // 7. partner share out, own trace cotangent E^T into v
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    if (f >= nF) continue;
-    long ep; int sl, fp;
-    unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-    if (sl == owner) {
-#pragma unroll
-      for (int v = 0; v < V; v++) Xtr[(e * V + v) * nF + f] = c[v * nF + f];
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-    const double l0 = tab_lE<N>(lops, 0)[id], l1 = tab_lE<N>(lops, 1)[id];
-    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
-#pragma unroll
-    for (int v = 0; v < V; v++) Ub[v] += l0 * to[v * nF + f0] + l1 * to[v * nF + f1];
-  }
-#pragma unroll
-  for (int v = 0; v < V; v++) vout[(e * V + v) * nS + t] = Ub[v];
-}
-
-// v += E^T (the owner's cotangent of this element's trace) at the points the partner owns.
-template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_adj_gather(const double *__restrict__ Xtr, double *__restrict__ vout,
-                                                            KArgs a) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  constexpr int owner = (BM == 2) ? 1 : 0;
-  extern __shared__ double lds[];
-  const int EB = a.EB;
-  double *lops = lds;
-  double *lT = lops + S::NOPS_PAD;
-  const int tid = threadIdx.x, NT = blockDim.x;
-  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const long e0 = (long)blockIdx.x * EB;
-  const long neb = min((long)EB, (long)a.E - e0);
-  const int el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  const bool valid = el < neb;
-  double *T = lT + el * V * nF;
-  if (valid) {
-#pragma unroll
-    for (int k = 0; k < S::FPT; k++) {
-      const int f = t + k * nS;
-      if (f >= nF) continue;
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-#pragma unroll
-      for (int v = 0; v < V; v++) T[v * nF + f] = (sl == owner) ? 0.0 : Xtr[(ep * V + v) * nF + fp];
-    }
-  }
-  __syncthreads();
-  if (!valid) return;
-  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-  double acc[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) acc[v] = vout[(e * V + v) * nS + t];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-    const double l0 = tab_lE<N>(lops, 0)[id], l1 = tab_lE<N>(lops, 1)[id];
-    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
-#pragma unroll
-    for (int v = 0; v < V; v++) acc[v] += l0 * T[v * nF + f0] + l1 * T[v * nF + f1];
-  }
-#pragma unroll
-  for (int v = 0; v < V; v++) vout[(e * V + v) * nS + t] = acc[v];
-}
+#include "adjoint.hpp"
 
 // ---------------------------------------------------------------- RK + norm
 // src/solver.cpp:2897-2941 (stage), 2997-3062 (final), element-major layout.
@@ -1225,11 +863,13 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
     case Op::Extrap: per_ele = (size_t)S::V * S::nS; break;
     case Op::Grad: per_ele = kUC > kG ? kUC : kG; break;
     case Op::Div: per_ele = kUC + kG; break;
-    case Op::OneSided: per_ele = kUC + kG + (S::nF + 2) / 2; break;
+    case Op::OneSided:
+      per_ele = (size_t)S::V * S::nS + (S::V * S::nF > (int)kG ? S::V * S::nF : kG) + (S::nF + 2) / 2;
+      break;
     case Op::FaceCorr: per_ele = (size_t)S::V * S::nF; break;
     case Op::AdjPrep: per_ele = (size_t)2 * S::V * S::nS; break;
     case Op::AdjMain:
-      per_ele = (size_t)2 * S::V * S::nS + 2 * S::V * S::nF + (d->viscous ? 2 * kG : 0) + (S::nF + 2) / 2;
+      per_ele = d->viscous ? (size_t)AdjLds<D, N, true>::per_ele : (size_t)AdjLds<D, N, false>::per_ele;
       break;
     case Op::AdjGather: per_ele = (size_t)S::V * S::nF; break;
   }
@@ -1238,7 +878,11 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
     return zfr::fail("the residual VJP supports LDG beta = +-0.5 (and Euler)");
   if ((op == Op::OneSided || op == Op::FaceCorr) && d->ldg_b != 0.5 && d->ldg_b != -0.5)
     return zfr::fail("one-sided LDG kernels need ldg_b = +-0.5");
-  const int EB = eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele);
+  // the viscous adjoint runs one element per workgroup: its LDS then allows ~7
+  // workgroups per CU, more than any larger tile
+  const int EB = (op == Op::AdjMain && d->viscous) ? 1 : eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele);
+  if (sizeof(double) * (S::NOPS_PAD + EB * per_ele) > kMaxLds)
+    return zfr::fail("kernel launch: one element needs more LDS than a workgroup may use (order too high)");
   const KArgs a = make_args(d, EB);
   const dim3 grid((unsigned)((d->nEles + EB - 1) / EB)), block((unsigned)(EB * S::nS));
   const size_t lds = sizeof(double) * (S::NOPS_PAD + EB * per_ele);

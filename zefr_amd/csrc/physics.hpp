@@ -221,7 +221,7 @@ ZFR_HD void viscous_vjp(const double *U, const Prim<D> &p, Fetch &&fetch, Phi &&
   for (int d = 0; d < D; d++) ke2 += p.vel[d] * p.vel[d];
   const double e_int = U[V - 1] * p.invrho - 0.5 * ke2;
   const double mu = viscosity(e_int, a);
-  double dvel[D][D], de[D];
+  double dvel[D][D], de[D], d0[D];
   ZFR_UNROLL
   for (int j = 0; j < D; j++) {
     double dUj[V];
@@ -233,6 +233,8 @@ ZFR_HD void viscous_vjp(const double *U, const Prim<D> &p, Fetch &&fetch, Phi &&
     for (int i = 0; i < D; i++) w += p.vel[i] * dvel[i][j];
     const double dke = 0.5 * ke2 * dUj[0] + U[0] * w;
     de[j] = (dUj[V - 1] - dke - dUj[0] * e_int) * p.invrho;
+    d0[j] = dUj[0];
+    asm volatile("" ::: "memory");
   }
   double div = 0.0;
   ZFR_UNROLL
@@ -271,26 +273,25 @@ ZFR_HD void viscous_vjp(const double *U, const Prim<D> &p, Fetch &&fetch, Phi &&
   for (int i = 0; i < D; i++)
     ZFR_UNROLL
     for (int j = 0; j < D; j++) velb[i][j] = mu * (tb[i][j] + tb[j][i]) - ((i == j) ? (2.0 / 3.0) * mu * trb : 0.0);
+  // The reverse sweep needs the gradient only through dU[0][j] (kept in d0) and through
+  // X_j = de_j / invrho and dU[1+i][j] - dU[0][j] u_i = dvel_ij / invrho, so it does not
+  // fetch the gradient again.
   double eib = 0.0, ke2b = 0.0, irb = 0.0, rhob = 0.0;
   ZFR_UNROLL
   for (int j = 0; j < D; j++) {
-    double dUj[V];
-    fetch(j, dUj);
     double dUb[V];
     // de_j = X_j invrho, X_j = dU[V-1][j] - dke_j - dU[0][j] e_int
     double w = 0.0;
     ZFR_UNROLL
     for (int i = 0; i < D; i++) w += p.vel[i] * dvel[i][j];
-    const double dke = 0.5 * ke2 * dUj[0] + U[0] * w;
-    const double X = dUj[V - 1] - dke - dUj[0] * e_int;
     const double Xb = deb[j] * p.invrho;
-    irb += deb[j] * X;
+    irb += deb[j] * de[j] * U[0];
     dUb[V - 1] = Xb;
     const double dkeb = -Xb;
     dUb[0] = -Xb * e_int;
-    eib += -Xb * dUj[0];
+    eib += -Xb * d0[j];
     // dke = ke2 dU0 / 2 + rho w
-    ke2b += 0.5 * dkeb * dUj[0];
+    ke2b += 0.5 * dkeb * d0[j];
     dUb[0] += 0.5 * ke2 * dkeb;
     rhob += dkeb * w;
     const double wb = dkeb * U[0];
@@ -298,10 +299,10 @@ ZFR_HD void viscous_vjp(const double *U, const Prim<D> &p, Fetch &&fetch, Phi &&
     ZFR_UNROLL
     for (int i = 0; i < D; i++) {
       const double vb = velb[i][j] + wb * p.vel[i];
-      ub[i] += wb * dvel[i][j] - vb * dUj[0] * p.invrho;
+      ub[i] += wb * dvel[i][j] - vb * d0[j] * p.invrho;
       dUb[i + 1] = vb * p.invrho;
       dUb[0] -= vb * p.vel[i] * p.invrho;
-      irb += vb * (dUj[i + 1] - dUj[0] * p.vel[i]);
+      irb += vb * dvel[i][j] * U[0];
     }
     sink(j, dUb);
   }
