@@ -42,6 +42,7 @@ def parse():
     p.add_argument("--tile", type=int, default=-1, help="elements per workgroup (-1: autotune)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-vjp", action="store_true", help="skip the VJP timing")
+    p.add_argument("--no-steps", action="store_true", help="skip the RK4 step timings")
     p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
     p.add_argument("--cpu-steps", type=int, default=150)
     p.add_argument("--workdir", default=None)
@@ -160,6 +161,67 @@ def stream_copy_gbs(L, nbytes=2 << 30, reps=20):
     return 2 * n * 8 / (ms * 1e-3) / 1e9, ok
 
 
+def step_timing(z, steps):
+    """Mean wall time of one RK4 step (FRSolver::update) per execution mode: the plain
+    launch sequence, the RK update fused into each residual's last kernel, and the fused
+    step replayed from a HIP graph.  The state is restored after each mode."""
+    U0 = z.state("U", layout="device")
+    out = {}
+    for name, fuse, graph in (("plain", False, False), ("fused", True, False), ("fused_graph", True, True)):
+        z.set_step_mode(fuse, graph)
+        z.set_state(U0, layout="device")
+        z.do_step()
+        z.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            z.do_step()
+        z.synchronize()
+        out[name + "_ms"] = round((time.perf_counter() - t) / steps * 1e3, 4)
+    z.set_step_mode(True, False)
+    z.set_state(U0, layout="device")
+    return out
+
+
+def small_case_steps(workdir, steps=200):
+    """The CPU-runnable case: 2D Euler isentropic vortex, 32^2 quads on [-5,5]^2, p=3,
+    RK44 dt=1e-3; per-step latency per execution mode (launch-bound at this size)."""
+    from zefr_amd import zefr
+    from zefr_amd.mesh import write_box
+    mesh = write_box(os.path.join(workdir, "vortex32.msh"), (32, 32), (-5.0, -5.0), (5.0, 5.0))
+    deck = os.path.join(workdir, "vortex.deck")
+    with open(deck, "w") as f:
+        f.write(f"""nDims 2
+meshfile {mesh}
+mesh_bound periodic periodic
+mesh_bound fluid fluid
+order 3
+equation EulerNS
+viscous 0
+n_steps {steps}
+dt_scheme RK44
+dt 1e-3
+dt_type 0
+ic_type 1
+test_case 1
+fconv_type Rusanov
+rus_k 0.0
+gamma 1.4
+disable_nondim 1
+rho_fs 1.0
+P_fs 1.0
+mach_fs 0.1
+output_prefix vortex
+""")
+    z = zefr.Zefr()
+    z.read_input(deck)
+    z.setup_solver()
+    d = z.dims()
+    res = step_timing(z, steps)
+    res["workload"] = (f"2D Euler isentropic vortex, 32^2 quads on [-5,5]^2, p=3 "
+                       f"({d['nEles'] * d['nSpts'] * d['nVars']} DoF), RK44 steps")
+    return res
+
+
 def vjp_timing(z, reps):
     """The residual's vector-Jacobian product on the same mesh and state: mean time of
     `reps` back-to-back calls (HIP events on the torch stream both run on) against the
@@ -276,6 +338,9 @@ def main():
     res_gbs = model * E / (ms_step * 1e-3) / 1e9
     copy_gbs, copy_ok = stream_copy_gbs(lib())
     vjp = None if args.no_vjp else vjp_timing(z, args.steps)
+    rk = None
+    if not args.no_steps:
+        rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
 
     rec = {
         "metric": METRIC,
@@ -305,6 +370,7 @@ def main():
                               "frac_of_stream_copy": round(res_gbs / copy_gbs, 4)},
         "kernels": per_kernel,
         "vjp": vjp,
+        "rk4_step": rk,
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(workdir, args.cpu_n, order, args.cpu_steps)

@@ -147,6 +147,11 @@ long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n);
  * candidates on the device (0 = keep the built-in default). Returns the chosen value. */
 int zefr_autotune(zefr_t *z, int reps);
 int zefr_set_tile(zefr_t *z, int eles_per_block);
+/* How FRSolver::step_RK (src/solver.cpp:2865-3097) runs: fuse_rk folds each stage's RK
+ * update into the residual's last kernel (default on); use_graph captures one whole time
+ * step (the U_ini copy and every launch) as a HIP graph and replays it (default off;
+ * recaptured when dt or the tile change).  Results are identical either way. */
+int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph);
 /* Run `reps` residuals (stage 0) with HIP events around each kernel on the solver's
  * stream; ms_out[0..2] = average duration of the extrapolate / gradient / divergence
  * launch (gradient is 0 for inviscid runs).  No counterpart in the reference: this is the
@@ -232,6 +237,22 @@ int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const 
  * sum squares (res_type 0/1/2), reduced on the device; work holds >= 1024*V doubles. */
 int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work,
                       double *res_out_device, void *stream);
+/* RK update fused into the last kernel of a residual (the stage update of
+ * src/solver.cpp:2897-2941 or the final combination of :2997-3062, same arithmetic as
+ * zfr_rk_stage / zfr_rk_final): after divF of `stage` is formed, mode 1 writes
+ * U = U_ini - alpha dt/detJ divF, mode 2 writes U = U_ini - sum_s beta_s dt/detJ divF_s
+ * with the other stages read from divF0 ([nStages][E][V][nS], stage 0 first). */
+typedef struct zfr_rk_args {
+  double *U;
+  const double *U_ini;
+  const double *divF0;
+  int mode, stage, nStages;
+  double alpha, dt;
+  double beta[4];
+} zfr_rk_args_t;
+int zfr_face_correction_rk(const zfr_ele_desc_t *d, double *divF, const zfr_rk_args_t *rk, void *stream);
+int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *rk,
+                      void *stream);
 /* Vector-Jacobian product of one residual (new; no reference counterpart): for the state
  * U [E][V][nS] and a cotangent w of divF (same layout) writes v = (d divF / dU)^T w.
  * Euler, or Navier-Stokes with LDG beta = +-1/2.  Overwrites d->Utr with the trace of U;

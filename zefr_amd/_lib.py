@@ -105,6 +105,7 @@ SIGNATURES = {
     "zefr_get_host_array": (C.c_long, [_P, _CP, _DP, C.c_long]),
     "zefr_autotune": (_I, [_P, _I]),
     "zefr_set_tile": (_I, [_P, _I]),
+    "zefr_set_step_mode": (_I, [_P, _I, _I]),
     "zefr_time_residual_kernels": (_I, [_P, _I, _DP]),
     "zefr_residual_at": (_I, [_P, _P, _P, _P]),
     "zefr_residual_vjp": (_I, [_P, _P, _P, _P, _P]),
@@ -127,6 +128,8 @@ SIGNATURES = {
     "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_residual_norm": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P, _P]),
     "zfr_residual_vjp": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, _P, _P]),
+    "zfr_face_correction_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
+    "zfr_divergence_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P]),
     "zfr_stream_copy": (_I, [_P, _P, C.c_long, _P]),
     "zfr_host_point_flux": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP]),
     "zfr_host_point_flux_vjp": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP, _DP, _DP]),

@@ -340,6 +340,10 @@ int zefr_set_tile(zefr_t *z, int eles_per_block) {
   });
 }
 
+int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph) {
+  return zfr::guard([&] { z->s().set_step_mode(fuse_rk != 0, use_graph != 0); });
+}
+
 // ---------------------------------------------------------------- singleton
 int zefr_initialize(const char *input_file) {
   return zfr::guard([&] {

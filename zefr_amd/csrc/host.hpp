@@ -177,10 +177,23 @@ class FRSolver {
   bool device_ready = false;
   DevBuf norm_work_d, norm_out_d;
   DevBuf Wtr_d, Xtr_d;  // reverse-mode scratch [E][V][nF] each, allocated on first use
+  // Time-step execution: the RK update fused into each residual's last kernel, and the
+  // whole step replayed from a captured HIP graph (recaptured when dt or the tile change).
+  bool fuse_rk = true;
+  bool use_graph = false;
+  void set_step_mode(bool fuse, bool graph);
 
  private:
   void rk_stage_update(int stage);
   void rk_final_update();
+  void finish_with_rk(int stage);
+  void step_RK_eager();
+  void drop_graph();
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0.0;
+  int graph_tile = -1;
+  bool graph_fused = false;
 };
 
 // p-multigrid (include/multigrid.hpp:17-44): the interface is kept; cycling is not

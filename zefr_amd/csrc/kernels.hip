@@ -70,6 +70,14 @@ struct KArgs {
   double *Utr;
   double *Gtr;
   double *Ftr;
+  // RK update fused into the last kernel of a residual (mode 0: none, 1: stage, 2: final)
+  struct Rk {
+    double *U;
+    const double *Uini, *divF0;  // divF0: stage 0 of the [nStages][E][V][nS] residuals
+    long total;
+    double alpha, dt, beta[4];
+    int nStages, stage, mode;
+  } rk;
 };
 
 // Flux point f of the reference ordering (src/quads.cpp:86-123, src/hexas.cpp:133-194):
@@ -151,6 +159,29 @@ __device__ __forceinline__ double line_extrap(const double *lE, const double *bl
 #pragma unroll
   for (int m = 0; m < N; m++) s += lE[m] * blk[base + m * str];
   return s;
+}
+
+// RK update of one spt straight after its residual, in the last kernel of the residual
+// (the stage update of src/solver.cpp:2897-2941 and the final combination of
+// :2997-3062, with the same arithmetic as k_rk_stage / k_rk_final).  acc holds this
+// stage's divF at the spt; the other stages' come from memory.
+template <int D>
+__device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc) {
+  constexpr int V = D + 2, nM = D * D + 1;
+  if (a.rk.mode == 0) return;
+  const double detJ = a.metric[(e * nM + D * D) * nS + t];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    const long i = (e * V + v) * nS + t;
+    if (a.rk.mode == 1) {
+      a.rk.U[i] = a.rk.Uini[i] - a.rk.alpha * a.rk.dt / detJ * acc[v];
+    } else {
+      double u = a.rk.Uini[i];
+      for (int s = 0; s < a.rk.nStages; s++)
+        u -= a.rk.beta[s] * a.rk.dt / detJ * ((s == a.rk.stage) ? acc[v] : a.rk.divF0[s * a.rk.total + i]);
+      a.rk.U[i] = u;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- kernel 1: traces
@@ -492,6 +523,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
   }
 #pragma unroll
   for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+  rk_epilogue<D>(a, e, t, nS, acc);
 }
 
 // ---------------------------------------------------------------- one-sided LDG path
@@ -706,6 +738,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restr
   }
 #pragma unroll
   for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+  rk_epilogue<D>(a, e, t, nS, acc);
 }
 
 // ---------------------------------------------------------------- reverse mode
@@ -838,6 +871,7 @@ inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
   a.rus_k = d->rus_k; a.ldg_b = d->ldg_b; a.ldg_tau = d->ldg_tau;
   a.ops = d->ops1d; a.metric = d->metric; a.fgeo = d->fgeo; a.nbr = d->nbr; a.Utr = d->Utr; a.Gtr = d->Gtr;
   a.Ftr = d->Ftr;
+  a.rk = KArgs::Rk{};
   return a;
 }
 
@@ -848,6 +882,7 @@ struct AdjPtrs {
   const double *w = nullptr;  // cotangent of divF
   double *Wtr = nullptr;      // [E][V][nF] face weights of w
   double *Xtr = nullptr;      // [E][V][nF] partner trace cotangent at owned points
+  const KArgs::Rk *rk = nullptr;  // fused RK update for Div / FaceCorr
 };
 
 template <int D, int N>
@@ -883,7 +918,11 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   const int EB = (op == Op::AdjMain && d->viscous) ? 1 : eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele);
   if (sizeof(double) * (S::NOPS_PAD + EB * per_ele) > kMaxLds)
     return zfr::fail("kernel launch: one element needs more LDS than a workgroup may use (order too high)");
-  const KArgs a = make_args(d, EB);
+  KArgs a = make_args(d, EB);
+  if (j.rk) {
+    if (op != Op::Div && op != Op::FaceCorr) return zfr::fail("kernel launch: RK fusion is for the last kernel only");
+    a.rk = *j.rk;
+  }
   const dim3 grid((unsigned)((d->nEles + EB - 1) / EB)), block((unsigned)(EB * S::nS));
   const size_t lds = sizeof(double) * (S::NOPS_PAD + EB * per_ele);
   switch (op) {
@@ -978,6 +1017,41 @@ int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, vo
 int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream) {
   if (!d->viscous) return zfr::fail("zfr_face_correction: descriptor is inviscid");
   return zk::dispatch(zk::Op::FaceCorr, d, nullptr, divF, (hipStream_t)stream);
+}
+
+namespace {
+int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *r,
+             hipStream_t st) {
+  if (!r || !r->U || !r->U_ini || (r->mode == 2 && r->nStages > 1 && !r->divF0))
+    return zfr::fail("RK fusion: missing buffer");
+  if (r->mode != 1 && r->mode != 2) return zfr::fail("RK fusion: mode must be 1 (stage) or 2 (final)");
+  if (r->nStages < 1 || r->nStages > 4 || r->stage < 0 || r->stage >= r->nStages)
+    return zfr::fail("RK fusion: stage out of range");
+  zk::KArgs::Rk rk{};
+  rk.U = r->U;
+  rk.Uini = r->U_ini;
+  rk.divF0 = r->divF0;
+  rk.total = (long)d->nEles * d->nVars * d->nSpts;
+  rk.alpha = r->alpha;
+  rk.dt = r->dt;
+  for (int s = 0; s < 4; s++) rk.beta[s] = r->beta[s];
+  rk.nStages = r->nStages;
+  rk.stage = r->stage;
+  rk.mode = r->mode;
+  zk::AdjPtrs j;
+  j.rk = &rk;
+  return zk::dispatch(op, d, U, divF, st, j);
+}
+}  // namespace
+
+int zfr_face_correction_rk(const zfr_ele_desc_t *d, double *divF, const zfr_rk_args_t *rk, void *stream) {
+  if (!d->viscous) return zfr::fail("zfr_face_correction_rk: descriptor is inviscid");
+  return rk_fused(zk::Op::FaceCorr, d, nullptr, divF, rk, (hipStream_t)stream);
+}
+
+int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *rk,
+                      void *stream) {
+  return rk_fused(zk::Op::Div, d, U, divF, rk, (hipStream_t)stream);
 }
 
 int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF, double alpha,

@@ -289,6 +289,7 @@ std::vector<double> Faces::export_array(const std::string &name, const Elements 
 FRSolver::FRSolver(zefr_input_t *in, int ord) : input(in), order(ord < 0 ? (int)in->order : ord) {}
 
 FRSolver::~FRSolver() {
+  drop_graph();
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -457,19 +458,88 @@ void FRSolver::rk_final_update() {
                                input->dt, stream));
 }
 
-// step_RK (src/solver.cpp:2865-3097) with a supplied dt.
-void FRSolver::step_RK() {
-  need_device(*this);
+// compute_residual_finish with the RK update of this stage done by the same kernel.
+void FRSolver::finish_with_rk(int stage) {
+  const auto d = eles->desc(*input);
+  const int ns = (int)input->nStages;
+  const size_t st = eles->state_size();
+  zfr_rk_args_t r{};
+  r.U = eles->U_spts_d.as<double>();
+  r.U_ini = (ns > 1) ? eles->U_ini_d.as<double>() : eles->U_spts_d.as<double>();
+  r.divF0 = eles->divF_d.as<double>();
+  r.stage = stage;
+  r.nStages = ns;
+  r.dt = input->dt;
+  if (stage < ns - 1) {
+    r.mode = 1;
+    r.alpha = kRkAlpha[stage];
+  } else {
+    r.mode = 2;
+    for (int s = 0; s < ns; s++) r.beta[s] = (ns > 1) ? kRkBeta[s] : 1.0;
+  }
+  double *divF = eles->divF_d.as<double>() + (size_t)stage * st;
+  if (onesided()) status_or_throw(zfr_face_correction_rk(&d, divF, &r, stream));
+  else status_or_throw(zfr_divergence_rk(&d, eles->U_spts_d.as<double>(), divF, &r, stream));
+}
+
+void FRSolver::step_RK_eager() {
   const int ns = (int)input->nStages;
   if (ns > 1)
     hip_or_throw(hipMemcpyAsync(eles->U_ini_d.p, eles->U_spts_d.p, eles->U_spts_d.bytes, hipMemcpyDeviceToDevice,
                                 stream), "hipMemcpyAsync");
-  for (int stage = 0; stage < ns - 1; stage++) {
-    compute_residual(stage);
-    rk_stage_update(stage);
+  for (int stage = 0; stage < ns; stage++) {
+    if (fuse_rk) {
+      compute_residual_start(stage);
+      compute_residual_mid(stage);
+      finish_with_rk(stage);
+    } else {
+      compute_residual(stage);
+      if (stage < ns - 1) rk_stage_update(stage);
+      else rk_final_update();
+    }
   }
-  compute_residual(ns - 1);
-  rk_final_update();
+}
+
+void FRSolver::drop_graph() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+}
+
+void FRSolver::set_step_mode(bool fuse, bool use) {
+  fuse_rk = fuse;
+  use_graph = use;
+  drop_graph();
+}
+
+// step_RK (src/solver.cpp:2865-3097) with a supplied dt.  With use_graph the step's
+// launches (the U_ini copy, three kernels per stage) are captured once and replayed; the
+// kernel arguments bake in dt and the tile, so a change of either recaptures.
+void FRSolver::step_RK() {
+  need_device(*this);
+  if (!use_graph) {
+    step_RK_eager();
+    return;
+  }
+  if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile || graph_fused != fuse_rk) {
+    drop_graph();
+    hip_or_throw(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      step_RK_eager();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hip_or_throw(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+    hip_or_throw(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    graph_dt = input->dt;
+    graph_tile = eles->tile;
+    graph_fused = fuse_rk;
+  }
+  hip_or_throw(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
 }
 
 void FRSolver::update() {

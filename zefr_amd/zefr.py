@@ -169,6 +169,10 @@ class Zefr:
     def set_tile(self, eles_per_block):
         check(lib().zefr_set_tile(self._h, eles_per_block))
 
+    def set_step_mode(self, fuse_rk=True, use_graph=False):
+        """RK update fused into each residual's last kernel; whole step as a HIP graph."""
+        check(lib().zefr_set_step_mode(self._h, int(fuse_rk), int(use_graph)))
+
     # -- functional residual and its vector-Jacobian product on device tensors
     def state_tensor(self):
         """A device tensor copy of the current state in the device layout."""
