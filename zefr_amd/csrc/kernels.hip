@@ -619,9 +619,14 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
 #pragma unroll
       for (int v = 0; v < V; v++) Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
       double G[V];
+      // one variable's line at a time (a compiler barrier after each): 128 VGPRs instead
+      // of 160, i.e. one more wave per SIMD, worth more than the loads' overlap
       visc_normal<D>(Uo, [&](int j, double *dUj) {
 #pragma unroll
-        for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+        for (int v = 0; v < V; v++) {
+          dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+          asm volatile("" ::: "memory");
+        }
       }, n, a, G);
       asm volatile("" ::: "memory");  // load the partner's state only now: fewer live registers
       {
