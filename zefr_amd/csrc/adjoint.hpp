@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restri
 
 // LDS per element: U [V][nS]; one region that holds the common solution, then the forward
 // gradient [D][V][nS]; the gradient cotangent [D][V][nS] (viscous); the own trace
-// cotangent [V][nF]; the owned-point list.  w is read from global memory (each value by
+// cotangent [V][nF]; the owned-face mask.  w is read from global memory (each value by
 // the few threads of its lines, from cache).  The partner's share of the trace cotangent
 // goes straight to Xtr; __syncthreads orders this block's global accesses too.
 template <int D, int N, bool VISC>
@@ -88,7 +88,7 @@ struct AdjLds {
   using S = Shape<D, N>;
   static constexpr int R = VISC ? ((S::V * S::nF > D * S::V * S::nS) ? S::V * S::nF : D * S::V * S::nS) : 0;
   static constexpr int GB = VISC ? D * S::V * S::nS : 0;
-  static constexpr int per_ele = S::V * S::nS + R + GB + S::V * S::nF + (S::nF + 2) / 2;  // doubles
+  static constexpr int per_ele = S::V * S::nS + R + GB + S::V * S::nF + 1;  // doubles
 };
 
 template <int D, int N, bool VISC, int BM>
@@ -106,12 +106,11 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
   double *lR = lU + EB * V * nS;
   double *lGB = lR + EB * L::R;
   double *lTo = lGB + EB * L::GB;
-  int *lcnt = reinterpret_cast<int *>(lTo + EB * V * nF);
-  int *llist = lcnt + EB;
+  unsigned *lmask = reinterpret_cast<unsigned *>(lTo + EB * V * nF);  // owned faces per element
   const long e0 = (long)blockIdx.x * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  if (threadIdx.x < EB) lcnt[threadIdx.x] = 0;
+  if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
   __syncthreads();
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
@@ -120,7 +119,6 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
   const double *we = w + e * V * nS;
   double *c = lR + el * L::R, *g_l = lR + el * L::R;
   double *gb = lGB + el * L::GB, *to = lTo + el * V * nF;
-  int *list = llist + el * nF;
   const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
   // 1. owned list, zeroed own-trace cotangent, common solution (viscous)
   if (valid) {
@@ -130,7 +128,7 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
       if (f >= nF) continue;
       long ep; int sl, fp;
       unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      if (sl == owner) list[atomicAdd(&lcnt[el], 1)] = f;
+      if (sl == owner && f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
       if (VISC) {
         if (sl == owner) {
 #pragma unroll
@@ -207,7 +205,8 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
   // 4. face adjoint at owned points; the partner's trace cotangent goes to Xtr, the
   //    gradient cotangent spreads face by face (lines of one face never meet, so the
   //    accumulation order is fixed)
-  const int cnt = valid ? lcnt[el] : 0;
+  const unsigned mask = valid ? lmask[el] : 0u;
+  const int cnt = __popc(mask) * S::nFF;
 #pragma unroll
   for (int kk = 0; kk < S::FPT; kk++) {
     const int k = t + kk * nS;
@@ -215,7 +214,7 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
     int f = 0, dim = 0, side = 0, base = 0, str = 0;
     double dGb[V][D];
     if (has) {
-      f = list[k];
+      f = owned_fpt<D, N>(mask, k);
       fpt_info<D, N>(f, dim, side, base, str);
       const double *lE = tab_lE<N>(lops, side);
       const double *fg = a.fgeo + e * (D + 1) * nF + f;

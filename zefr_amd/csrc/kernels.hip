@@ -184,6 +184,22 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
   }
 }
 
+// The k-th owned flux point, owned faces taken in increasing order (mask bit q = face q).
+template <int D, int N>
+__device__ __forceinline__ int owned_fpt(unsigned mask, int k) {
+  using S = Shape<D, N>;
+  int q = k / S::nFF;
+  const int r = k - q * S::nFF;
+#pragma unroll
+  for (int b = 0; b < 2 * D; b++) {
+    if ((mask >> b) & 1u) {
+      if (q == 0) return b * S::nFF + r;
+      q--;
+    }
+  }
+  return 0;  // not reached for k < popcount(mask) * nFF
+}
+
 // ---------------------------------------------------------------- kernel 1: traces
 // Elements::extrapolate_U (src/elements.cpp:1337-1360): U_fpts = oppE U_spts.
 template <int D, int N>
@@ -536,9 +552,17 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
 //   k_extrapolate -> k_visc_onesided (gradient, volume flux divergence, owned common
 //   fluxes) -> k_face_correction (oppDiv_fpts Fcomm, both sides).
 
+// Register budget: for hex p=3 four waves per SIMD (<= 128 VGPRs) is what the kernel's
+// LDS allows per CU and the face phase fits it without spilling, so the allocator is held
+// to it; other shapes would spill under that cap and keep the default.
+template <int D, int N>
+struct OneSidedWaves {
+  static constexpr int value = (D == 3 && N == 4) ? 4 : 1;
+};
+
 template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__restrict__ U,
-                                                               double *__restrict__ divF, KArgs a) {
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N>::value)))
+void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   constexpr int owner = (BM == 1) ? 0 : 1;  // slot whose gradient enters the common flux
@@ -550,12 +574,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
   double *lops = lds;
   double *lU = lops + S::NOPS_PAD;
   double *lW = lU + EB * V * nS;
-  int *lcnt = reinterpret_cast<int *>(lW + EB * W);
-  int *llist = lcnt + EB;
+  unsigned *lmask = reinterpret_cast<unsigned *>(lW + EB * W);  // owned faces per element
   const long e0 = (long)blockIdx.x * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  if (threadIdx.x < EB) lcnt[threadIdx.x] = 0;
+  if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
   __syncthreads();
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
@@ -563,8 +586,8 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
   const double *u = lU + el * V * nS;
   double *c = lW + el * W;
   double *g_l = lW + el * W;
-  int *list = llist + el * nF;
-  // common solution (= the non-owner side's state) and the list of owned flux points
+  // common solution (= the non-owner side's state) and the mask of owned faces (all flux
+  // points of a face share its slot, so ownership is per face)
   if (valid) {
 #pragma unroll
     for (int k = 0; k < S::FPT; k++) {
@@ -575,7 +598,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
       if (sl == owner) {
 #pragma unroll
         for (int v = 0; v < V; v++) c[v * nF + f] = a.Utr[(ep * V + v) * nF + fp];
-        list[atomicAdd(&lcnt[el], 1)] = f;
+        if (f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
       } else {
         int dim, side, base, str;
         fpt_info<D, N>(f, dim, side, base, str);
@@ -604,10 +627,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided(const double *__r
   __syncthreads();
   // common normal flux (unscaled) at the owned flux points -> Ftr
   if (valid) {
-    const int cnt = lcnt[el];
+    const unsigned mask = lmask[el];
+    const int cnt = __popc(mask) * S::nFF;
 #pragma unroll 1
     for (int k = t; k < cnt; k += nS) {
-      const int f = list[k];
+      const int f = owned_fpt<D, N>(mask, k);  // face-major: neighbouring lanes, neighbouring points
       int dim, side, base, str;
       fpt_info<D, N>(f, dim, side, base, str);
       const double *lE = tab_lE<N>(lops, side);
@@ -904,7 +928,7 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
     case Op::Grad: per_ele = kUC > kG ? kUC : kG; break;
     case Op::Div: per_ele = kUC + kG; break;
     case Op::OneSided:
-      per_ele = (size_t)S::V * S::nS + (S::V * S::nF > (int)kG ? S::V * S::nF : kG) + (S::nF + 2) / 2;
+      per_ele = (size_t)S::V * S::nS + (S::V * S::nF > (int)kG ? S::V * S::nF : kG) + 1;
       break;
     case Op::FaceCorr: per_ele = (size_t)S::V * S::nF; break;
     case Op::AdjPrep: per_ele = (size_t)2 * S::V * S::nS; break;
