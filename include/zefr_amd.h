@@ -33,6 +33,7 @@ extern "C" {
 
 #define ZEFR_MAX_BOUNDS 16
 #define ZEFR_NAME_LEN 64
+#define ZEFR_MAX_LEVELS 8
 
 /* Hot-path subset of InputStruct (include/inputstruct.hpp:113-194); field names and
  * meanings as in the reference.  Fields the residual path does not use are omitted. */
@@ -61,6 +62,14 @@ typedef struct zefr_input {
   char meshBoundBC[ZEFR_MAX_BOUNDS][ZEFR_NAME_LEN];
   unsigned int iter, initIter;
   double time;
+  /* p-multigrid (include/inputstruct.hpp:116-122; src/input.cpp:137-142) */
+  int p_multi;
+  unsigned int nLevels;                       /* entries used in mg_levels / mg_steps    */
+  unsigned int mg_levels[ZEFR_MAX_LEVELS];    /* orders, finest first (mg_levels[0]=order) */
+  unsigned int mg_steps[ZEFR_MAX_LEVELS];     /* smoothing steps per level               */
+  char mg_cycle[8];                           /* "V" or "FMG"                            */
+  unsigned int FMG_vcycles;
+  double rel_fac;                             /* prolongation relaxation factor          */
 } zefr_input_t;
 
 typedef struct zefr_s zefr_t;
@@ -167,6 +176,13 @@ int zefr_residual_at(zefr_t *z, const double *U, double *divF, void *stream);
  * null stream): the residual's vector-Jacobian product (new; no reference counterpart).
  * Euler, or Navier-Stokes with LDG beta = +-1/2.  Same scratch rule as residual_at. */
 int zefr_residual_vjp(zefr_t *z, const double *U, const double *w, double *v, void *stream);
+/* p-multigrid (PMGrid, include/multigrid.hpp:17-44): with p_multi set, setup_solver builds
+ * the coarse FRSolvers of mg_levels (PMGrid::setup, src/multigrid.cpp:18-103) and do_step
+ * runs PMGrid::cycle (src/multigrid.cpp:263-333) instead of one update.  The per-level
+ * arrays are copied here in the device layout [E][V][(P+1)^nDims] of level `level`
+ * (1..nLevels-1; level 0 is the solver itself, see zefr_copy_state): which = 0 U_spts,
+ * 2 divF(0), 3 source term, 4 stored solution, 5 correction.  n = capacity in doubles. */
+int zefr_pmg_copy_state(zefr_t *z, int level, int which, double *host, long n, int to_device);
 
 /* ---------------------------------------------------------------- singleton ------ */
 
@@ -233,6 +249,13 @@ int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const 
  * in stage order; divF holds nStages consecutive [E][V][nS] blocks. */
 int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
                  const double *beta, int nStages, double dt, void *stream);
+/* The same two updates with a p-multigrid source term, [E][V][nS] (RK_update_source,
+ * src/solver_kernels.cu:253-329, with the CPU path's arithmetic, src/solver.cpp:2946-2962,
+ * 3061-3073): every divF_s is replaced by divF_s + source.  source = NULL is the plain update. */
+int zfr_rk_stage_source(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
+                        const double *source, double alpha, double dt, void *stream);
+int zfr_rk_final_source(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
+                        const double *source, const double *beta, int nStages, double dt, void *stream);
 /* Residual norm of divF/detJ (src/solver.cpp:7005-7026): res[V] = max | sum abs |
  * sum squares (res_type 0/1/2), reduced on the device; work holds >= 1024*V doubles. */
 int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work,
@@ -249,6 +272,7 @@ typedef struct zfr_rk_args {
   int mode, stage, nStages;
   double alpha, dt;
   double beta[4];
+  const double *source;  /* p-multigrid source term added to each divF_s, or NULL */
 } zfr_rk_args_t;
 int zfr_face_correction_rk(const zfr_ele_desc_t *d, double *divF, const zfr_rk_args_t *rk, void *stream);
 int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *rk,
@@ -262,6 +286,19 @@ int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, 
 /* Plain device copy dst[i] = src[i], i < n (bandwidth calibration for the benchmark; no
  * counterpart in the reference). */
 int zfr_stream_copy(double *dst, const double *src, long n, void *stream);
+/* p-multigrid transfer between orders: the operator products of restrict_pmg, prolong_err
+ * and prolong_U (src/multigrid.cpp:335-472), which the reference runs as cublasDGEMM with the
+ * dense [nSpts_out][nSpts_in] operator oppRes / oppPro (src/quads.cpp:363-465,
+ * src/hexas.cpp:586-704).  That operator is the tensor product of a 1D table, so this takes
+ * the 1D table A [n_out][n_in] (row-major, device memory; n = order + 1 points per
+ * direction, 1..6) and computes, per element and variable,
+ * out = alpha * (A x A [x A]) in + beta * out, in [nEles][nVars][n_in^nDims] ->
+ * [nEles][nVars][n_out^nDims].  beta = 0 does not read out. */
+int zfr_pmg_transfer(int nDims, int nVars, long nEles, int n_in, int n_out, const double *A, double alpha,
+                     const double *in, double beta, double *out, void *stream);
+/* z[i] = a*x[i] + b*y[i], i < n (device_copy / device_add / device_subtract,
+ * src/aux_kernels.cu:143-205); z may alias x or y. */
+int zfr_axpby(long n, double a, const double *x, double b, const double *y, double *z, void *stream);
 
 /* ---------------------------------------------------------------- host physics --- */
 

@@ -62,6 +62,15 @@ def lib():
         L.zo_lagrange_d1.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double]
         L.zo_point_flux.argtypes = [C.c_int, C.POINTER(ZoPhys), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.zo_pmg_create.restype = C.c_void_p
+        L.zo_pmg_create.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.c_double, C.c_char_p, C.c_int]
+        L.zo_pmg_destroy.argtypes = [C.c_void_p]
+        L.zo_pmg_cycle.argtypes = [C.c_void_p, C.c_int]
+        L.zo_pmg_array.restype = C.c_long
+        L.zo_pmg_array.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_long, C.c_int]
+        L.zo_pmg_operator.restype = C.c_long
+        L.zo_pmg_operator.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_long]
         _lib = L
     return _lib
 
@@ -154,6 +163,61 @@ class Oracle:
         r = np.zeros(self.nVars)
         _check(lib().zo_report_residuals(self._h, _dp(r)))
         return r
+
+
+class OraclePMG:
+    """p-multigrid over an Oracle (multigrid.cpp:18-518): coarse oracle solvers at the
+    orders ``levels`` (finest first, equal to the fine order) on the same mesh."""
+
+    _WHICH = {"U": 0, "divF": 2, "source": 3, "solution": 4, "correction": 5}
+
+    def __init__(self, fine: Oracle, meshfile, levels, steps, rel_fac=1.0, cycle="V", fmg_vcycles=1):
+        self.fine = fine  # keeps the fine solver alive as long as the cycle object
+        self.levels = list(levels)
+        n = len(self.levels)
+        lv = (C.c_int * n)(*self.levels)
+        st = (C.c_int * n)(*steps)
+        self._h = lib().zo_pmg_create(fine._h, meshfile.encode(), n, lv, st, rel_fac, cycle.encode(), fmg_vcycles)
+        if not self._h:
+            raise RuntimeError(lib().zo_last_error().decode())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().zo_pmg_destroy(h)
+            self._h = None
+
+    def cycle(self, n=1):
+        _check(lib().zo_pmg_cycle(self._h, n))
+
+    def _shape(self, level):
+        f = self.fine
+        return ((self.levels[level] + 1) ** f.nDims, f.nVars, f.nEles)
+
+    def array(self, level, which="U"):
+        a = np.zeros(self._shape(level))
+        if lib().zo_pmg_array(self._h, level, self._WHICH[which], _dp(a), a.size, 0) < 0:
+            raise RuntimeError(lib().zo_last_error().decode())
+        return a
+
+    def set_U(self, level, U):
+        a = np.ascontiguousarray(U, np.float64)
+        assert a.shape == self._shape(level)
+        if lib().zo_pmg_array(self._h, level, 0, _dp(a), a.size, 1) < 0:
+            raise RuntimeError(lib().zo_last_error().decode())
+
+    def operator(self, level, kind):
+        """Dense operator of a level: kind 'res' (to level+1) or 'pro' (to level-1),
+        [nSpts_to][nSpts_from]."""
+        k = {"res": 0, "pro": 1}[kind]
+        cnt = lib().zo_pmg_operator(self._h, level, k, None, 0)
+        if cnt < 0:
+            raise RuntimeError(lib().zo_last_error().decode())
+        a = np.zeros(cnt)
+        lib().zo_pmg_operator(self._h, level, k, _dp(a), cnt)
+        D = self.fine.nDims
+        to = self.levels[level + 1] if k == 0 else self.levels[level - 1]
+        return a.reshape((to + 1) ** D, (self.levels[level] + 1) ** D)
 
 
 def gl_points(n):

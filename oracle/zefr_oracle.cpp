@@ -50,6 +50,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <set>
 #include <sstream>
 #include <stdexcept>
@@ -965,8 +966,9 @@ struct Oracle {
     if (nDims == 2) residual_t<4, 2>(stage); else residual_t<5, 3>(stage);
   }
 
-  // step_RK (solver.cpp:2865-3097) with dt_type 0
-  void step_rk() {
+  // step_RK (solver.cpp:2865-3097) with dt_type 0; src (reference layout [nS][V][E]) is
+  // the p-multigrid source term of the sourceBT branches (solver.cpp:2946-2962, 3061-3073)
+  void step_rk(const double *src = nullptr) {
     const size_t E = nEles, V = nVars, n = V * E;
     const double dt = ph.dt;
     if (ph.nStages > 1) U_ini = U_spts;
@@ -978,7 +980,10 @@ struct Oracle {
         for (size_t v = 0; v < V; v++)
           for (size_t e = 0; e < E; e++) {
             size_t k = ((size_t)s * V + v) * E + e;
-            U_spts[k] = U_ini[k] - rk_alpha[stage] * dt / jaco_det_spts[(size_t)s * E + e] * dv[k];
+            if (src)
+              U_spts[k] = U_ini[k] - rk_alpha[stage] * dt / jaco_det_spts[(size_t)s * E + e] * (dv[k] + src[k]);
+            else
+              U_spts[k] = U_ini[k] - rk_alpha[stage] * dt / jaco_det_spts[(size_t)s * E + e] * dv[k];
           }
     }
     compute_residual(ph.nStages - 1);
@@ -992,7 +997,10 @@ struct Oracle {
         for (size_t v = 0; v < V; v++)
           for (size_t e = 0; e < E; e++) {
             size_t k = ((size_t)s * V + v) * E + e;
-            U_spts[k] -= b * dt / jaco_det_spts[(size_t)s * E + e] * dv[k];
+            if (src)
+              U_spts[k] -= b * dt / jaco_det_spts[(size_t)s * E + e] * (dv[k] + src[k]);
+            else
+              U_spts[k] -= b * dt / jaco_det_spts[(size_t)s * E + e] * dv[k];
           }
     }
   }
@@ -1014,6 +1022,180 @@ struct Oracle {
     for (size_t v = 0; v < V; v++) {
       double x = (ph.res_type == 2) ? std::sqrt(r[v]) : r[v];
       res[v] = x / nDoF;
+    }
+  }
+};
+
+// ---------------------------------------------------------------- p-multigrid
+// PMGrid (multigrid.cpp:18-518) with the dense tensor operators of Quads/Hexas::setup_PMG
+// (quads.cpp:363-465, hexas.cpp:586-704): oppRes/oppPro [nSpts_to][nSpts_from], built as
+// the reference builds them (single-order Lagrange steps between Gauss-Legendre point sets,
+// chained by dense matrix products in the reference's association) and applied as the
+// reference's cblas_dgemm over the [nSpts][nVars*nEles] arrays.
+struct OraclePMG {
+  Oracle *fine = nullptr;
+  std::vector<std::unique_ptr<Oracle>> grids;  // grids[0] empty: level 0 is `fine`
+  std::vector<int> levels, steps;
+  double rel_fac = 1.0;
+  std::string cycle_type = "V";
+  int fmg_vcycles = 1;
+  std::vector<std::vector<double>> oppRes, oppPro;           // per level
+  std::vector<std::vector<double>> sources, solutions, corrections;  // [nS][V][E] per level
+
+  int nLevels() const { return (int)levels.size(); }
+  Oracle &level(int n) { return n == 0 ? *fine : *grids[n]; }
+
+  // dense single-order step between orders: rows = points of order `to`, columns = order `from`
+  static std::vector<double> step_op(int D, int from, int to) {
+    const unsigned nf = from + 1, nt = to + 1;
+    const auto xf = gl_points(nf), xt = gl_points(nt);
+    const unsigned Nf = D == 2 ? nf * nf : nf * nf * nf, Nt = D == 2 ? nt * nt : nt * nt * nt;
+    std::vector<double> A((size_t)Nt * Nf);
+    for (unsigned spt = 0; spt < Nf; spt++) {
+      const unsigned i1 = spt % nf, j1 = (spt / nf) % nf, k1 = spt / (nf * nf);
+      for (unsigned r = 0; r < Nt; r++) {
+        const unsigned i2 = r % nt, j2 = (r / nt) % nt, k2 = r / (nt * nt);
+        double v = lagrange(xf, i1, xt[i2]) * lagrange(xf, j1, xt[j2]);
+        if (D == 3) v *= lagrange(xf, k1, xt[k2]);
+        A[(size_t)r * Nf + spt] = v;
+      }
+    }
+    return A;
+  }
+  static std::vector<double> dense_mm(const std::vector<double> &A, const std::vector<double> &B, int m, int k, int n) {
+    std::vector<double> C((size_t)m * n, 0.0);
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < n; j++) {
+        double s = 0.0;
+        for (int l = 0; l < k; l++) s += A[(size_t)i * k + l] * B[(size_t)l * n + j];
+        C[(size_t)i * n + j] = s;
+      }
+    return C;
+  }
+  static int npts(int D, int p) { return D == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1); }
+  // hexas.cpp:646-697: ((opps[res] opps[res+1]) opps[res+2]) ... opps[order-1]
+  static std::vector<double> restriction(int D, int order, int res) {
+    std::vector<double> R = step_op(D, res + 1, res);
+    for (int P = res + 1; P < order; P++) R = dense_mm(R, step_op(D, P + 1, P), npts(D, res), npts(D, P), npts(D, P + 1));
+    return R;
+  }
+  // hexas.cpp:594-644: ((opps[pro] opps[pro-1]) ...) opps[order+1]
+  static std::vector<double> prolongation(int D, int order, int pro) {
+    std::vector<double> Pm = step_op(D, pro - 1, pro);
+    for (int P = pro - 1; P > order; P--) Pm = dense_mm(Pm, step_op(D, P - 1, P), npts(D, pro), npts(D, P), npts(D, P - 1));
+    return Pm;
+  }
+
+  // multigrid.cpp:18-103
+  void setup(Oracle *f, const std::string &meshfile) {
+    fine = f;
+    const int D = f->nDims, L = nLevels();
+    if (L < 2 || levels[0] != f->order) fail("oracle: mg_levels must start at the fine order with >= 2 levels");
+    grids.clear();
+    grids.emplace_back();
+    oppRes.assign(L, {});
+    oppPro.assign(L, {});
+    sources.assign(L, {});
+    solutions.assign(L, {});
+    corrections.assign(L, {});
+    oppRes[0] = restriction(D, levels[0], levels[1]);
+    for (int n = 1; n < L; n++) {
+      auto g = std::make_unique<Oracle>();
+      g->nDims = D;
+      g->order = levels[n];
+      g->ph = f->ph;
+      g->setup(meshfile);
+      oppPro[n] = prolongation(D, levels[n], levels[n - 1]);
+      if (n + 1 < L) oppRes[n] = restriction(D, levels[n], levels[n + 1]);
+      const size_t sz = g->U_spts.size();
+      sources[n].assign(sz, 0.0);
+      solutions[n].assign(sz, 0.0);
+      corrections[n].assign(sz, 0.0);
+      grids.push_back(std::move(g));
+    }
+  }
+
+  // C[nSc][V*E] = alpha A[nSc][nSf] B[nSf][V*E] + beta C
+  static void gemm(const std::vector<double> &A, int m, int k, const double *B, int n, double alpha, double beta,
+                   double *C) {
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < m; i++) {
+        double s = 0.0;
+        for (int l = 0; l < k; l++) s += A[(size_t)i * k + l] * B[(size_t)l * n + j];
+        C[(size_t)i * n + j] = (beta == 0.0) ? alpha * s : alpha * s + beta * C[(size_t)i * n + j];
+      }
+  }
+
+  // multigrid.cpp:335-392
+  void restrict_pmg(int nf) {
+    Oracle &f = level(nf), &c = level(nf + 1);
+    const int n = f.nVars * f.nEles;
+    gemm(oppRes[nf], c.nSpts, f.nSpts, f.U_spts.data(), n, 1.0, 0.0, c.U_spts.data());
+    gemm(oppRes[nf], c.nSpts, f.nSpts, f.divF.data(), n, 1.0, 0.0, c.divF.data());
+  }
+  // multigrid.cpp:394-413
+  void prolong_err(int nc, const std::vector<double> &corr) {
+    Oracle &c = level(nc), &f = level(nc - 1);
+    gemm(oppPro[nc], f.nSpts, c.nSpts, corr.data(), c.nVars * c.nEles, rel_fac, 1.0, f.U_spts.data());
+  }
+  // multigrid.cpp:435-457
+  void prolong_U(int nc) {
+    Oracle &c = level(nc), &f = level(nc - 1);
+    gemm(oppPro[nc], f.nSpts, c.nSpts, c.U_spts.data(), c.nVars * c.nEles, 1.0, 0.0, f.U_spts.data());
+  }
+  // multigrid.cpp:474-502
+  void compute_source_term(int n) {
+    Oracle &g = level(n);
+    const size_t sz = g.U_spts.size();
+    for (size_t i = 0; i < sz; i++) sources[n][i] = g.divF[i];
+    g.compute_residual(0);
+    for (size_t i = 0; i < sz; i++) sources[n][i] -= g.divF[i];
+  }
+  // multigrid.cpp:105-261
+  void v_cycle(int lvl) {
+    const int L = nLevels();
+    if (lvl != L - 1) {
+      level(lvl).compute_residual(0);
+      restrict_pmg(lvl);
+    }
+    for (int n = lvl + 1; n < L; n++) {
+      compute_source_term(n);
+      solutions[n] = grids[n]->U_spts;
+      for (int s = 0; s < steps[n]; s++) grids[n]->step_rk(sources[n].data());
+      if (n + 1 < L) {
+        grids[n]->compute_residual(0);
+        for (size_t i = 0; i < sources[n].size(); i++) grids[n]->divF[i] += sources[n][i];
+        restrict_pmg(n);
+      }
+    }
+    for (int n = L - 1; n > lvl; n--) {
+      if (n != L - 1)
+        for (int s = 0; s < steps[n]; s++) grids[n]->step_rk(sources[n].data());
+      for (size_t i = 0; i < corrections[n].size(); i++) corrections[n][i] = grids[n]->U_spts[i] - solutions[n][i];
+      if (n > lvl + 1) prolong_err(n, corrections[n]);
+    }
+    if (lvl != L - 1) prolong_err(lvl + 1, corrections[lvl + 1]);
+  }
+  // multigrid.cpp:263-333
+  void cycle() {
+    if (cycle_type == "V") {
+      for (int s = 0; s < steps[0]; s++) fine->step_rk();
+      v_cycle(0);
+    } else if (cycle_type == "FMG") {
+      for (int n = nLevels() - 1; n > 0; n--) {
+        for (int c = 0; c < fmg_vcycles; c++) {
+          grids[n]->step_rk();
+          v_cycle(n);
+        }
+        grids[n]->step_rk();
+        if (n > 1) prolong_U(n);
+      }
+      prolong_U(1);
+      cycle_type = "V";
+      cycle();
+    } else {
+      fail("oracle: Multigrid cycle type unknown!");
     }
   }
 };
@@ -1121,6 +1303,73 @@ int zo_gl_points(int n, double *pts, double *wts) {
     std::copy(p.begin(), p.end(), pts);
     std::copy(w.begin(), w.end(), wts);
   });
+}
+
+void *zo_pmg_create(void *h, const char *meshfile, int nLevels, const int *levels, const int *steps, double rel_fac,
+                    const char *cycle_type, int fmg_vcycles) {
+  OraclePMG *p = new OraclePMG();
+  int rc = guarded([&] {
+    p->levels.assign(levels, levels + nLevels);
+    p->steps.assign(steps, steps + nLevels);
+    for (int n = 1; n < nLevels; n++)
+      if (levels[n] >= levels[n - 1] || levels[n] < 0) fail("oracle: mg_levels must decrease strictly");
+    p->rel_fac = rel_fac;
+    p->cycle_type = cycle_type;
+    p->fmg_vcycles = fmg_vcycles;
+    p->setup(static_cast<Oracle *>(h), meshfile);
+  });
+  if (rc) { delete p; return nullptr; }
+  return p;
+}
+
+void zo_pmg_destroy(void *p) { delete static_cast<OraclePMG *>(p); }
+
+int zo_pmg_cycle(void *p, int ncycles) {
+  return guarded([&] {
+    for (int i = 0; i < ncycles; i++) static_cast<OraclePMG *>(p)->cycle();
+  });
+}
+
+// which: 0 U_spts, 2 divF(0), 3 source, 4 solution, 5 correction; reference layout [nS][V][E]
+long zo_pmg_array(void *h, int level, int which, double *out, long n, int set) {
+  long count = -1;
+  int rc = guarded([&] {
+    auto *p = static_cast<OraclePMG *>(h);
+    if (level < 1 || level >= p->nLevels()) fail("oracle: pmg level out of range");
+    Oracle &g = *p->grids[level];
+    std::vector<double> *v = nullptr;
+    switch (which) {
+      case 0: v = &g.U_spts; break;
+      case 2: v = &g.divF; break;
+      case 3: v = &p->sources[level]; break;
+      case 4: v = &p->solutions[level]; break;
+      case 5: v = &p->corrections[level]; break;
+      default: fail("oracle: which must be 0, 2, 3, 4 or 5");
+    }
+    const long sz = (long)g.U_spts.size();
+    count = sz;
+    if (!out) return;
+    if (n < sz) fail("oracle: buffer too small");
+    if (set) std::copy(out, out + sz, v->begin());
+    else std::copy(v->begin(), v->begin() + sz, out);
+  });
+  return rc ? -1 : count;
+}
+
+// the dense PMG operators: kind 0 = restriction of level `level`, 1 = prolongation
+long zo_pmg_operator(void *h, int level, int kind, double *out, long n) {
+  long count = -1;
+  int rc = guarded([&] {
+    auto *p = static_cast<OraclePMG *>(h);
+    if (level < 0 || level >= p->nLevels()) fail("oracle: pmg level out of range");
+    const auto &A = kind == 0 ? p->oppRes[level] : p->oppPro[level];
+    count = (long)A.size();
+    if (out) {
+      if (n < count) fail("oracle: buffer too small");
+      std::copy(A.begin(), A.end(), out);
+    }
+  });
+  return rc ? -1 : count;
 }
 
 double zo_lagrange(const double *grid, int n, int mode, double x) {

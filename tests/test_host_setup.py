@@ -161,7 +161,6 @@ def test_deck_nondim_closed_form():
 @pytest.mark.parametrize("extra,msg", [
     (["equation AdvDiff"], "AdvDiff"),
     (["dt_type 1"], "dt_type"),
-    (["p_multi 1"], "multigrid"),
     (["motion 1"], "moving"),
     (["fconv_type Roe"], "Convective interface flux"),  # the reference's message
 ])

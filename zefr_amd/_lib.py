@@ -17,6 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 
 MAX_BOUNDS = 16
 NAME_LEN = 64
+MAX_LEVELS = 8
 
 
 class ZefrInput(C.Structure):
@@ -40,6 +41,9 @@ class ZefrInput(C.Structure):
         ("meshBoundName", (C.c_char * NAME_LEN) * MAX_BOUNDS),
         ("meshBoundBC", (C.c_char * NAME_LEN) * MAX_BOUNDS),
         ("iter", C.c_uint), ("initIter", C.c_uint), ("time", C.c_double),
+        ("p_multi", C.c_int), ("nLevels", C.c_uint), ("mg_levels", C.c_uint * MAX_LEVELS),
+        ("mg_steps", C.c_uint * MAX_LEVELS), ("mg_cycle", C.c_char * 8), ("FMG_vcycles", C.c_uint),
+        ("rel_fac", C.c_double),
     ]
 
 
@@ -109,6 +113,7 @@ SIGNATURES = {
     "zefr_time_residual_kernels": (_I, [_P, _I, _DP]),
     "zefr_residual_at": (_I, [_P, _P, _P, _P]),
     "zefr_residual_vjp": (_I, [_P, _P, _P, _P, _P]),
+    "zefr_pmg_copy_state": (_I, [_P, _I, _I, _DP, C.c_long, _I]),
     "zefr_initialize": (_I, [_CP]),
     "zefr_get_zefr_object": (_P, []),
     "zefr_set_zefr_object": (None, [_P]),
@@ -131,6 +136,10 @@ SIGNATURES = {
     "zfr_face_correction_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_divergence_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P]),
     "zfr_stream_copy": (_I, [_P, _P, C.c_long, _P]),
+    "zfr_rk_stage_source": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, C.c_double, C.c_double, _P]),
+    "zfr_rk_final_source": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, _DP, _I, C.c_double, _P]),
+    "zfr_pmg_transfer": (_I, [_I, _I, C.c_long, _I, _I, _P, C.c_double, _P, C.c_double, _P, _P]),
+    "zfr_axpby": (_I, [C.c_long, C.c_double, _P, C.c_double, _P, _P, _P]),
     "zfr_host_point_flux": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP]),
     "zfr_host_point_flux_vjp": (_I, [_I, C.POINTER(ZfrPhys), _DP, _DP, _DP, _DP, _DP]),
     "zfr_host_common_flux": (_I, [_I, C.POINTER(ZfrPhys), _I, _DP, _DP, _DP, _DP, _DP, C.c_double, _DP]),

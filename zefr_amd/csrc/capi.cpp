@@ -31,6 +31,7 @@ struct zefr_s {
   zefr_input_t input;
   bool have_input = false;
   std::unique_ptr<zfr::FRSolver> solver;
+  std::unique_ptr<zfr::PMGrid> pmg;  // with p_multi (src/zefr.cpp:321-328)
 
   zfr::FRSolver &s() {
     if (!solver) zfr::raise("solver is not set up (call setup_solver first)");
@@ -110,8 +111,13 @@ int zefr_init_inputs(zefr_t *z) {
 
 int zefr_setup_host(zefr_t *z) {
   return zfr::guard([&] {
+    z->pmg.reset();
     z->solver = std::make_unique<zfr::FRSolver>(&z->input);
     z->solver->setup_host();
+    if (z->input.p_multi) {
+      z->pmg = std::make_unique<zfr::PMGrid>();
+      z->pmg->setup(&z->input, *z->solver);
+    }
   });
 }
 
@@ -120,14 +126,23 @@ int zefr_setup_solver(zefr_t *z) {
     int ndev = 0;
     zfr::hip_or_throw(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
     if (ndev < 1) zfr::raise("no HIP device available");
+    z->pmg.reset();
     z->solver = std::make_unique<zfr::FRSolver>(&z->input);
     z->solver->setup_host();
     z->solver->setup_device();
+    if (z->input.p_multi) {
+      z->pmg = std::make_unique<zfr::PMGrid>();
+      z->pmg->setup(&z->input, *z->solver);
+    }
   });
 }
 
 int zefr_do_step(zefr_t *z) {
-  return zfr::guard([&] { z->s().update(); z->input.iter++; });
+  return zfr::guard([&] {
+    if (z->pmg) z->pmg->cycle(z->s());
+    else z->s().update();
+    z->input.iter++;
+  });
 }
 
 int zefr_do_rk_stage(zefr_t *z, int iter, int stage) {
@@ -278,6 +293,32 @@ int zefr_copy_state(zefr_t *z, int which, int stage, double *host, long n, int t
     zfr::hip_or_throw(hipMemcpy(to_device ? (void *)dev : (void *)host, to_device ? (void *)host : (void *)dev,
                                 st * sizeof(double), to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost),
                       "copy_state");
+  });
+}
+
+int zefr_pmg_copy_state(zefr_t *z, int level, int which, double *host, long n, int to_device) {
+  return zfr::guard([&] {
+    if (!z->pmg) zfr::raise("pmg_copy_state: p-multigrid is not set up (p_multi 0)");
+    auto &pm = *z->pmg;
+    if (level < 1 || level >= pm.nLevels) zfr::raise("pmg_copy_state: level must be in 1..nLevels-1");
+    auto &g = *pm.grids[level];
+    auto &e = *g.eles;
+    const size_t st = e.state_size();
+    if ((size_t)n < st) zfr::raise("pmg_copy_state: buffer too small");
+    double *dev = nullptr;
+    switch (which) {
+      case 0: dev = e.U_spts_d.as<double>(); break;
+      case 2: dev = e.divF_d.as<double>(); break;
+      case 3: dev = pm.sourcesBT[level].as<double>(); break;
+      case 4: dev = pm.solutionsBT[level].as<double>(); break;
+      case 5: dev = pm.correctionsBT[level].as<double>(); break;
+      default: zfr::raise("pmg_copy_state: which must be 0 (U), 2 (divF), 3 (source), 4 (solution) or 5 (correction)");
+    }
+    if (!dev) zfr::raise("pmg_copy_state: device arrays are not allocated");
+    z->s().synchronize();
+    zfr::hip_or_throw(hipMemcpy(to_device ? (void *)dev : (void *)host, to_device ? (void *)host : (void *)dev,
+                                st * sizeof(double), to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost),
+                      "pmg_copy_state");
   });
 }
 

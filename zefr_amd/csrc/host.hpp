@@ -13,6 +13,7 @@
 #include <array>
 #include <functional>
 #include <cstddef>
+#include <deque>
 #include <memory>
 #include <string>
 #include <vector>
@@ -182,6 +183,10 @@ class FRSolver {
   bool fuse_rk = true;
   bool use_graph = false;
   void set_step_mode(bool fuse, bool graph);
+  // FRSolver::update(sourceBT) (src/solver.cpp:2599-2646): one step with a p-multigrid
+  // source term [E][V][nS] added to every stage residual (nullptr: plain update)
+  void update(const double *source);
+  bool own_stream = true;  // false for p-multigrid levels, which run on the fine stream
 
  private:
   void rk_stage_update(int stage);
@@ -194,14 +199,45 @@ class FRSolver {
   double graph_dt = 0.0;
   int graph_tile = -1;
   bool graph_fused = false;
+  const double *graph_source = nullptr;
+  const double *rk_source = nullptr;  // source term of the step in progress
 };
 
-// p-multigrid (include/multigrid.hpp:17-44): the interface is kept; cycling is not
-// implemented yet and raises.
+// p-multigrid (include/multigrid.hpp:17-44, src/multigrid.cpp:18-518) on quads/hexes:
+// coarse FRSolvers at the orders of mg_levels on the fine solver's mesh, all on the fine
+// solver's stream, with explicit RK smoothing under a source term.  Level 0 is the
+// solver passed in; grids[0] is left empty as in the reference.
 class PMGrid {
  public:
   void setup(zefr_input_t *input, FRSolver &solver);
   void cycle(FRSolver &solver);
+  void v_cycle(FRSolver &solver, int level);
+  void restrict_pmg(FRSolver &grid_f, FRSolver &grid_c);
+  void prolong_err(FRSolver &grid_c, const double *correction, FRSolver &grid_f);
+  void prolong_U(FRSolver &grid_c, FRSolver &grid_f);
+  void compute_source_term(FRSolver &grid, double *source);
+
+  zefr_input_t *input = nullptr;
+  int order = 0, nLevels = 0;
+  std::vector<std::shared_ptr<FRSolver>> grids;
+  // per level: source term, stored solution and correction [E][V][nS(level)] (level >= 1;
+  // the fine level has no buffers: its correction is prolonged straight into U)
+  std::deque<DevBuf> sourcesBT, solutionsBT, correctionsBT;
+  // 1D transfer tables per level (row-major [n_out][n_in]): oppRes maps level n to n+1,
+  // oppPro maps level n to n-1 (src/quads.cpp:363-465, src/hexas.cpp:586-704)
+  std::vector<std::vector<double>> oppRes, oppPro;
+  std::deque<DevBuf> oppRes_d, oppPro_d;
+
+ private:
+  FRSolver &level(FRSolver &fine, int n) { return n == 0 ? fine : *grids[n]; }
+  void transfer(const FRSolver &from, const FRSolver &to, const DevBuf &A, double alpha, const double *in,
+                double beta, double *out);
 };
+
+// 1D p-multigrid tables (src/quads.cpp:363-465 in one direction): the chained restriction
+// from order p down to order r < p, and the chained prolongation from order p up to q > p,
+// each the product of single-order Lagrange interpolations between Gauss-Legendre point sets.
+std::vector<double> pmg_restriction_1d(int p, int r);
+std::vector<double> pmg_prolongation_1d(int p, int q);
 
 }  // namespace zfr

@@ -162,7 +162,27 @@ void read_input_file(const std::string &file, zefr_input_t &in) {
   if (f.uint("motion", 0)) raise("moving grids are not supported");
   if (f.uint("filt_on", 0)) raise("filtering is not supported");
   if (f.uint("adapt_dt", 0)) raise("adaptive time stepping is not supported");
-  if (f.uint("p_multi", 0)) raise("p-multigrid is not supported yet");
+  // p-multigrid (src/input.cpp:137-142); the level checks of PMGrid::setup
+  // (src/multigrid.cpp:24-29) are made here, where the lists are read
+  in.p_multi = f.uint("p_multi", 0) != 0;
+  copy_str(in.mg_cycle, sizeof(in.mg_cycle), f.str("mg_cycle", "V"));
+  in.FMG_vcycles = f.uint("FMG_vcycles", 1);
+  in.rel_fac = f.num("rel_fac", 1.0);
+  {
+    const auto *lv = f.find("mg_levels");
+    const auto *st = f.find("mg_steps");
+    const size_t nl = lv ? lv->size() : 0, ns = st ? st->size() : 0;
+    if (in.p_multi) {
+      if (nl == 0 || ns == 0) raise("Need to provide mg_levels and/or mg_steps to run multigrid!");
+      if (nl != ns) raise("Inconsistent number of mg_levels and mg_steps provided!");
+      if (nl > ZEFR_MAX_LEVELS) raise("at most 8 multigrid levels are supported");
+    }
+    in.nLevels = (unsigned)std::min<size_t>(std::min(nl, ns), ZEFR_MAX_LEVELS);
+    for (unsigned i = 0; i < in.nLevels; i++) {
+      in.mg_levels[i] = (unsigned)std::stoul((*lv)[i]);
+      in.mg_steps[i] = (unsigned)std::stoul((*st)[i]);
+    }
+  }
   in.n_steps = f.uint("n_steps");
   copy_str(in.dt_scheme, sizeof(in.dt_scheme), f.str("dt_scheme"));
   set_stages(in);

@@ -74,6 +74,7 @@ struct KArgs {
   struct Rk {
     double *U;
     const double *Uini, *divF0;  // divF0: stage 0 of the [nStages][E][V][nS] residuals
+    const double *src;           // p-multigrid source term added to every divF_s, or null
     long total;
     double alpha, dt, beta[4];
     int nStages, stage, mode;
@@ -164,7 +165,8 @@ __device__ __forceinline__ double line_extrap(const double *lE, const double *bl
 // RK update of one spt straight after its residual, in the last kernel of the residual
 // (the stage update of src/solver.cpp:2897-2941 and the final combination of
 // :2997-3062, with the same arithmetic as k_rk_stage / k_rk_final).  acc holds this
-// stage's divF at the spt; the other stages' come from memory.
+// stage's divF at the spt; the other stages' come from memory.  With a source term
+// (src/solver.cpp:2946-2962, 3061-3073) every divF_s is replaced by divF_s + src.
 template <int D>
 __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc) {
   constexpr int V = D + 2, nM = D * D + 1;
@@ -174,11 +176,15 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
   for (int v = 0; v < V; v++) {
     const long i = (e * V + v) * nS + t;
     if (a.rk.mode == 1) {
-      a.rk.U[i] = a.rk.Uini[i] - a.rk.alpha * a.rk.dt / detJ * acc[v];
+      const double r = a.rk.src ? acc[v] + a.rk.src[i] : acc[v];
+      a.rk.U[i] = a.rk.Uini[i] - a.rk.alpha * a.rk.dt / detJ * r;
     } else {
       double u = a.rk.Uini[i];
-      for (int s = 0; s < a.rk.nStages; s++)
-        u -= a.rk.beta[s] * a.rk.dt / detJ * ((s == a.rk.stage) ? acc[v] : a.rk.divF0[s * a.rk.total + i]);
+      for (int s = 0; s < a.rk.nStages; s++) {
+        double r = (s == a.rk.stage) ? acc[v] : a.rk.divF0[s * a.rk.total + i];
+        if (a.rk.src) r += a.rk.src[i];
+        u -= a.rk.beta[s] * a.rk.dt / detJ * r;
+      }
       a.rk.U[i] = u;
     }
   }
@@ -774,10 +780,14 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restr
 #include "adjoint.hpp"
 
 // ---------------------------------------------------------------- RK + norm
-// src/solver.cpp:2897-2941 (stage), 2997-3062 (final), element-major layout.
+// src/solver.cpp:2897-2941 (stage), 2997-3062 (final), element-major layout.  src, when
+// not null, is the p-multigrid source term added to each divF_s (RK_update_source,
+// src/solver_kernels.cu:253-303, with the CPU path's arithmetic, src/solver.cpp:2946-2962,
+// 3061-3073).
 template <int D>
 __global__ __launch_bounds__(kMaxThreads) void k_rk_stage(double *__restrict__ U, const double *__restrict__ Uini,
                                                           const double *__restrict__ divF,
+                                                          const double *__restrict__ src,
                                                           const double *__restrict__ metric, long total, int V,
                                                           int nS, double alpha, double dt) {
   constexpr int nM = D * D + 1;
@@ -785,13 +795,15 @@ __global__ __launch_bounds__(kMaxThreads) void k_rk_stage(double *__restrict__ U
     const long e = i / (V * nS);
     const int s = (int)(i % nS);
     const double detJ = metric[(e * nM + D * D) * nS + s];
-    U[i] = Uini[i] - alpha * dt / detJ * divF[i];
+    const double r = src ? divF[i] + src[i] : divF[i];
+    U[i] = Uini[i] - alpha * dt / detJ * r;
   }
 }
 
 template <int D>
 __global__ __launch_bounds__(kMaxThreads) void k_rk_final(double *U, const double *Uini,  // may alias (Euler)
                                                           const double *__restrict__ divF,
+                                                          const double *__restrict__ src,
                                                           const double *__restrict__ metric, long total, int V,
                                                           int nS, double b0, double b1, double b2, double b3,
                                                           int nStages, double dt) {
@@ -802,9 +814,75 @@ __global__ __launch_bounds__(kMaxThreads) void k_rk_final(double *U, const doubl
     const int s = (int)(i % nS);
     const double detJ = metric[(e * nM + D * D) * nS + s];
     double u = Uini[i];
-    for (int st = 0; st < nStages; st++) u -= b[st] * dt / detJ * divF[st * total + i];
+    for (int st = 0; st < nStages; st++) {
+      const double r = src ? divF[st * total + i] + src[i] : divF[st * total + i];
+      u -= b[st] * dt / detJ * r;
+    }
     U[i] = u;
   }
+}
+
+// ---------------------------------------------------------------- p-multigrid
+// Transfer between orders (restrict_pmg, prolong_err, prolong_U; src/multigrid.cpp:335-472).
+// The reference's operator [nSpts_out][nSpts_in] is a tensor product of one 1D table A
+// [n_out][n_in] with itself (src/quads.cpp:363-465, src/hexas.cpp:586-704), so per element
+// and variable ("line") out = alpha (A x A [x A]) in + beta out is applied sum-factorised:
+// along x, then y, then z, with points x fastest.  A workgroup takes L consecutive lines
+// (one contiguous chunk of in and of out), staged in LDS and ping-ponged between passes.
+constexpr int kPmgBuf = 2048;  // doubles per LDS buffer: L * max(n_in, n_out)^D <= kPmgBuf
+constexpr int kPmgLds = 64 + 2 * kPmgBuf;
+
+template <int D>
+__global__ __launch_bounds__(kMaxThreads) void k_pmg_transfer(const double *__restrict__ in, double *__restrict__ out,
+                                                              const double *__restrict__ A, long lines, int ni,
+                                                              int no, int L, double alpha, double beta) {
+  extern __shared__ double lds[];  // kPmgLds doubles
+  double *lA = lds, *buf = lds + 64;
+  const long l0 = (long)blockIdx.x * L;
+  const int nl = (int)min((long)L, lines - l0);
+  int szi = 1, szo = 1;
+  for (int d = 0; d < D; d++) {
+    szi *= ni;
+    szo *= no;
+  }
+  for (int k = threadIdx.x; k < no * ni; k += blockDim.x) lA[k] = A[k];
+  const double *gin = in + l0 * szi;
+  for (int k = threadIdx.x; k < nl * szi; k += blockDim.x) buf[k] = gin[k];
+  __syncthreads();
+  double *src = buf, *dst = buf + kPmgBuf;
+  int inner = 1, outer = szi / ni;
+  for (int d = 0; d < D; d++) {
+    // src [nl][outer][ni][inner] -> dst [nl][outer][no][inner]
+    const int n = nl * outer * no * inner;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const int q = k % inner, o = (k / inner) % no, lo = k / (inner * no);
+      const double *s = src + lo * ni * inner + q;
+      double acc = 0.0;
+      for (int i = 0; i < ni; i++) acc += lA[o * ni + i] * s[i * inner];
+      if (d == D - 1) {  // outer == 1: k runs over [nl][no^D] in output order
+        double *g = out + l0 * szo + k;
+        *g = (beta == 0.0) ? alpha * acc : alpha * acc + beta * *g;
+      } else {
+        dst[k] = acc;
+      }
+    }
+    if (d < D - 1) {
+      __syncthreads();
+      double *t = src;
+      src = dst;
+      dst = t;
+      inner *= no;
+      outer /= ni;
+    }
+  }
+}
+
+// z = a x + b y elementwise (device_copy / device_add / device_subtract,
+// src/aux_kernels.cu:143-205, as one kernel).
+__global__ __launch_bounds__(kMaxThreads) void k_axpby(double *z, double a, const double *x, double b,
+                                                       const double *y, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    z[i] = a * x[i] + b * y[i];
 }
 
 __device__ __forceinline__ double wave_reduce(double x, int type) {
@@ -1060,6 +1138,7 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
   rk.U = r->U;
   rk.Uini = r->U_ini;
   rk.divF0 = r->divF0;
+  rk.src = r->source;
   rk.total = (long)d->nEles * d->nVars * d->nSpts;
   rk.alpha = r->alpha;
   rk.dt = r->dt;
@@ -1083,21 +1162,26 @@ int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, co
   return rk_fused(zk::Op::Div, d, U, divF, rk, (hipStream_t)stream);
 }
 
-int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF, double alpha,
-                 double dt, void *stream) {
+int zfr_rk_stage_source(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
+                        const double *source, double alpha, double dt, void *stream) {
   const long total = (long)d->nEles * d->nVars * d->nSpts;
   if (total <= 0) return 0;
   if (d->nDims == 2)
     hipLaunchKernelGGL(zk::k_rk_stage<2>, dim3(zk::grid_for(total)), dim3(zk::kMaxThreads), 0, (hipStream_t)stream,
-                       U, U_ini, divF, d->metric, total, d->nVars, d->nSpts, alpha, dt);
+                       U, U_ini, divF, source, d->metric, total, d->nVars, d->nSpts, alpha, dt);
   else
     hipLaunchKernelGGL(zk::k_rk_stage<3>, dim3(zk::grid_for(total)), dim3(zk::kMaxThreads), 0, (hipStream_t)stream,
-                       U, U_ini, divF, d->metric, total, d->nVars, d->nSpts, alpha, dt);
+                       U, U_ini, divF, source, d->metric, total, d->nVars, d->nSpts, alpha, dt);
   return zfr::check_hip(hipGetLastError(), "zfr_rk_stage");
 }
 
-int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF, const double *beta,
-                 int nStages, double dt, void *stream) {
+int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF, double alpha,
+                 double dt, void *stream) {
+  return zfr_rk_stage_source(d, U, U_ini, divF, nullptr, alpha, dt, stream);
+}
+
+int zfr_rk_final_source(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
+                        const double *source, const double *beta, int nStages, double dt, void *stream) {
   if (nStages < 1 || nStages > 4) return zfr::fail("zfr_rk_final: nStages must be 1..4");
   const long total = (long)d->nEles * d->nVars * d->nSpts;
   if (total <= 0) return 0;
@@ -1105,11 +1189,53 @@ int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const 
   for (int i = 0; i < nStages; i++) b[i] = beta[i];
   if (d->nDims == 2)
     hipLaunchKernelGGL(zk::k_rk_final<2>, dim3(zk::grid_for(total)), dim3(zk::kMaxThreads), 0, (hipStream_t)stream,
-                       U, U_ini, divF, d->metric, total, d->nVars, d->nSpts, b[0], b[1], b[2], b[3], nStages, dt);
+                       U, U_ini, divF, source, d->metric, total, d->nVars, d->nSpts, b[0], b[1], b[2], b[3],
+                       nStages, dt);
   else
     hipLaunchKernelGGL(zk::k_rk_final<3>, dim3(zk::grid_for(total)), dim3(zk::kMaxThreads), 0, (hipStream_t)stream,
-                       U, U_ini, divF, d->metric, total, d->nVars, d->nSpts, b[0], b[1], b[2], b[3], nStages, dt);
+                       U, U_ini, divF, source, d->metric, total, d->nVars, d->nSpts, b[0], b[1], b[2], b[3],
+                       nStages, dt);
   return zfr::check_hip(hipGetLastError(), "zfr_rk_final");
+}
+
+int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF, const double *beta,
+                 int nStages, double dt, void *stream) {
+  return zfr_rk_final_source(d, U, U_ini, divF, nullptr, beta, nStages, dt, stream);
+}
+
+int zfr_pmg_transfer(int nDims, int nVars, long nEles, int n_in, int n_out, const double *A, double alpha,
+                     const double *in, double beta, double *out, void *stream) {
+  if (nDims != 2 && nDims != 3) return zfr::fail("zfr_pmg_transfer: nDims must be 2 or 3");
+  if (n_in < 1 || n_in > 6 || n_out < 1 || n_out > 6)
+    return zfr::fail("zfr_pmg_transfer: points per direction must be in 1..6 (orders 0..5)");
+  if (nVars < 1 || nEles < 0) return zfr::fail("zfr_pmg_transfer: bad sizes");
+  if (!A || !in || !out) return zfr::fail("zfr_pmg_transfer: missing buffer");
+  const long lines = nEles * nVars;
+  if (lines == 0) return 0;
+  const int M = n_in > n_out ? n_in : n_out;
+  int cube = 1;
+  for (int d = 0; d < nDims; d++) cube *= M;
+  int L = zk::kPmgBuf / cube;
+  if (L > zk::kMaxThreads) L = zk::kMaxThreads;
+  const long blocks = (lines + L - 1) / L;
+  if (blocks > 0x7fffffffL) return zfr::fail("zfr_pmg_transfer: too many elements for one launch");
+  if (nDims == 2)
+    hipLaunchKernelGGL(zk::k_pmg_transfer<2>, dim3((unsigned)blocks), dim3(zk::kMaxThreads),
+                       zk::kPmgLds * sizeof(double), (hipStream_t)stream,
+                       in, out, A, lines, n_in, n_out, L, alpha, beta);
+  else
+    hipLaunchKernelGGL(zk::k_pmg_transfer<3>, dim3((unsigned)blocks), dim3(zk::kMaxThreads),
+                       zk::kPmgLds * sizeof(double), (hipStream_t)stream,
+                       in, out, A, lines, n_in, n_out, L, alpha, beta);
+  return zfr::check_hip(hipGetLastError(), "zfr_pmg_transfer");
+}
+
+int zfr_axpby(long n, double a, const double *x, double b, const double *y, double *z, void *stream) {
+  if (n <= 0) return 0;
+  if (!x || !y || !z) return zfr::fail("zfr_axpby: missing buffer");
+  hipLaunchKernelGGL(zk::k_axpby, dim3(zk::grid_for(n)), dim3(zk::kMaxThreads), 0, (hipStream_t)stream, z, a, x, b,
+                     y, n);
+  return zfr::check_hip(hipGetLastError(), "zfr_axpby");
 }
 
 int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work, double *res_out,

@@ -1,0 +1,232 @@
+// multigrid.cpp -- p-multigrid (PMGrid) on quad / hex meshes.
+//
+// The cycle is the reference's (src/multigrid.cpp:18-518): coarse FRSolvers at the orders
+// of mg_levels on the same mesh; restriction of the solution and of the stage-0 residual;
+// a source term that makes each coarse problem reproduce the restricted fine residual at
+// the restricted state; explicit RK smoothing under that source; and prolongation of each
+// coarse level's change back to the next finer level.  Every level runs on the fine
+// solver's stream, so the whole cycle is one ordered sequence of launches.  The operator
+// products go through zfr_pmg_transfer (the 1D interpolation table applied per direction,
+// where the reference multiplies by the dense tensor operator), the vector updates through
+// zfr_axpby, the smoothing through the solvers' own (RK-fused) steps.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "host.hpp"
+
+namespace zfr {
+
+namespace {
+
+// Interpolation between Gauss-Legendre point sets: A[i][j] = l_j(x_i), with l_j the
+// Lagrange polynomials on n_from points and x_i the n_to points.
+std::vector<double> interp_1d(int n_from, int n_to) {
+  const auto xf = gauss_legendre_points((unsigned)n_from), xt = gauss_legendre_points((unsigned)n_to);
+  std::vector<double> A((size_t)n_to * n_from);
+  for (int i = 0; i < n_to; i++)
+    for (int j = 0; j < n_from; j++) A[(size_t)i * n_from + j] = lagrange(xf, (unsigned)j, xt[i]);
+  return A;
+}
+
+// C[m][n] = A[m][k] B[k][n], row-major, summed in increasing k.
+std::vector<double> matmul(const std::vector<double> &A, const std::vector<double> &B, int m, int k, int n) {
+  std::vector<double> C((size_t)m * n, 0.0);
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < n; j++) {
+      double s = 0.0;
+      for (int l = 0; l < k; l++) s += A[(size_t)i * k + l] * B[(size_t)l * n + j];
+      C[(size_t)i * n + j] = s;
+    }
+  return C;
+}
+
+}  // namespace
+
+// Restriction from order p to order r < p: the single-order steps (order P+1 -> P, the
+// order-(P+1) interpolant evaluated at the order-P points) multiplied as
+// ((I[r<-r+1] I[r+1<-r+2]) I[r+2<-r+3]) ... I[p-1<-p], the association of
+// src/quads.cpp:416-457.  The composition is not the direct interpolation from p to r.
+std::vector<double> pmg_restriction_1d(int p, int r) {
+  if (r < 0 || r >= p || p > 5) raise("p-multigrid restriction needs 0 <= coarse order < fine order <= 5");
+  std::vector<double> R = interp_1d(r + 2, r + 1);
+  for (int P = r + 1; P < p; P++) R = matmul(R, interp_1d(P + 2, P + 1), r + 1, P + 1, P + 2);
+  return R;  // [r+1][p+1]
+}
+
+// Prolongation from order p to order q > p: ((I[q<-q-1] I[q-1<-q-2]) ...) I[p+1<-p]
+// (src/quads.cpp:373-414).  Interpolation reproduces polynomials, so this equals the direct
+// interpolation from p to q up to rounding.
+std::vector<double> pmg_prolongation_1d(int p, int q) {
+  if (p < 0 || q <= p || q > 5) raise("p-multigrid prolongation needs 0 <= coarse order < fine order <= 5");
+  std::vector<double> Pm = interp_1d(q, q + 1);
+  for (int P = q - 1; P > p; P--) Pm = matmul(Pm, interp_1d(P, P + 1), q + 1, P + 1, P);
+  return Pm;  // [q+1][p+1]
+}
+
+// PMGrid::setup (src/multigrid.cpp:18-103).
+void PMGrid::setup(zefr_input_t *in, FRSolver &solver) {
+  input = in;
+  order = solver.order;
+  nLevels = (int)in->nLevels;
+  if (nLevels == 0) raise("Need to provide mg_levels and/or mg_steps to run multigrid!");
+  if ((int)in->mg_levels[0] != order) raise("Invalid mg_levels provided. First level must equal order!");
+  // the reference reads mg_levels[1] unconditionally and builds empty operators for a
+  // non-decreasing list; both are rejected here
+  if (nLevels < 2) raise("mg_levels needs at least two levels");
+  for (int n = 1; n < nLevels; n++)
+    if (in->mg_levels[n] >= in->mg_levels[n - 1]) raise("mg_levels must decrease strictly from the fine order");
+  grids.clear();
+  grids.emplace_back();  // level 0 is the solver itself
+  oppRes.assign(nLevels, {});
+  oppPro.assign(nLevels, {});
+  sourcesBT.clear();
+  solutionsBT.clear();
+  correctionsBT.clear();
+  oppRes_d.clear();
+  oppPro_d.clear();
+  const bool device = solver.device_ready;
+  oppRes[0] = pmg_restriction_1d(order, (int)in->mg_levels[1]);
+  for (int n = 1; n < nLevels; n++) {
+    const int P = (int)in->mg_levels[n], P_pro = (int)in->mg_levels[n - 1];
+    auto g = std::make_shared<FRSolver>(in, P);
+    g->own_stream = false;
+    g->stream = solver.stream;
+    g->setup_host();
+    if (device) g->setup_device();
+    grids.push_back(g);
+    oppPro[n] = pmg_prolongation_1d(P, P_pro);
+    if (n + 1 < nLevels) oppRes[n] = pmg_restriction_1d(P, (int)in->mg_levels[n + 1]);
+  }
+  for (int n = 0; n < nLevels; n++) {
+    oppRes_d.emplace_back();
+    oppPro_d.emplace_back();
+    sourcesBT.emplace_back();
+    solutionsBT.emplace_back();
+    correctionsBT.emplace_back();
+  }
+  if (!device) return;
+  auto upload = [&](DevBuf &b, const std::vector<double> &v) {
+    b.alloc(v.size() * sizeof(double));
+    if (!v.empty()) hip_or_throw(hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+  };
+  for (int n = 0; n < nLevels; n++) {
+    upload(oppRes_d[n], oppRes[n]);
+    upload(oppPro_d[n], oppPro[n]);
+    if (n == 0) continue;
+    const size_t bytes = grids[n]->eles->state_size() * sizeof(double);
+    for (auto *b : {&sourcesBT[n], &solutionsBT[n], &correctionsBT[n]}) {
+      b->alloc(bytes);
+      hip_or_throw(hipMemsetAsync(b->p, 0, bytes, solver.stream), "hipMemsetAsync");
+    }
+  }
+  hip_or_throw(hipStreamSynchronize(solver.stream), "hipStreamSynchronize");
+}
+
+void PMGrid::transfer(const FRSolver &from, const FRSolver &to, const DevBuf &A, double alpha, const double *in,
+                      double beta, double *out) {
+  const Elements &ef = *from.eles;
+  if (!A.p) raise("p-multigrid: transfer table missing for this pair of levels");
+  status_or_throw(zfr_pmg_transfer(ef.nDims, ef.nVars, ef.nEles, from.order + 1, to.order + 1, A.as<double>(),
+                                   alpha, in, beta, out, from.stream));
+}
+
+static int level_of(const std::vector<std::shared_ptr<FRSolver>> &grids, const FRSolver &g) {
+  for (size_t n = 1; n < grids.size(); n++)
+    if (grids[n].get() == &g) return (int)n;
+  return 0;
+}
+
+// PMGrid::restrict_pmg (src/multigrid.cpp:335-392): solution and stage-0 residual.
+void PMGrid::restrict_pmg(FRSolver &grid_f, FRSolver &grid_c) {
+  const DevBuf &R = oppRes_d[level_of(grids, grid_f)];
+  transfer(grid_f, grid_c, R, 1.0, grid_f.eles->U_spts_d.as<double>(), 0.0, grid_c.eles->U_spts_d.as<double>());
+  transfer(grid_f, grid_c, R, 1.0, grid_f.eles->divF_d.as<double>(), 0.0, grid_c.eles->divF_d.as<double>());
+}
+
+// PMGrid::prolong_err (src/multigrid.cpp:394-433): U_f += rel_fac * P correction_c.
+void PMGrid::prolong_err(FRSolver &grid_c, const double *correction, FRSolver &grid_f) {
+  transfer(grid_c, grid_f, oppPro_d[level_of(grids, grid_c)], input->rel_fac, correction, 1.0,
+           grid_f.eles->U_spts_d.as<double>());
+}
+
+// PMGrid::prolong_U (src/multigrid.cpp:435-472): U_f = P U_c.
+void PMGrid::prolong_U(FRSolver &grid_c, FRSolver &grid_f) {
+  transfer(grid_c, grid_f, oppPro_d[level_of(grids, grid_c)], 1.0, grid_c.eles->U_spts_d.as<double>(), 0.0,
+           grid_f.eles->U_spts_d.as<double>());
+}
+
+// PMGrid::compute_source_term (src/multigrid.cpp:474-518): source = restricted residual
+// minus the coarse residual at the restricted state.
+void PMGrid::compute_source_term(FRSolver &grid, double *source) {
+  Elements &e = *grid.eles;
+  const size_t n = e.state_size();
+  hip_or_throw(hipMemcpyAsync(source, e.divF_d.p, n * sizeof(double), hipMemcpyDeviceToDevice, grid.stream),
+               "hipMemcpyAsync");
+  grid.compute_residual(0);
+  status_or_throw(zfr_axpby((long)n, 1.0, source, -1.0, e.divF_d.as<double>(), source, grid.stream));
+}
+
+// PMGrid::v_cycle (src/multigrid.cpp:105-261) from `level` (0 = the fine solver) down.
+void PMGrid::v_cycle(FRSolver &solver, int lvl) {
+  if (lvl != nLevels - 1) {
+    solver.compute_residual(0);
+    restrict_pmg(solver, *grids[lvl + 1]);
+  }
+  for (int n = lvl + 1; n < nLevels; n++) {
+    FRSolver &g = *grids[n];
+    Elements &e = *g.eles;
+    const size_t bytes = e.state_size() * sizeof(double);
+    double *src = sourcesBT[n].as<double>();
+    compute_source_term(g, src);
+    hip_or_throw(hipMemcpyAsync(solutionsBT[n].p, e.U_spts_d.p, bytes, hipMemcpyDeviceToDevice, g.stream),
+                 "hipMemcpyAsync");
+    for (unsigned s = 0; s < input->mg_steps[n]; s++) g.update(src);
+    if (n + 1 < nLevels) {
+      g.compute_residual(0);
+      status_or_throw(zfr_axpby((long)e.state_size(), 1.0, e.divF_d.as<double>(), 1.0, src, e.divF_d.as<double>(),
+                                g.stream));
+      restrict_pmg(g, *grids[n + 1]);
+    }
+  }
+  for (int n = nLevels - 1; n > lvl; n--) {
+    FRSolver &g = *grids[n];
+    Elements &e = *g.eles;
+    if (n != nLevels - 1)
+      for (unsigned s = 0; s < input->mg_steps[n]; s++) g.update(sourcesBT[n].as<double>());
+    // correction = U - stored solution (the CPU path's form; the GPU path of the
+    // reference leaves the difference in U as well, which nothing reads afterwards)
+    status_or_throw(zfr_axpby((long)e.state_size(), 1.0, e.U_spts_d.as<double>(), -1.0,
+                              solutionsBT[n].as<double>(), correctionsBT[n].as<double>(), g.stream));
+    if (n > lvl + 1) prolong_err(g, correctionsBT[n].as<double>(), *grids[n - 1]);
+  }
+  if (lvl != nLevels - 1) prolong_err(*grids[lvl + 1], correctionsBT[lvl + 1].as<double>(), solver);
+}
+
+// PMGrid::cycle (src/multigrid.cpp:263-333).  No history or solution files are written.
+void PMGrid::cycle(FRSolver &solver) {
+  if (!solver.device_ready) raise("solver is not set up on the device (call setup_solver)");
+  const std::string type = input->mg_cycle;
+  if (type == "V") {
+    for (unsigned s = 0; s < input->mg_steps[0]; s++) solver.update();
+    v_cycle(solver, 0);
+  } else if (type == "FMG") {
+    for (int n = nLevels - 1; n > 0; n--) {
+      for (unsigned c = 0; c < input->FMG_vcycles; c++) {
+        grids[n]->update();
+        v_cycle(*grids[n], n);
+      }
+      grids[n]->update();
+      if (n > 1) prolong_U(*grids[n], *grids[n - 1]);
+    }
+    prolong_U(*grids[1], solver);
+    std::memset(input->mg_cycle, 0, sizeof(input->mg_cycle));
+    std::strcpy(input->mg_cycle, "V");
+    cycle(solver);
+  } else {
+    raise("Multigrid cycle type unknown!");
+  }
+}
+
+}  // namespace zfr

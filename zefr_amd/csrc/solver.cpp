@@ -1,4 +1,4 @@
-// solver.cpp -- Elements / Faces / FRSolver / PMGrid host classes.
+// solver.cpp -- Elements / Faces / FRSolver host classes.
 //
 // Setup follows the reference's sequence (src/solver.cpp:55-180): read the mesh, build
 // connectivity, compute geometry and metrics, set the initial state, move everything to
@@ -290,7 +290,7 @@ FRSolver::FRSolver(zefr_input_t *in, int ord) : input(in), order(ord < 0 ? (int)
 
 FRSolver::~FRSolver() {
   drop_graph();
-  if (stream) (void)hipStreamDestroy(stream);
+  if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
 namespace {
@@ -444,8 +444,9 @@ static const double kRkBeta[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};     // :7
 void FRSolver::rk_stage_update(int stage) {
   const auto d = eles->desc(*input);
   const size_t st = eles->state_size();
-  status_or_throw(zfr_rk_stage(&d, eles->U_spts_d.as<double>(), eles->U_ini_d.as<double>(),
-                               eles->divF_d.as<double>() + (size_t)stage * st, kRkAlpha[stage], input->dt, stream));
+  status_or_throw(zfr_rk_stage_source(&d, eles->U_spts_d.as<double>(), eles->U_ini_d.as<double>(),
+                                      eles->divF_d.as<double>() + (size_t)stage * st, rk_source, kRkAlpha[stage],
+                                      input->dt, stream));
 }
 
 void FRSolver::rk_final_update() {
@@ -454,8 +455,8 @@ void FRSolver::rk_final_update() {
   const double one = 1.0;
   const double *beta = (ns > 1) ? kRkBeta : &one;
   const double *Uini = (ns > 1) ? eles->U_ini_d.as<double>() : eles->U_spts_d.as<double>();
-  status_or_throw(zfr_rk_final(&d, eles->U_spts_d.as<double>(), Uini, eles->divF_d.as<double>(), beta, ns,
-                               input->dt, stream));
+  status_or_throw(zfr_rk_final_source(&d, eles->U_spts_d.as<double>(), Uini, eles->divF_d.as<double>(), rk_source,
+                                      beta, ns, input->dt, stream));
 }
 
 // compute_residual_finish with the RK update of this stage done by the same kernel.
@@ -470,6 +471,7 @@ void FRSolver::finish_with_rk(int stage) {
   r.stage = stage;
   r.nStages = ns;
   r.dt = input->dt;
+  r.source = rk_source;
   if (stage < ns - 1) {
     r.mode = 1;
     r.alpha = kRkAlpha[stage];
@@ -522,7 +524,8 @@ void FRSolver::step_RK() {
     step_RK_eager();
     return;
   }
-  if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile || graph_fused != fuse_rk) {
+  if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile || graph_fused != fuse_rk ||
+      graph_source != rk_source) {
     drop_graph();
     hip_or_throw(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
@@ -538,13 +541,23 @@ void FRSolver::step_RK() {
     graph_dt = input->dt;
     graph_tile = eles->tile;
     graph_fused = fuse_rk;
+    graph_source = rk_source;
   }
   hip_or_throw(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
 }
 
-void FRSolver::update() {
+void FRSolver::update() { update(nullptr); }
+
+void FRSolver::update(const double *source) {
   prev_time = flow_time;
-  step_RK();
+  rk_source = source;
+  try {
+    step_RK();
+  } catch (...) {
+    rk_source = nullptr;
+    throw;
+  }
+  rk_source = nullptr;
   flow_time = prev_time + input->dt;
   current_iter++;
 }
@@ -663,8 +676,5 @@ void FRSolver::time_kernels(int reps, double *ms_out) {
   for (int k = 0; k < 3; k++) ms_out[k] = acc[k] / reps;
 }
 
-// ---------------------------------------------------------------- PMGrid
-void PMGrid::setup(zefr_input_t *, FRSolver &) { raise("p-multigrid is not implemented in this build"); }
-void PMGrid::cycle(FRSolver &) { raise("p-multigrid is not implemented in this build"); }
 
 }  // namespace zfr

@@ -29,7 +29,8 @@ __all__ = ["InputStruct", "DataStruct", "Zefr", "initialize", "get_zefr_object",
            "finalize", "use_gpus", "get_q_spts", "get_dq_spts", "get_q_spts_d", "get_dq_spts_d",
            "get_q_spt"]
 
-_STR_FIELDS = {"dt_scheme", "meshfile", "output_prefix", "spt_type"}
+_STR_FIELDS = {"dt_scheme", "meshfile", "output_prefix", "spt_type", "mg_cycle"}
+_LEVEL_FIELDS = ("mg_levels", "mg_steps")
 
 
 class InputStruct:
@@ -48,6 +49,8 @@ class InputStruct:
             return v.decode()
         if name in ("V_fs", "norm_fs"):
             return list(v)
+        if name in _LEVEL_FIELDS:
+            return list(v)[:s.nLevels]
         return v
 
     def __setattr__(self, name, value):
@@ -58,6 +61,12 @@ class InputStruct:
             arr = getattr(s, name)
             for i, x in enumerate(value):
                 arr[i] = x
+            return
+        elif name in _LEVEL_FIELDS:
+            arr = getattr(s, name)
+            for i, x in enumerate(value):
+                arr[i] = x
+            s.nLevels = len(value)
             return
         elif name == "meshBounds":
             s.nMeshBounds = len(value)
@@ -277,6 +286,22 @@ class Zefr:
         a = np.ascontiguousarray(U)
         assert a.shape == (d["nEles"], d["nVars"], d["nSpts"])
         self._copy(0, 0, a, True)
+
+    def pmg_state(self, level, which="U"):
+        """A p-multigrid level's array in the device layout [nEles][nVars][nSpts(level)]:
+        which = 'U', 'divF' (stage 0), 'source', 'solution' or 'correction'."""
+        d = self.dims()
+        inp = self.get_input()
+        nS = (inp.mg_levels[level] + 1) ** d["nDims"]
+        a = np.empty((d["nEles"], d["nVars"], nS))
+        code = {"U": 0, "divF": 2, "source": 3, "solution": 4, "correction": 5}[which]
+        check(lib().zefr_pmg_copy_state(self._h, level, code, a.ctypes.data_as(C.POINTER(C.c_double)), a.size, 0))
+        return a
+
+    def set_pmg_state(self, level, U):
+        """Overwrite a p-multigrid level's U from the device layout [nEles][nVars][nSpts(level)]."""
+        a = np.ascontiguousarray(np.asarray(U, dtype=np.float64))
+        check(lib().zefr_pmg_copy_state(self._h, level, 0, a.ctypes.data_as(C.POINTER(C.c_double)), a.size, 1))
 
     def fpt_maps(self):
         d = self.dims()
