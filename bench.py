@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-vjp", action="store_true", help="skip the VJP timing")
     p.add_argument("--no-steps", action="store_true", help="skip the RK4 step timings")
+    p.add_argument("--no-pmg", action="store_true", help="skip the p-multigrid timing")
     p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
     p.add_argument("--cpu-steps", type=int, default=150)
     p.add_argument("--workdir", default=None)
@@ -222,6 +223,122 @@ output_prefix vortex
     return res
 
 
+def pmg_timing(workdir, cycles=10):
+    """p-multigrid on the quad half of the steady case: 128^2 periodic quads on [0,1]^2, p=4,
+    Euler, levels 4-3-2-1-0 with one RK44 smoothing step per level and V-cycles (the
+    reference's mixed tri/quad variant is outside this build).  Mean wall time of one
+    V-cycle against one fine RK44 step, and the restriction kernel's bandwidth (HIP events,
+    fine U -> order 3, bytes = elements x vars x (25 + 16) x 8)."""
+    import numpy as np
+    import torch
+    from zefr_amd import zefr
+    from zefr_amd._lib import lib
+    from zefr_amd.mesh import write_box
+    mesh = write_box(os.path.join(workdir, "quad128.msh"), (128, 128), (0.0, 0.0), (1.0, 1.0))
+    deck = os.path.join(workdir, "pmg.deck")
+    with open(deck, "w") as f:
+        f.write(f"""nDims 2
+meshfile {mesh}
+mesh_bound periodic periodic
+mesh_bound fluid fluid
+order 4
+equation EulerNS
+viscous 0
+n_steps 10
+dt_scheme RK44
+dt 1e-4
+dt_type 0
+ic_type 0
+fconv_type Rusanov
+rus_k 0.0
+gamma 1.4
+disable_nondim 1
+rho_fs 1.0
+P_fs 1.0
+mach_fs 0.5
+p_multi 1
+mg_levels 4 3 2 1 0
+mg_steps 1 1 1 1 1
+mg_cycle V
+output_prefix pmg
+""")
+    z = zefr.Zefr()
+    z.read_input(deck)
+    z.setup_solver()
+    d = z.dims()
+    E, V, nS = d["nEles"], d["nVars"], d["nSpts"]
+    rng = np.random.default_rng(0)
+    U0 = z.state("U", layout="device")
+    U0 = U0 * (1.0 + 0.05 * rng.standard_normal(U0.shape))
+    z.set_state(U0, layout="device")
+    z.do_step()
+    z.synchronize()
+    t = time.perf_counter()
+    for _ in range(cycles):
+        z.do_step()
+    z.synchronize()
+    cyc_ms = (time.perf_counter() - t) / cycles * 1e3
+    # one fine RK44 step on the same mesh (a solver without multigrid)
+    z2 = zefr.Zefr()
+    z2.read_input(deck)
+    z2.get_input().p_multi = 0
+    z2.setup_solver()
+    z2.set_state(U0, layout="device")
+    z2.do_step()
+    z2.synchronize()
+    t = time.perf_counter()
+    for _ in range(cycles):
+        z2.do_step()
+    z2.synchronize()
+    step_ms = (time.perf_counter() - t) / cycles * 1e3
+    # restriction kernel 4 -> 3 on the fine state
+    from zefr_amd.zefr import check
+    A1 = torch.from_numpy(_restrict_table(4, 3)).cuda()
+    x = torch.from_numpy(U0).cuda()
+    y = torch.empty((E, V, 16), dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream()
+    reps = 50
+    for _ in range(3):
+        check(lib().zfr_pmg_transfer(2, V, E, 5, 4, C.c_void_p(A1.data_ptr()), 1.0, C.c_void_p(x.data_ptr()), 0.0,
+                                     C.c_void_p(y.data_ptr()), C.c_void_p(st.cuda_stream)))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        lib().zfr_pmg_transfer(2, V, E, 5, 4, C.c_void_p(A1.data_ptr()), 1.0, C.c_void_p(x.data_ptr()), 0.0,
+                               C.c_void_p(y.data_ptr()), C.c_void_p(st.cuda_stream))
+    e1.record(st)
+    e1.synchronize()
+    kms = e0.elapsed_time(e1) / reps
+    kbytes = E * V * (nS + 16) * 8
+    del z, z2
+    return {"workload": "p-multigrid V-cycle, 128^2 periodic quads on [0,1]^2, p=4, Euler, levels 4-3-2-1-0, "
+                        "one RK44 step per level",
+            "vcycle_ms": round(cyc_ms, 4), "fine_rk44_step_ms": round(step_ms, 4),
+            "vcycle_over_fine_step": round(cyc_ms / step_ms, 3),
+            "restrict_kernel": {"name": "k_pmg_transfer<2>", "avg_ms": round(kms, 5), "bytes": kbytes,
+                                "achieved_GBs": round(kbytes / (kms * 1e-3) / 1e9, 1)}}
+
+
+def _restrict_table(p, r):
+    """1D restriction table [r+1][p+1] between Gauss-Legendre point sets (order p -> r),
+    chained through the intermediate orders as the solver builds it."""
+    import numpy as np
+
+    def step(nf, nt):
+        xf, _ = np.polynomial.legendre.leggauss(nf)
+        xt, _ = np.polynomial.legendre.leggauss(nt)
+        A = np.ones((nt, nf))
+        for j in range(nf):
+            for m in range(nf):
+                if m != j:
+                    A[:, j] *= (xt - xf[m]) / (xf[j] - xf[m])
+        return A
+    R = step(r + 2, r + 1)
+    for P in range(r + 1, p):
+        R = R @ step(P + 2, P + 1)
+    return np.ascontiguousarray(R)
+
+
 def vjp_timing(z, reps):
     """The residual's vector-Jacobian product on the same mesh and state: mean time of
     `reps` back-to-back calls (HIP events on the torch stream both run on) against the
@@ -341,6 +458,7 @@ def main():
     rk = None
     if not args.no_steps:
         rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
+    pmg = None if args.no_pmg else pmg_timing(workdir)
 
     rec = {
         "metric": METRIC,
@@ -371,6 +489,7 @@ def main():
         "kernels": per_kernel,
         "vjp": vjp,
         "rk4_step": rk,
+        "pmg": pmg,
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(workdir, args.cpu_n, order, args.cpu_steps)
