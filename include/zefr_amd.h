@@ -70,6 +70,13 @@ typedef struct zefr_input {
   char mg_cycle[8];                           /* "V" or "FMG"                            */
   unsigned int FMG_vcycles;
   double rel_fac;                             /* prolongation relaxation factor          */
+  /* solution files (include/inputstruct.hpp:115-118; src/input.cpp:130-147) */
+  int restart;                                /* restart_solution reads a file           */
+  unsigned int restart_type;                  /* 2 = PyFR .pyfrs (1 = ParaView: rejected) */
+  unsigned int restart_iter;
+  char restart_case[256];                     /* reads <case>/<case>-<iter>.pyfrs        */
+  int write_pyfr;                             /* write_solution writes .pyfrs            */
+  int write_paraview;                         /* read for compatibility; not written     */
 } zefr_input_t;
 
 typedef struct zefr_s zefr_t;
@@ -183,6 +190,16 @@ int zefr_residual_vjp(zefr_t *z, const double *U, const double *w, double *v, vo
  * (1..nLevels-1; level 0 is the solver itself, see zefr_copy_state): which = 0 U_spts,
  * 2 divF(0), 3 source term, 4 stored solution, 5 correction.  n = capacity in doubles. */
 int zefr_pmg_copy_state(zefr_t *z, int level, int which, double *host, long n, int to_device);
+/* Zefr::write_solution (src/zefr.cpp:445-453): with write_pyfr, the current solution as a
+ * PyFR file <output_prefix>/<output_prefix>-<iter>.pyfrs (FRSolver::write_solution_pyfr,
+ * src/solver.cpp:4275-4668; HDF5 loaded at run time, ZEFR_HDF5_LIBRARY overrides the
+ * search).  ParaView / surface plot output is not written. */
+int zefr_write_solution(zefr_t *z);
+/* Zefr::restart_solution (src/zefr.cpp:366-370): with restart set, read
+ * <restart_case>/<restart_case>-<restart_iter>.pyfrs (FRSolver::restart_pyfr,
+ * src/solver.cpp:4670-4837): mesh_uuid check, tcurr and iteration, the solution
+ * (interpolated when written at another order).  restart_type must be 2. */
+int zefr_restart_solution(zefr_t *z);
 
 /* ---------------------------------------------------------------- singleton ------ */
 

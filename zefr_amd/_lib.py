@@ -44,6 +44,8 @@ class ZefrInput(C.Structure):
         ("p_multi", C.c_int), ("nLevels", C.c_uint), ("mg_levels", C.c_uint * MAX_LEVELS),
         ("mg_steps", C.c_uint * MAX_LEVELS), ("mg_cycle", C.c_char * 8), ("FMG_vcycles", C.c_uint),
         ("rel_fac", C.c_double),
+        ("restart", C.c_int), ("restart_type", C.c_uint), ("restart_iter", C.c_uint),
+        ("restart_case", C.c_char * 256), ("write_pyfr", C.c_int), ("write_paraview", C.c_int),
     ]
 
 
@@ -114,6 +116,8 @@ SIGNATURES = {
     "zefr_residual_at": (_I, [_P, _P, _P, _P]),
     "zefr_residual_vjp": (_I, [_P, _P, _P, _P, _P]),
     "zefr_pmg_copy_state": (_I, [_P, _I, _I, _DP, C.c_long, _I]),
+    "zefr_write_solution": (_I, [_P]),
+    "zefr_restart_solution": (_I, [_P]),
     "zefr_initialize": (_I, [_CP]),
     "zefr_get_zefr_object": (_P, []),
     "zefr_set_zefr_object": (None, [_P]),

@@ -322,6 +322,18 @@ int zefr_pmg_copy_state(zefr_t *z, int level, int which, double *host, long n, i
   });
 }
 
+int zefr_write_solution(zefr_t *z) {
+  return zfr::guard([&] {
+    if (z->input.write_pyfr) z->s().write_solution_pyfr(z->input.output_prefix);
+  });
+}
+
+int zefr_restart_solution(zefr_t *z) {
+  return zfr::guard([&] {
+    if (z->input.restart) z->s().restart_solution();
+  });
+}
+
 int zefr_get_fpt_maps(zefr_t *z, int *fpt2gfpt, signed char *slot) {
   return zfr::guard([&] {
     auto &c = z->s().conn;

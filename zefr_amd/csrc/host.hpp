@@ -187,6 +187,10 @@ class FRSolver {
   // source term [E][V][nS] added to every stage residual (nullptr: plain update)
   void update(const double *source);
   bool own_stream = true;  // false for p-multigrid levels, which run on the fine stream
+  // PyFR-format solution files (src/solver.cpp:4275-4837): write returns the file name
+  std::string write_solution_pyfr(const std::string &prefix);
+  void restart_pyfr(const std::string &restart_case, unsigned restart_iter);
+  void restart_solution();  // src/solver.cpp:212-222, restart_type 2 only
 
  private:
   void rk_stage_update(int stage);

@@ -162,6 +162,13 @@ void read_input_file(const std::string &file, zefr_input_t &in) {
   if (f.uint("motion", 0)) raise("moving grids are not supported");
   if (f.uint("filt_on", 0)) raise("filtering is not supported");
   if (f.uint("adapt_dt", 0)) raise("adaptive time stepping is not supported");
+  // solution files (src/input.cpp:130-147)
+  in.restart = f.uint("restart", 0) != 0;
+  copy_str(in.restart_case, sizeof(in.restart_case), f.str("restart_case", ""));
+  in.restart_type = f.uint("restart_type", 0);
+  in.restart_iter = f.uint("restart_iter", 0);
+  in.write_pyfr = f.uint("write_pyfr", 0) != 0;
+  in.write_paraview = f.uint("write_paraview", 1) != 0;
   // p-multigrid (src/input.cpp:137-142); the level checks of PMGrid::setup
   // (src/multigrid.cpp:24-29) are made here, where the lists are read
   in.p_multi = f.uint("p_multi", 0) != 0;

@@ -546,6 +546,13 @@ void FRSolver::step_RK() {
   hip_or_throw(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
 }
 
+// FRSolver::restart_solution (src/solver.cpp:212-222) for PyFR files.
+void FRSolver::restart_solution() {
+  if (input->restart_type == 2) restart_pyfr(input->restart_case, input->restart_iter);
+  else if (input->restart_type == 1) raise("ParaView restart files are not supported; use restart_type 2 (PyFR)");
+  else raise("restart_type must be 2 (PyFR .pyfrs files)");
+}
+
 void FRSolver::update() { update(nullptr); }
 
 void FRSolver::update(const double *source) {

@@ -29,7 +29,7 @@ __all__ = ["InputStruct", "DataStruct", "Zefr", "initialize", "get_zefr_object",
            "finalize", "use_gpus", "get_q_spts", "get_dq_spts", "get_q_spts_d", "get_dq_spts_d",
            "get_q_spt"]
 
-_STR_FIELDS = {"dt_scheme", "meshfile", "output_prefix", "spt_type", "mg_cycle"}
+_STR_FIELDS = {"dt_scheme", "meshfile", "output_prefix", "spt_type", "mg_cycle", "restart_case"}
 _LEVEL_FIELDS = ("mg_levels", "mg_steps")
 
 
@@ -154,6 +154,15 @@ class Zefr:
         return r
 
     # -- FRSolver-level calls
+    def write_solution(self):
+        """Zefr::write_solution: with write_pyfr, <output_prefix>/<output_prefix>-<iter>.pyfrs."""
+        check(lib().zefr_write_solution(self._h))
+
+    def restart_solution(self):
+        """Zefr::restart_solution: with restart (restart_type 2), read
+        <restart_case>/<restart_case>-<restart_iter>.pyfrs into the solver."""
+        check(lib().zefr_restart_solution(self._h))
+
     def compute_residual(self, stage=0):
         check(lib().zefr_compute_residual(self._h, stage))
 
