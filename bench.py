@@ -92,7 +92,7 @@ def kernel_bytes(D, N):
     LDG beta = +-1/2 (every input read once, every output written once; FP64 values,
     uint32 partner map).  Each face has exactly one owner side, so an element owns nF/2
     flux points on average.
-      extrapolate   : U (nS V) -> Utr (nF V)
+      extrapolate   : U (nS V), nbr (nF) -> Utr at the partner-owned fpts (nF/2 V)
       gradient_flux : U, partner Utr at owned fpts (nF/2 V), adjugate + detJ (nS (D^2+1)),
                       owned normals (nF/2 D), nbr (nF) -> Ftr at owned fpts (nF/2 V), divF (nS V)
       face_correction: divF (nS V) in/out, owner Ftr (nF V), signed dA (nF), nbr (nF)"""
@@ -100,7 +100,7 @@ def kernel_bytes(D, N):
     nS = N ** D
     nF = 2 * D * N ** (D - 1)
     h = nF // 2
-    extrap = 8 * (nS * V + nF * V)
+    extrap = 8 * (nS * V + h * V) + 4 * nF
     gflux = 8 * (nS * V + h * V + nS * (D * D + 1) + h * D + h * V + nS * V) + 4 * nF
     fcorr = 8 * (2 * nS * V + nF * V + nF) + 4 * nF
     return {"extrapolate": extrap, "gradient_flux": gflux, "face_correction": fcorr}

@@ -241,6 +241,12 @@ typedef struct zfr_ele_desc {
 
 /* Elements::extrapolate_U (src/elements.cpp:1337-1360): U[E][V][nS] -> Utr trace. */
 int zfr_extrapolate(const zfr_ele_desc_t *d, const double *U, void *stream);
+/* The same trace written only at the flux points of faces the partner element owns,
+ * which is all the one-sided LDG path (ldg_b = +-1/2) reads: there the owner takes the
+ * other side's trace as the common solution and for its common flux, and no element reads
+ * its own trace.  Half the writes of zfr_extrapolate; the other points are left as they
+ * were.  No reference counterpart (the reference always writes U_fpts in full). */
+int zfr_extrapolate_partner(const zfr_ele_desc_t *d, const double *U, void *stream);
 /* Faces::compute_common_U + Elements::compute_dU_spts/_fpts + the gradient part of
  * Elements::compute_F + Elements::extrapolate_dU (src/faces.cpp:1366-1474,
  * src/elements.cpp:1362-1447, 1588-1699): writes the gradient trace Gtr and, when

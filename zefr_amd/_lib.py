@@ -129,6 +129,7 @@ SIGNATURES = {
     "zefr_get_dq_spts_d": (_P, [_IP, _IP, _IP, _IP, _I]),
     "zefr_get_q_spt": (C.c_double, [_I, _I, _I]),
     "zfr_extrapolate": (_I, [C.POINTER(EleDesc), _P, _P]),
+    "zfr_extrapolate_partner": (_I, [C.POINTER(EleDesc), _P, _P]),
     "zfr_gradient": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_divergence": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_gradient_flux": (_I, [C.POINTER(EleDesc), _P, _P, _P]),

@@ -70,6 +70,9 @@ struct KArgs {
   double *Utr;
   double *Gtr;
   double *Ftr;
+  // k_extrapolate: 0 = the trace at every flux point; 1 = only where the partner owns the
+  // face (the one-sided LDG path reads nothing else)
+  int trace_sel;
   // RK update fused into the last kernel of a residual (mode 0: none, 1: stage, 2: final)
   struct Rk {
     double *U;
@@ -207,7 +210,9 @@ __device__ __forceinline__ int owned_fpt(unsigned mask, int k) {
 }
 
 // ---------------------------------------------------------------- kernel 1: traces
-// Elements::extrapolate_U (src/elements.cpp:1337-1360): U_fpts = oppE U_spts.
+// Elements::extrapolate_U (src/elements.cpp:1337-1360): U_fpts = oppE U_spts.  With
+// trace_sel = 1 only the points of faces the partner owns are written: on the one-sided
+// LDG path the owner reads the other side's trace and nothing reads its own.
 template <int D, int N>
 __global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__restrict__ U, KArgs a) {
   using S = Shape<D, N>;
@@ -217,16 +222,25 @@ __global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__res
   double *lU = lds + S::NOPS_PAD;
   const long e0 = (long)blockIdx.x * a.EB;
   const long neb = min((long)a.EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  __syncthreads();
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
+  // this thread's slots, fetched before the staging barrier so the loads overlap it
+  int slot[S::FPT];
+#pragma unroll
+  for (int k = 0; k < S::FPT; k++) {
+    const int f = t + k * nS;
+    slot[k] = (a.trace_sel && el < neb && f < nF) ? (int)((a.nbr[e * nF + f] >> 8) & 1u) : -1;
+  }
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  __syncthreads();
   if (el >= neb) return;
   const double *u = lU + el * V * nS;
+  const int owner = (a.ldg_b == 0.5) ? 0 : 1;
 #pragma unroll
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
     if (f < nF) {
+      if (a.trace_sel && slot[k] == owner) continue;
       int dim, side, base, str;
       fpt_info<D, N>(f, dim, side, base, str);
       const double *lE = tab_lE<N>(lops, side);
@@ -583,12 +597,30 @@ void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KA
   unsigned *lmask = reinterpret_cast<unsigned *>(lW + EB * W);  // owned faces per element
   const long e0 = (long)blockIdx.x * EB;
   const long neb = min((long)EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
-  __syncthreads();
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   const bool valid = el < neb;
+  // the slots of this thread's flux points and, where this element owns the face, the
+  // partner's trace: fetched before the staging barrier so the loads overlap it
+  int slot[S::FPT];
+  double cp[S::FPT][V];
+#pragma unroll
+  for (int k = 0; k < S::FPT; k++) {
+    const int f = t + k * nS;
+    slot[k] = -1;
+    if (valid && f < nF) {
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      slot[k] = sl;
+      if (sl == owner) {
+#pragma unroll
+        for (int v = 0; v < V; v++) cp[k][v] = a.Utr[(ep * V + v) * nF + fp];
+      }
+    }
+  }
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
+  __syncthreads();
   const double *u = lU + el * V * nS;
   double *c = lW + el * W;
   double *g_l = lW + el * W;
@@ -599,11 +631,9 @@ void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KA
     for (int k = 0; k < S::FPT; k++) {
       const int f = t + k * nS;
       if (f >= nF) continue;
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      if (sl == owner) {
+      if (slot[k] == owner) {
 #pragma unroll
-        for (int v = 0; v < V; v++) c[v * nF + f] = a.Utr[(ep * V + v) * nF + fp];
+        for (int v = 0; v < V; v++) c[v * nF + f] = cp[k][v];
         if (f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
       } else {
         int dim, side, base, str;
@@ -978,11 +1008,12 @@ inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
   a.rus_k = d->rus_k; a.ldg_b = d->ldg_b; a.ldg_tau = d->ldg_tau;
   a.ops = d->ops1d; a.metric = d->metric; a.fgeo = d->fgeo; a.nbr = d->nbr; a.Utr = d->Utr; a.Gtr = d->Gtr;
   a.Ftr = d->Ftr;
+  a.trace_sel = 0;
   a.rk = KArgs::Rk{};
   return a;
 }
 
-enum class Op { Extrap, Grad, Div, OneSided, FaceCorr, AdjPrep, AdjMain, AdjGather };
+enum class Op { Extrap, ExtrapPartner, Grad, Div, OneSided, FaceCorr, AdjPrep, AdjMain, AdjGather };
 
 // Extra operands of the reverse-mode launches.
 struct AdjPtrs {
@@ -1002,7 +1033,8 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   // LDS doubles per element (the owned-point list of the one-sided kernel is nF + 1 ints)
   size_t per_ele = 0;
   switch (op) {
-    case Op::Extrap: per_ele = (size_t)S::V * S::nS; break;
+    case Op::Extrap:
+    case Op::ExtrapPartner: per_ele = (size_t)S::V * S::nS; break;
     case Op::Grad: per_ele = kUC > kG ? kUC : kG; break;
     case Op::Div: per_ele = kUC + kG; break;
     case Op::OneSided:
@@ -1034,6 +1066,10 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   const size_t lds = sizeof(double) * (S::NOPS_PAD + EB * per_ele);
   switch (op) {
     case Op::Extrap:
+      hipLaunchKernelGGL((k_extrapolate<D, N>), grid, block, lds, st, U, a);
+      break;
+    case Op::ExtrapPartner:
+      a.trace_sel = 1;
       hipLaunchKernelGGL((k_extrapolate<D, N>), grid, block, lds, st, U, a);
       break;
     case Op::Grad:
@@ -1105,6 +1141,12 @@ extern "C" {
 
 int zfr_extrapolate(const zfr_ele_desc_t *d, const double *U, void *stream) {
   return zk::dispatch(zk::Op::Extrap, d, U, nullptr, (hipStream_t)stream);
+}
+
+int zfr_extrapolate_partner(const zfr_ele_desc_t *d, const double *U, void *stream) {
+  if (!d->viscous || (d->ldg_b != 0.5 && d->ldg_b != -0.5))
+    return zfr::fail("zfr_extrapolate_partner: the descriptor is not on the one-sided LDG path (beta = +-1/2)");
+  return zk::dispatch(zk::Op::ExtrapPartner, d, U, nullptr, (hipStream_t)stream);
 }
 
 int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void *stream) {

@@ -373,7 +373,8 @@ void FRSolver::compute_residual_start(unsigned, int color) {
   need_device(*this);
   if (color > -1) raise("multi-colour (MCGS) residuals are not supported");
   const auto d = eles->desc(*input);
-  status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
+  if (onesided()) status_or_throw(zfr_extrapolate_partner(&d, eles->U_spts_d.as<double>(), stream));
+  else status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
 }
 
 void FRSolver::compute_residual_mid(unsigned stage, int) {
@@ -406,11 +407,12 @@ void FRSolver::compute_residual(unsigned stage, int color) {
 void FRSolver::residual_at(const double *U, double *divF, hipStream_t st) {
   need_device(*this);
   const auto d = eles->desc(*input);
-  status_or_throw(zfr_extrapolate(&d, U, st));
   if (onesided()) {
+    status_or_throw(zfr_extrapolate_partner(&d, U, st));
     status_or_throw(zfr_gradient_flux(&d, U, divF, st));
     status_or_throw(zfr_face_correction(&d, divF, st));
   } else {
+    status_or_throw(zfr_extrapolate(&d, U, st));
     if (input->viscous) status_or_throw(zfr_gradient(&d, U, nullptr, st));
     status_or_throw(zfr_divergence(&d, U, divF, st));
   }
@@ -663,7 +665,8 @@ void FRSolver::time_kernels(int reps, double *ms_out) {
   double *U = el.U_spts_d.as<double>(), *divF = el.divF_d.as<double>();
   for (int r = 0; r < reps; r++) {
     hip_or_throw(hipEventRecord(ev[3 * r], stream), "hipEventRecord");
-    status_or_throw(zfr_extrapolate(&d, U, stream));
+    if (onesided()) status_or_throw(zfr_extrapolate_partner(&d, U, stream));
+    else status_or_throw(zfr_extrapolate(&d, U, stream));
     hip_or_throw(hipEventRecord(ev[3 * r + 1], stream), "hipEventRecord");
     if (onesided()) status_or_throw(zfr_gradient_flux(&d, U, divF, stream));
     else if (input->viscous) status_or_throw(zfr_gradient(&d, U, nullptr, stream));
