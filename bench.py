@@ -44,13 +44,14 @@ def parse():
     p.add_argument("--no-vjp", action="store_true", help="skip the VJP timing")
     p.add_argument("--no-steps", action="store_true", help="skip the RK4 step timings")
     p.add_argument("--no-pmg", action="store_true", help="skip the p-multigrid timing")
+    p.add_argument("--no-roe", action="store_true", help="skip the Roe+LDG variant timing")
     p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
     p.add_argument("--cpu-steps", type=int, default=150)
     p.add_argument("--workdir", default=None)
     return p.parse_args()
 
 
-def write_deck(path, meshfile, order, dt=2e-4):
+def write_deck(path, meshfile, order, dt=2e-4, fconv="Rusanov"):
     with open(path, "w") as f:
         f.write(f"""nDims 3
 meshfile {meshfile}
@@ -68,7 +69,7 @@ report_freq 10
 res_type 2
 ic_type 1
 test_case 4
-fconv_type Rusanov
+fconv_type {fconv}
 fvisc_type LDG
 rus_k 0.0
 ldg_b 0.5
@@ -339,6 +340,37 @@ def _restrict_table(p, r):
     return np.ascontiguousarray(R)
 
 
+def roe_timing(workdir, mesh, order, steps, tile):
+    """The Roe+LDG variant of the same Taylor-Green workload (the reference has no Roe
+    flux; it is timed separately from the Rusanov+LDG headline): residual throughput and
+    the three kernels' average durations, HIP events on the solver's stream."""
+    from zefr_amd import zefr
+    from zefr_amd._lib import lib
+    deck = write_deck(os.path.join(workdir, "tgv_roe.deck"), mesh, order, fconv="Roe")
+    z = zefr.Zefr()
+    z.read_input(deck)
+    z.setup_solver()
+    z.set_tile(tile)
+    d = z.dims()
+    dof = d["nEles"] * d["nSpts"] * d["nVars"]
+    for _ in range(3):
+        z.compute_residual(0)
+    z.synchronize()
+    ms3 = (C.c_double * 3)()
+    t = time.perf_counter()
+    rc = lib().zefr_time_residual_kernels(z._h, steps, ms3)
+    z.synchronize()
+    wall = time.perf_counter() - t
+    if rc != 0:
+        raise RuntimeError(lib().zefr_last_error().decode())
+    del z
+    return {"workload": "3D Navier-Stokes Taylor-Green residual with Roe+LDG (Harten entropy fix 0.1 a), same mesh",
+            "value": round(dof / (wall / steps) / 1e9, 4), "unit": "Gdof-residuals/s",
+            "ms_per_step": round(wall / steps * 1e3, 4),
+            "kernels_ms": {"extrapolate": round(ms3[0], 4), "gradient_flux": round(ms3[1], 4),
+                           "face_correction": round(ms3[2], 4)}}
+
+
 def vjp_timing(z, reps):
     """The residual's vector-Jacobian product on the same mesh and state: mean time of
     `reps` back-to-back calls (HIP events on the torch stream both run on) against the
@@ -459,6 +491,7 @@ def main():
     if not args.no_steps:
         rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
     pmg = None if args.no_pmg else pmg_timing(workdir)
+    roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps, tile)
 
     rec = {
         "metric": METRIC,
@@ -490,6 +523,7 @@ def main():
         "vjp": vjp,
         "rk4_step": rk,
         "pmg": pmg,
+        "roe": roe,
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(workdir, args.cpu_n, order, args.cpu_steps)

@@ -41,7 +41,7 @@ typedef struct zefr_input {
   unsigned int equation;          /* 1 = EulerNS (include/input.hpp:25-28)          */
   unsigned int nDims, order;
   int viscous;
-  unsigned int fconv_type;        /* 0 = Rusanov (include/input.hpp:30-33)           */
+  unsigned int fconv_type;        /* 0 = Rusanov (include/input.hpp:30-33), 2 = Roe   */
   unsigned int fvisc_type;        /* 1 = LDG                                         */
   double rus_k, ldg_b, ldg_tau;
   char dt_scheme[16];             /* "Euler" or "RK44"                               */
@@ -237,6 +237,7 @@ typedef struct zfr_ele_desc {
   double *Gtr;                     /* [E][D][V][nF] gradient trace (scratch, viscous)    */
   double *Ftr;                     /* [E][V][nF] common normal flux at the flux points
                                       whose side owns it (scratch, LDG beta = +-1/2)     */
+  int fconv_type;                  /* convective common flux: 0 Rusanov, 2 Roe           */
 } zfr_ele_desc_t;
 
 /* Elements::extrapolate_U (src/elements.cpp:1337-1360): U[E][V][nS] -> Utr trace. */
@@ -330,6 +331,7 @@ typedef struct zfr_phys {
   double gamma, prandtl, mu, rt, c_sth;
   int fix_vis;
   double rus_k, ldg_b, ldg_tau;
+  int fconv_type;                  /* 0 Rusanov, 2 Roe (Harten entropy fix, 0.1 a)       */
 } zfr_phys_t;
 
 /* Host builds of the pointwise physics the kernels run, for checking on the CPU.

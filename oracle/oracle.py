@@ -26,7 +26,7 @@ class ZoPhys(C.Structure):
                 ("mu", C.c_double), ("rt", C.c_double), ("c_sth", C.c_double),
                 ("fix_vis", C.c_int), ("rus_k", C.c_double), ("ldg_b", C.c_double),
                 ("ldg_tau", C.c_double), ("dt", C.c_double), ("nStages", C.c_int),
-                ("res_type", C.c_int)]
+                ("res_type", C.c_int), ("fconv", C.c_int)]
 
 
 def build():
@@ -62,6 +62,8 @@ def lib():
         L.zo_lagrange_d1.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double]
         L.zo_point_flux.argtypes = [C.c_int, C.POINTER(ZoPhys), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.zo_common_flux.argtypes = [C.c_int, C.POINTER(ZoPhys), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.zo_pmg_create.restype = C.c_void_p
         L.zo_pmg_create.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.c_double, C.c_char_p, C.c_int]
@@ -85,9 +87,10 @@ def _check(rc):
 
 
 def make_phys(viscous=False, gamma=1.4, prandtl=0.72, mu=0.0, rt=0.0, c_sth=0.0, fix_vis=True,
-              rus_k=0.0, ldg_b=0.5, ldg_tau=0.1, dt=1e-3, nStages=4, res_type=2):
+              rus_k=0.0, ldg_b=0.5, ldg_tau=0.1, dt=1e-3, nStages=4, res_type=2, fconv=0):
+    """fconv: 0 Rusanov (the reference's flux), 2 Roe (textbook, Harten entropy fix)."""
     return ZoPhys(int(viscous), gamma, prandtl, mu, rt, c_sth, int(fix_vis), rus_k, ldg_b,
-                  ldg_tau, dt, nStages, res_type)
+                  ldg_tau, dt, nStages, res_type, fconv)
 
 
 class Oracle:
@@ -230,6 +233,17 @@ def lagrange(grid, mode, x, deriv=False):
     g = np.ascontiguousarray(grid, np.float64)
     f = lib().zo_lagrange_d1 if deriv else lib().zo_lagrange
     return f(_dp(g), g.size, mode, float(x))
+
+
+def roe_flux(nDims, UL, UR, n, gamma=1.4):
+    """The oracle's Roe common normal flux for the unit normal n."""
+    phys = make_phys(gamma=gamma, fconv=2)
+    F = np.zeros(nDims + 2)
+    UL = np.ascontiguousarray(UL, np.float64)
+    UR = np.ascontiguousarray(UR, np.float64)
+    n = np.ascontiguousarray(n, np.float64)
+    _check(lib().zo_common_flux(nDims, C.byref(phys), _dp(UL), _dp(UR), _dp(n), _dp(F)))
+    return F
 
 
 def point_flux(nDims, phys, U, dU=None):

@@ -69,6 +69,7 @@ struct Phys {
   double rus_k, ldg_b, ldg_tau, dt;
   int nStages;
   int res_type;
+  int fconv;  // 0 Rusanov (the reference's), 2 Roe (textbook, below)
 };
 
 [[noreturn]] void fail(const std::string &msg) { throw std::runtime_error(msg); }
@@ -211,6 +212,60 @@ void fconv(const double *U, double F[V][D], double &P, double gamma) {
       F[d + 1][d] += P;
       F[4][d] = U[d + 1] * H;
     }
+  }
+}
+
+// Roe's flux (Roe 1981) with Harten's entropy fix on the acoustic waves, delta = 0.1 a~.
+// The reference has no Roe flux: this is a textbook restatement, not a restatement of
+// reference code; tests/test_roe.py pins it against numpy's eigen-decomposition of the
+// flux Jacobian at the Roe-averaged state.  Fn = F.n for the unit normal n.
+template <int V, int D>
+void roe_flux(const double *UL, const double *UR, const double *n, double gamma, double *Fn) {
+  double FL[V][D], FR[V][D], PL, PR;
+  fconv<V, D>(UL, FL, PL, gamma);
+  fconv<V, D>(UR, FR, PR, gamma);
+  double uL[D], uR[D];
+  for (int d = 0; d < D; d++) { uL[d] = UL[d + 1] / UL[0]; uR[d] = UR[d + 1] / UR[0]; }
+  const double HL = (UL[V - 1] + PL) / UL[0], HR = (UR[V - 1] + PR) / UR[0];
+  const double rl = std::sqrt(UL[0]), rr = std::sqrt(UR[0]);
+  double ut[D], unt = 0, q2 = 0, unL = 0, unR = 0;
+  for (int d = 0; d < D; d++) {
+    ut[d] = (rl * uL[d] + rr * uR[d]) / (rl + rr);
+    unt += ut[d] * n[d]; q2 += ut[d] * ut[d];
+    unL += uL[d] * n[d]; unR += uR[d] * n[d];
+  }
+  const double Ht = (rl * HL + rr * HR) / (rl + rr);
+  const double rhot = rl * rr;
+  const double at = std::sqrt((gamma - 1.0) * (Ht - 0.5 * q2));
+  const double delta = 0.1 * at;
+  auto fix = [&](double lam) {
+    double l = std::abs(lam);
+    return (l < delta) ? (lam * lam + delta * delta) / (2.0 * delta) : l;
+  };
+  const double lam[3] = {fix(unt - at), std::abs(unt), fix(unt + at)};
+  const double dP = PR - PL, dUn = unR - unL, dRho = UR[0] - UL[0];
+  // wave strengths and right eigenvectors: acoustic (-), entropy, acoustic (+), shear
+  const double alpha[3] = {(dP - rhot * at * dUn) / (2.0 * at * at), dRho - dP / (at * at),
+                           (dP + rhot * at * dUn) / (2.0 * at * at)};
+  double r[3][V];
+  r[0][0] = 1.0; r[1][0] = 1.0; r[2][0] = 1.0;
+  for (int d = 0; d < D; d++) {
+    r[0][d + 1] = ut[d] - at * n[d];
+    r[1][d + 1] = ut[d];
+    r[2][d + 1] = ut[d] + at * n[d];
+  }
+  r[0][V - 1] = Ht - at * unt; r[1][V - 1] = 0.5 * q2; r[2][V - 1] = Ht + at * unt;
+  double diss[V] = {0};
+  for (int k = 0; k < 3; k++)
+    for (int m = 0; m < V; m++) diss[m] += lam[k] * alpha[k] * r[k][m];
+  double sh[D], ush = 0;
+  for (int d = 0; d < D; d++) { sh[d] = (uR[d] - uL[d]) - dUn * n[d]; ush += ut[d] * sh[d]; }
+  for (int d = 0; d < D; d++) diss[d + 1] += lam[1] * rhot * sh[d];
+  diss[V - 1] += lam[1] * rhot * ush;
+  for (int m = 0; m < V; m++) {
+    double fl = 0, fr = 0;
+    for (int d = 0; d < D; d++) { fl += FL[m][d] * n[d]; fr += FR[m][d] * n[d]; }
+    Fn[m] = 0.5 * (fl + fr) - 0.5 * diss[m];
   }
 }
 
@@ -881,6 +936,16 @@ struct Oracle {
     for (int g = 0; g < (int)nGfpts; g++) {
       double UL[V], UR[V], FL[V][D], FR[V][D], PL, PR;
       for (int n = 0; n < V; n++) { UL[n] = U_fpts[fview(U_fpts, 0, n, g)]; UR[n] = U_fpts[fview(U_fpts, 1, n, g)]; }
+      if (ph.fconv == 2) {
+        double nv[D], F[V];
+        for (int d = 0; d < D; d++) nv[d] = norm[(size_t)d * nGfpts + g];
+        roe_flux<V, D>(UL, UR, nv, ph.gamma, F);
+        for (int n = 0; n < V; n++) {
+          Fcomm[fview(Fcomm, 0, n, g)] = F[n] * dA[g];
+          Fcomm[fview(Fcomm, 1, n, g)] = -F[n] * dA[nGfpts + g];
+        }
+        continue;
+      }
       fconv<V, D>(UL, FL, PL, ph.gamma);
       fconv<V, D>(UR, FR, PR, ph.gamma);
       double aL = std::sqrt(ph.gamma * PL / UL[0]);
@@ -1220,6 +1285,7 @@ struct zo_phys {
   double rus_k, ldg_b, ldg_tau, dt;
   int nStages;
   int res_type;
+  int fconv;
 };
 
 const char *zo_last_error(void) { return g_err.c_str(); }
@@ -1230,6 +1296,7 @@ void *zo_create(const char *meshfile, int nDims, int order, const zo_phys *p) {
     if (nDims != 2 && nDims != 3) fail("oracle: nDims must be 2 or 3");
     if (order < 0 || order > 5) fail("oracle: order must be in 0..5");
     if (p->nStages != 1 && p->nStages != 4) fail("oracle: nStages must be 1 or 4");
+    if (p->fconv != 0 && p->fconv != 2) fail("oracle: fconv must be 0 (Rusanov) or 2 (Roe)");
     o->nDims = nDims; o->order = order;
     std::memcpy(&o->ph, p, sizeof(zo_phys));
     o->setup(meshfile);
@@ -1381,6 +1448,16 @@ double zo_lagrange_d1(const double *grid, int n, int mode, double x) {
 }
 
 // Pointwise fluxes on one state (nDims 2 or 3): F[V][D] convective (+ viscous if dU given).
+// Convective common normal flux for the unit normal n (no area): Rusanov or Roe.
+int zo_common_flux(int nDims, const zo_phys *p, const double *UL, const double *UR, const double *n, double *F) {
+  return guarded([&] {
+    Phys ph; std::memcpy(&ph, p, sizeof(zo_phys));
+    if (ph.fconv != 2) fail("oracle: zo_common_flux is the Roe flux (fconv 2)");
+    if (nDims == 2) roe_flux<4, 2>(UL, UR, n, ph.gamma, F);
+    else roe_flux<5, 3>(UL, UR, n, ph.gamma, F);
+  });
+}
+
 int zo_point_flux(int nDims, const zo_phys *p, const double *U, const double *dU, double *F) {
   return guarded([&] {
     Phys ph; std::memcpy(&ph, p, sizeof(zo_phys));

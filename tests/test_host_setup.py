@@ -162,7 +162,7 @@ def test_deck_nondim_closed_form():
     (["equation AdvDiff"], "AdvDiff"),
     (["dt_type 1"], "dt_type"),
     (["motion 1"], "moving"),
-    (["fconv_type Roe"], "Convective interface flux"),  # the reference's message
+    (["fconv_type HLLC"], "Convective interface flux"),  # the reference's message
 ])
 def test_deck_rejects_unsupported_options(extra, msg):
     with pytest.raises(RuntimeError, match=msg):

@@ -59,6 +59,7 @@ class EleDesc(C.Structure):
         ("c_sth", C.c_double), ("fix_vis", C.c_int), ("rus_k", C.c_double), ("ldg_b", C.c_double),
         ("ldg_tau", C.c_double), ("ops1d", C.c_void_p), ("metric", C.c_void_p), ("fgeo", C.c_void_p),
         ("nbr", C.c_void_p), ("Utr", C.c_void_p), ("Gtr", C.c_void_p), ("Ftr", C.c_void_p),
+        ("fconv_type", C.c_int),
     ]
 
 
@@ -67,7 +68,7 @@ class ZfrPhys(C.Structure):
 
     _fields_ = [("gamma", C.c_double), ("prandtl", C.c_double), ("mu", C.c_double), ("rt", C.c_double),
                 ("c_sth", C.c_double), ("fix_vis", C.c_int), ("rus_k", C.c_double), ("ldg_b", C.c_double),
-                ("ldg_tau", C.c_double)]
+                ("ldg_tau", C.c_double), ("fconv_type", C.c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/zefr_amd.h

@@ -203,8 +203,10 @@ void read_input_file(const std::string &file, zefr_input_t &in) {
   in.res_field = f.uint("res_field", 0);
   in.test_case = f.uint("test_case", 0);
   const std::string fc = f.str("fconv_type", "Rusanov");
-  if (fc != "Rusanov") raise("Convective interface flux not recognized!");
-  in.fconv_type = 0;
+  // the reference knows Rusanov only (src/input.cpp:177-182); Roe is this build's addition
+  if (fc == "Rusanov") in.fconv_type = 0;
+  else if (fc == "Roe") in.fconv_type = 2;
+  else raise("Convective interface flux not recognized!");
   const std::string fv = f.str("fvisc_type", "LDG");
   if (fv != "LDG") raise("Viscous interface flux not recognized!");
   in.fvisc_type = 1;

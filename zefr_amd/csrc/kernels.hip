@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restri
 // compute_F + compute_divF_spts + compute_common_F + compute_divF_fpts.
 // Phases (viscous): U -> LDS | Ucomm -> lC | gradient -> lG | common flux -> lC |
 // transformed flux -> lG | divergence.  Inviscid: U -> LDS | flux + common flux | divergence.
-template <int D, int N, bool VISC, int BM>
+template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__restrict__ U, double *__restrict__ divF,
                                                             KArgs a) {
   using S = Shape<D, N>;
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
       double UL[V], UR[V];
       if (sl) { par_state(UL); own_state(UR); } else { own_state(UL); par_state(UR); }
       double Fo[V];
-      common_flux<D, VISC, BM>(UL, UR, G1, G2, n, dAs, a, Fo);
+      common_flux<D, VISC, BM, FC>(UL, UR, G1, G2, n, dAs, a, Fo);
 #pragma unroll
       for (int v = 0; v < V; v++) c[v * nF + f] = Fo[v];
     }
@@ -573,15 +573,17 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
 //   fluxes) -> k_face_correction (oppDiv_fpts Fcomm, both sides).
 
 // Register budget: for hex p=3 four waves per SIMD (<= 128 VGPRs) is what the kernel's
-// LDS allows per CU and the face phase fits it without spilling, so the allocator is held
-// to it; other shapes would spill under that cap and keep the default.
-template <int D, int N>
+// LDS allows per CU and the Rusanov face phase fits it without spilling, so the allocator
+// is held to it.  The Roe face phase needs more: three waves (<= 168 VGPRs) measured
+// fastest (four spill, 2.76 ms; three 1.52 ms; uncapped, two waves, 1.80 ms at 64^3).
+// Other shapes would spill under a cap and keep the default.
+template <int D, int N, int FC>
 struct OneSidedWaves {
-  static constexpr int value = (D == 3 && N == 4) ? 4 : 1;
+  static constexpr int value = (D == 3 && N == 4) ? (FC == 1 ? 3 : 4) : 1;
 };
 
-template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N>::value)))
+template <int D, int N, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
 void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
@@ -678,6 +680,30 @@ void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KA
       double Uo[V], Up[V];
 #pragma unroll
       for (int v = 0; v < V; v++) Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
+      if constexpr (FC == 1) {
+        // Roe: the convective flux first, so that its many temporaries are dead before the
+        // gradient's lines are gathered (formed in the Rusanov order it needs ~350 registers)
+        long ep; int sl, fp;
+        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+#pragma unroll
+        for (int v = 0; v < V; v++) Up[v] = a.Utr[(ep * V + v) * nF + fp];
+        double Fo[V], G[V];
+        if (owner == 0) conv_common_flux<D, FC>(Uo, Up, n, a, Fo);
+        else conv_common_flux<D, FC>(Up, Uo, n, a, Fo);
+        asm volatile("" ::: "memory");
+        visc_normal<D>(Uo, [&](int j, double *dUj) {
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+            asm volatile("" ::: "memory");
+          }
+        }, n, a, G);
+        if (owner == 0) ldg_common_add<D, BM>(Uo, Up, G, G, a, Fo);
+        else ldg_common_add<D, BM>(Up, Uo, G, G, a, Fo);
+#pragma unroll
+        for (int v = 0; v < V; v++) a.Ftr[(e * V + v) * nF + f] = Fo[v];
+        continue;
+      }
       double G[V];
       // one variable's line at a time (a compiler barrier after each): 128 VGPRs instead
       // of 160, i.e. one more wave per SIMD, worth more than the loads' overlap
@@ -696,8 +722,8 @@ void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KA
         for (int v = 0; v < V; v++) Up[v] = a.Utr[(ep * V + v) * nF + fp];  // cached since phase 1
       }
       double Fo[V];
-      if (owner == 0) common_flux<D, true, BM>(Uo, Up, G, G, n, 1.0, a, Fo);
-      else common_flux<D, true, BM>(Up, Uo, G, G, n, 1.0, a, Fo);
+      if (owner == 0) common_flux<D, true, BM, FC>(Uo, Up, G, G, n, 1.0, a, Fo);
+      else common_flux<D, true, BM, FC>(Up, Uo, G, G, n, 1.0, a, Fo);
 #pragma unroll
       for (int v = 0; v < V; v++) a.Ftr[(e * V + v) * nF + f] = Fo[v];
     }
@@ -1015,6 +1041,16 @@ inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
 
 enum class Op { Extrap, ExtrapPartner, Grad, Div, OneSided, FaceCorr, AdjPrep, AdjMain, AdjGather };
 
+// k_divergence for one convective flux (FC: 0 Rusanov, 1 Roe) and the LDG beta branch.
+template <int D, int N, int FC>
+void launch_div(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds, hipStream_t st, const double *U,
+                double *out, const KArgs &a) {
+  if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC>), grid, block, lds, st, U, out, a);
+  else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 1, FC>), grid, block, lds, st, U, out, a);
+  else if (d->ldg_b == -0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 2, FC>), grid, block, lds, st, U, out, a);
+  else hipLaunchKernelGGL((k_divergence<D, N, true, 0, FC>), grid, block, lds, st, U, out, a);
+}
+
 // Extra operands of the reverse-mode launches.
 struct AdjPtrs {
   const double *w = nullptr;  // cotangent of divF
@@ -1050,6 +1086,8 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   const bool adj = op == Op::AdjPrep || op == Op::AdjMain || op == Op::AdjGather;
   if (adj && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5)
     return zfr::fail("the residual VJP supports LDG beta = +-0.5 (and Euler)");
+  if (adj && d->fconv_type != 0) return zfr::fail("the residual VJP supports the Rusanov flux only");
+  if (d->fconv_type != 0 && d->fconv_type != 2) return zfr::fail("fconv_type must be 0 (Rusanov) or 2 (Roe)");
   if ((op == Op::OneSided || op == Op::FaceCorr) && d->ldg_b != 0.5 && d->ldg_b != -0.5)
     return zfr::fail("one-sided LDG kernels need ldg_b = +-0.5");
   // the viscous adjoint runs one element per workgroup: its LDS then allows ~7
@@ -1076,14 +1114,17 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       hipLaunchKernelGGL((k_gradient<D, N>), grid, block, lds, st, U, out, a);
       break;
     case Op::Div:
-      if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0>), grid, block, lds, st, U, out, a);
-      else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 1>), grid, block, lds, st, U, out, a);
-      else if (d->ldg_b == -0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 2>), grid, block, lds, st, U, out, a);
-      else hipLaunchKernelGGL((k_divergence<D, N, true, 0>), grid, block, lds, st, U, out, a);
+      if (d->fconv_type == 2) launch_div<D, N, 1>(d, grid, block, lds, st, U, out, a);
+      else launch_div<D, N, 0>(d, grid, block, lds, st, U, out, a);
       break;
     case Op::OneSided:
-      if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_visc_onesided<D, N, 1>), grid, block, lds, st, U, out, a);
-      else hipLaunchKernelGGL((k_visc_onesided<D, N, 2>), grid, block, lds, st, U, out, a);
+      if (d->fconv_type == 2) {
+        if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_visc_onesided<D, N, 1, 1>), grid, block, lds, st, U, out, a);
+        else hipLaunchKernelGGL((k_visc_onesided<D, N, 2, 1>), grid, block, lds, st, U, out, a);
+      } else {
+        if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_visc_onesided<D, N, 1, 0>), grid, block, lds, st, U, out, a);
+        else hipLaunchKernelGGL((k_visc_onesided<D, N, 2, 0>), grid, block, lds, st, U, out, a);
+      }
       break;
     case Op::FaceCorr:
       if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_face_correction<D, N, 1>), grid, block, lds, st, out, a);

@@ -358,6 +358,60 @@ ZFR_HD void wavespeed_vjp(const double *U, const Prim<D> &p, const double *n, do
   Ub[0] += rhob;
 }
 
+// Roe's approximate Riemann solver (P. L. Roe, J. Comput. Phys. 43 (1981) 357-372) for the
+// Euler flux across the unit normal n: the dissipation |A~| (UR - UL) of the Roe matrix at
+// the Roe-averaged state, by wave strengths and right eigenvectors (the acoustic waves
+// un -+ a, the entropy wave and the shear waves at un).  Harten's entropy fix (A. Harten,
+// J. Comput. Phys. 49 (1983) 357-393) smooths the acoustic |lambda| below
+// delta = kRoeEntropyFix * a~ to (lambda^2 + delta^2) / (2 delta).  The reference has
+// no Roe flux; this is the textbook solver.
+constexpr double kRoeEntropyFix = 0.1;
+
+ZFR_HD double harten_abs(double lam, double delta) {
+  const double l = fabs(lam);
+  return (l < delta) ? 0.5 * (lam * lam + delta * delta) / delta : l;
+}
+
+template <int D>
+ZFR_HD void roe_dissipation(const double *UL, const double *UR, const Prim<D> &pL, const Prim<D> &pR,
+                            const double *n, double gamma, double *diss) {
+  constexpr int V = D + 2;
+  const double sL = sqrt(UL[0]), sR = sqrt(UR[0]);
+  const double wden = 1.0 / (sL + sR);
+  double u[D], un = 0.0, q2 = 0.0, unL = 0.0, unR = 0.0;
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++) {
+    u[d] = (sL * pL.vel[d] + sR * pR.vel[d]) * wden;
+    un += u[d] * n[d];
+    q2 += u[d] * u[d];
+    unL += pL.vel[d] * n[d];
+    unR += pR.vel[d] * n[d];
+  }
+  const double H = (sL * pL.H + sR * pR.H) * wden;
+  const double rho = sL * sR;
+  const double a2 = (gamma - 1.0) * (H - 0.5 * q2);
+  const double a = sqrt(a2);
+  const double dr = UR[0] - UL[0], dp = pR.P - pL.P, dun = unR - unL;
+  const double delta = kRoeEntropyFix * a;
+  const double l1 = harten_abs(un - a, delta), l2 = fabs(un), l3 = harten_abs(un + a, delta);
+  const double a1 = 0.5 * (dp - rho * a * dun) / a2;  // strength of the un - a wave
+  const double a3 = 0.5 * (dp + rho * a * dun) / a2;  // un + a
+  const double a2e = dr - dp / a2;                     // entropy wave
+  const double w1 = l1 * a1, w3 = l3 * a3, w2 = l2 * a2e;
+  // shear waves: rho (du - dun n) at speed un
+  double du_t[D], ut_du = 0.0;
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++) {
+    du_t[d] = (pR.vel[d] - pL.vel[d]) - dun * n[d];
+    ut_du += u[d] * du_t[d];
+  }
+  diss[0] = w1 + w2 + w3;
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++)
+    diss[d + 1] = w1 * (u[d] - a * n[d]) + w2 * u[d] + w3 * (u[d] + a * n[d]) + l2 * rho * du_t[d];
+  diss[V - 1] = w1 * (H - a * un) + w2 * 0.5 * q2 + w3 * (H + a * un) + l2 * rho * ut_du;
+}
+
 // LDG beta branches of src/faces.cpp:1520-1545, fixed at compile time: BM = 0 any beta
 // (both sides' gradients), 1 beta = +1/2 (left side only), 2 beta = -1/2 (right only).
 // Common normal flux for this element's side of a flux point: Rusanov
@@ -365,7 +419,46 @@ ZFR_HD void wavespeed_vjp(const double *U, const Prim<D> &p, const double *n, do
 // the signed area of the side (Fcomm(0) = F dA0, Fcomm(1) = -F dA1).
 // G1 / G2: viscous normal fluxes already formed (G1 = left for BM 0 / 1, right for BM 2;
 // G2 = right for BM 0).
-template <int D, bool VISC, int BM, typename P>
+// The convective part of common_flux alone (no area), for the kernels that form it before
+// the viscous part: Roe here, whose temporaries are then dead before the gradient is read.
+template <int D, int FC, typename P>
+ZFR_HD void conv_common_flux(const double *UL, const double *UR, const double *n, const P &a, double *F) {
+  constexpr int V = D + 2;
+  const Prim<D> pL = primitives<D>(UL, a.gamma), pR = primitives<D>(UR, a.gamma);
+  double FnL[V], FnR[V];
+  conv_proj<D>(UL, pL, n, FnL);
+  conv_proj<D>(UR, pR, n, FnR);
+  if (FC == 1) {
+    double dis[V];
+    roe_dissipation<D>(UL, UR, pL, pR, n, a.gamma, dis);
+    ZFR_UNROLL
+    for (int v = 0; v < V; v++) F[v] = 0.5 * (FnR[v] + FnL[v]) - 0.5 * dis[v];
+  } else {
+    const double aL = sqrt(a.gamma * pL.P * pL.invrho);
+    const double aR = sqrt(a.gamma * pR.P * pR.invrho);
+    const double eig = fmax(fabs(FnL[0] * pL.invrho) + aL, fabs(FnR[0] * pR.invrho) + aR);
+    const double diss = 0.5 * eig * (1.0 - a.rus_k);
+    ZFR_UNROLL
+    for (int v = 0; v < V; v++) F[v] = 0.5 * (FnR[v] + FnL[v]) - diss * (UR[v] - UL[v]);
+  }
+}
+
+// The LDG part of common_flux added to F (no area): G1 / G2 as in common_flux.
+template <int D, int BM, typename P>
+ZFR_HD void ldg_common_add(const double *UL, const double *UR, const double *G1, const double *G2, const P &a,
+                           double *F) {
+  constexpr int V = D + 2;
+  const double beta = a.ldg_b, tau = a.ldg_tau;
+  ZFR_UNROLL
+  for (int v = 0; v < V; v++) {
+    const double GL = (BM == 2) ? 0.0 : G1[v];
+    const double GR = (BM == 0) ? G2[v] : ((BM == 2) ? G1[v] : 0.0);
+    F[v] += 0.5 * (GL + GR) + tau * (UL[v] - UR[v]) + beta * (GL - GR);
+  }
+}
+
+// FC selects the convective flux: 0 Rusanov (the reference's), 1 Roe (roe_dissipation).
+template <int D, bool VISC, int BM, int FC = 0, typename P>
 ZFR_HD void common_flux(const double *UL, const double *UR, const double *G1, const double *G2, const double *n,
                         double dAs, const P &a, double *Fout) {
   constexpr int V = D + 2;
@@ -373,12 +466,19 @@ ZFR_HD void common_flux(const double *UL, const double *UR, const double *G1, co
   double FnL[V], FnR[V];
   conv_proj<D>(UL, pL, n, FnL);
   conv_proj<D>(UR, pR, n, FnR);
-  const double aL = sqrt(a.gamma * pL.P * pL.invrho);
-  const double aR = sqrt(a.gamma * pR.P * pR.invrho);
-  const double eig = fmax(fabs(FnL[0] * pL.invrho) + aL, fabs(FnR[0] * pR.invrho) + aR);
-  const double diss = 0.5 * eig * (1.0 - a.rus_k);
-  ZFR_UNROLL
-  for (int v = 0; v < V; v++) Fout[v] = (0.5 * (FnR[v] + FnL[v]) - diss * (UR[v] - UL[v])) * dAs;
+  if (FC == 1) {
+    double dis[V];
+    roe_dissipation<D>(UL, UR, pL, pR, n, a.gamma, dis);
+    ZFR_UNROLL
+    for (int v = 0; v < V; v++) Fout[v] = (0.5 * (FnR[v] + FnL[v]) - 0.5 * dis[v]) * dAs;
+  } else {
+    const double aL = sqrt(a.gamma * pL.P * pL.invrho);
+    const double aR = sqrt(a.gamma * pR.P * pR.invrho);
+    const double eig = fmax(fabs(FnL[0] * pL.invrho) + aL, fabs(FnR[0] * pR.invrho) + aR);
+    const double diss = 0.5 * eig * (1.0 - a.rus_k);
+    ZFR_UNROLL
+    for (int v = 0; v < V; v++) Fout[v] = (0.5 * (FnR[v] + FnL[v]) - diss * (UR[v] - UL[v])) * dAs;
+  }
   if (VISC) {
     const double beta = a.ldg_b, tau = a.ldg_tau;
     ZFR_UNROLL

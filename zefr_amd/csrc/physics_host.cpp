@@ -57,20 +57,20 @@ void side_visc(const zfr_phys_t &a, const double *U, const double *dU, const dou
   visc_normal<D>(U, [&](int j, double *dUj) { for (int v = 0; v < D + 2; v++) dUj[v] = dU[v * D + j]; }, n, a, G);
 }
 
-template <int D, int BM>
+template <int D, int BM, int FC>
 void common(const zfr_phys_t &a, int viscous, const double *UL, const double *UR, const double *dUL,
             const double *dUR, const double *n, double dAs, double *F) {
   constexpr int V = D + 2;
   if (!viscous) {
-    common_flux<D, false, 0>(UL, UR, nullptr, nullptr, n, dAs, a, F);
+    common_flux<D, false, 0, FC>(UL, UR, nullptr, nullptr, n, dAs, a, F);
     return;
   }
   double GL[V], GR[V];
   if (BM != 2) side_visc<D>(a, UL, dUL, n, GL);
   if (BM != 1) side_visc<D>(a, UR, dUR, n, GR);
-  if (BM == 0) common_flux<D, true, 0>(UL, UR, GL, GR, n, dAs, a, F);
-  else if (BM == 1) common_flux<D, true, 1>(UL, UR, GL, nullptr, n, dAs, a, F);
-  else common_flux<D, true, 2>(UL, UR, GR, nullptr, n, dAs, a, F);
+  if (BM == 0) common_flux<D, true, 0, FC>(UL, UR, GL, GR, n, dAs, a, F);
+  else if (BM == 1) common_flux<D, true, 1, FC>(UL, UR, GL, nullptr, n, dAs, a, F);
+  else common_flux<D, true, 2, FC>(UL, UR, GR, nullptr, n, dAs, a, F);
 }
 
 template <int D, int BM>
@@ -131,17 +131,15 @@ int zfr_host_common_flux(int nDims, const zfr_phys_t *a, int viscous, const doub
                          const double *dUL, const double *dUR, const double *n, double dAs, double *F) {
   return zfr::guard([&] {
     const int bm = bm_of(a->ldg_b);
-    if (nDims == 2) {
-      if (bm == 0) common<2, 0>(*a, viscous, UL, UR, dUL, dUR, n, dAs, F);
-      else if (bm == 1) common<2, 1>(*a, viscous, UL, UR, dUL, dUR, n, dAs, F);
-      else common<2, 2>(*a, viscous, UL, UR, dUL, dUR, n, dAs, F);
-    } else if (nDims == 3) {
-      if (bm == 0) common<3, 0>(*a, viscous, UL, UR, dUL, dUR, n, dAs, F);
-      else if (bm == 1) common<3, 1>(*a, viscous, UL, UR, dUL, dUR, n, dAs, F);
-      else common<3, 2>(*a, viscous, UL, UR, dUL, dUR, n, dAs, F);
-    } else {
-      zfr::raise("nDims must be 2 or 3");
-    }
+    if (nDims != 2 && nDims != 3) zfr::raise("nDims must be 2 or 3");
+    if (a->fconv_type != 0 && a->fconv_type != 2) zfr::raise("fconv_type must be 0 (Rusanov) or 2 (Roe)");
+    const bool roe = a->fconv_type == 2;
+    using Fn = void (*)(const zfr_phys_t &, int, const double *, const double *, const double *, const double *,
+                        const double *, double, double *);
+    static const Fn table[2][2][3] = {
+        {{common<2, 0, 0>, common<2, 1, 0>, common<2, 2, 0>}, {common<2, 0, 1>, common<2, 1, 1>, common<2, 2, 1>}},
+        {{common<3, 0, 0>, common<3, 1, 0>, common<3, 2, 0>}, {common<3, 0, 1>, common<3, 1, 1>, common<3, 2, 1>}}};
+    table[nDims - 2][roe ? 1 : 0][bm](*a, viscous, UL, UR, dUL, dUR, n, dAs, F);
   });
 }
 
@@ -149,6 +147,7 @@ int zfr_host_common_flux_vjp(int nDims, const zfr_phys_t *a, int viscous, const 
                              const double *dUL, const double *dUR, const double *n, double dAs, const double *Fb,
                              double *ULb, double *URb, double *dULb, double *dURb) {
   return zfr::guard([&] {
+    if (a->fconv_type != 0) zfr::raise("the common-flux VJP supports the Rusanov flux only");
     const int bm = bm_of(a->ldg_b);
     if (nDims == 2) {
       if (bm == 0) common_vjp<2, 0>(*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb, dURb);

@@ -223,6 +223,7 @@ zfr_ele_desc_t Elements::desc(const zefr_input_t &in) const {
   d.Utr = Utr_d.as<double>();
   d.Gtr = Gtr_d.as<double>();
   d.Ftr = Ftr_d.as<double>();
+  d.fconv_type = (int)in.fconv_type;
   return d;
 }
 
