@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--no-steps", action="store_true", help="skip the RK4 step timings")
     p.add_argument("--no-pmg", action="store_true", help="skip the p-multigrid timing")
     p.add_argument("--no-roe", action="store_true", help="skip the Roe+LDG variant timing")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched small-case timing")
     p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
     p.add_argument("--cpu-steps", type=int, default=150)
     p.add_argument("--workdir", default=None)
@@ -340,6 +341,57 @@ def _restrict_table(p, r):
     return np.ascontiguousarray(R)
 
 
+def batch_timing(workdir, count=64, reps=20, seed=2024):
+    """The batched half of the multigrid configuration: 64 independent periodic quad cases,
+    seeded sizes n x n (n in 8..64) and orders 1..5, Euler, seeded perturbed freestream
+    (5% Gaussian point noise).  One residual of every case per batch: eager launches on
+    the solvers' own streams against one replay of the captured batch graph."""
+    import numpy as np
+    from zefr_amd import zefr
+    from zefr_amd.mesh import write_box
+    rng = np.random.default_rng(seed)
+    zs, dof = [], 0
+    for i in range(count):
+        n, p = int(rng.integers(8, 65)), int(rng.integers(1, 6))
+        mesh = write_box(os.path.join(workdir, f"batch{i}.msh"), (n, n), (0.0, 0.0), (1.0, 1.0))
+        deck = os.path.join(workdir, f"batch{i}.deck")
+        with open(deck, "w") as f:
+            f.write(f"nDims 2\nmeshfile {mesh}\nmesh_bound periodic periodic\nmesh_bound fluid fluid\n"
+                    f"order {p}\nequation EulerNS\nviscous 0\nn_steps 1\ndt_scheme RK44\ndt 1e-4\n"
+                    f"dt_type 0\nic_type 0\nfconv_type Rusanov\ngamma 1.4\ndisable_nondim 1\nrho_fs 1.0\n"
+                    f"P_fs 1.0\nmach_fs 0.5\noutput_prefix batch\n")
+        z = zefr.Zefr()
+        z.read_input(deck)
+        z.setup_solver()
+        U = z.state("U", layout="device")
+        z.set_state(U * (1.0 + 0.05 * rng.standard_normal(U.shape)), layout="device")
+        d = z.dims()
+        dof += d["nEles"] * d["nSpts"] * d["nVars"]
+        zs.append(z)
+    b = zefr.Batch(zs)
+
+    def eager():
+        for z in zs:
+            z.compute_residual(0)
+        for z in zs:
+            z.synchronize()
+
+    def graph():
+        b.residual(sync=True)
+
+    out = {}
+    for name, fn in (("eager", eager), ("graph", graph)):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        out[name + "_ms"] = round((time.perf_counter() - t) / reps * 1e3, 4)
+    del b, zs
+    return {"workload": f"{count} independent periodic quad cases (n in 8..64, p in 1..5, seed {seed}), Euler, "
+                        "one residual each", "dof": dof, "eager_ms": out["eager_ms"], "graph_ms": out["graph_ms"],
+            "graph_Gdof_res_per_s": round(dof / (out["graph_ms"] * 1e-3) / 1e9, 3)}
+
+
 def roe_timing(workdir, mesh, order, steps, tile):
     """The Roe+LDG variant of the same Taylor-Green workload (the reference has no Roe
     flux; it is timed separately from the Rusanov+LDG headline): residual throughput and
@@ -492,6 +544,7 @@ def main():
         rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
     pmg = None if args.no_pmg else pmg_timing(workdir)
     roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps, tile)
+    batch = None if args.no_batch else batch_timing(workdir)
 
     rec = {
         "metric": METRIC,
@@ -524,6 +577,7 @@ def main():
         "rk4_step": rk,
         "pmg": pmg,
         "roe": roe,
+        "batch": batch,
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(workdir, args.cpu_n, order, args.cpu_steps)

@@ -201,6 +201,21 @@ int zefr_write_solution(zefr_t *z);
  * (interpolated when written at another order).  restart_type must be 2. */
 int zefr_restart_solution(zefr_t *z);
 
+/* Batched residuals of independent solvers (new; no reference counterpart): the
+ * compute_residual(0) launches of n set-up solvers captured once as one HIP graph, one
+ * branch per solver (fork and join through events on the batch's own stream), so that one
+ * graph launch evaluates all of them and the runtime overlaps the branches.  Meant for many
+ * small cases whose residuals are too small to fill the GPU or to outlast the launch cost.
+ * zefr_batch_residual launches the graph on the batch's stream and returns;
+ * zefr_batch_synchronize waits for it.  The member solvers must have no work pending when
+ * the batch is launched, and the batch must be recreated after a solver's tile changes
+ * (the graph holds the launch shapes).  Results land in each solver's divF(0). */
+typedef struct zefr_batch_s zefr_batch_t;
+zefr_batch_t *zefr_batch_create(zefr_t **solvers, int n);
+int zefr_batch_residual(zefr_batch_t *b);
+int zefr_batch_synchronize(zefr_batch_t *b);
+void zefr_batch_destroy(zefr_batch_t *b);
+
 /* ---------------------------------------------------------------- singleton ------ */
 
 /* zefr::initialize (include/zefr_interface.hpp:94, src/zefr_interface.cpp:27-33) */

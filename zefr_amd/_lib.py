@@ -117,6 +117,10 @@ SIGNATURES = {
     "zefr_residual_at": (_I, [_P, _P, _P, _P]),
     "zefr_residual_vjp": (_I, [_P, _P, _P, _P, _P]),
     "zefr_pmg_copy_state": (_I, [_P, _I, _I, _DP, C.c_long, _I]),
+    "zefr_batch_create": (_P, [C.POINTER(_P), _I]),
+    "zefr_batch_residual": (_I, [_P]),
+    "zefr_batch_synchronize": (_I, [_P]),
+    "zefr_batch_destroy": (None, [_P]),
     "zefr_write_solution": (_I, [_P]),
     "zefr_restart_solution": (_I, [_P]),
     "zefr_initialize": (_I, [_CP]),

@@ -397,6 +397,80 @@ int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph) {
   return zfr::guard([&] { z->s().set_step_mode(fuse_rk != 0, use_graph != 0); });
 }
 
+// ---------------------------------------------------------------- batches
+// Residuals of independent solvers as one HIP graph: captured once on the batch's own
+// stream, forking to every solver's stream (each solver's launches form one branch) and
+// joining back, so one graph launch runs all of them and the runtime overlaps branches.
+struct zefr_batch_s {
+  std::vector<zfr::FRSolver *> solvers;
+  hipStream_t stream = nullptr;
+  std::vector<hipEvent_t> events;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~zefr_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : events) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+zefr_batch_t *zefr_batch_create(zefr_t **zs, int n) {
+  zefr_batch_t *b = nullptr;
+  const int rc = zfr::guard([&] {
+    if (!zs || n < 1) zfr::raise("batch_create: need at least one solver");
+    auto batch = std::make_unique<zefr_batch_s>();
+    for (int i = 0; i < n; i++) {
+      if (!zs[i]) zfr::raise("batch_create: null solver");
+      zfr::FRSolver &s = zs[i]->s();
+      if (!s.device_ready) zfr::raise("batch_create: every solver must be set up on the device");
+      s.synchronize();
+      batch->solvers.push_back(&s);
+    }
+    zfr::hip_or_throw(hipStreamCreateWithFlags(&batch->stream, hipStreamNonBlocking), "hipStreamCreate");
+    batch->events.resize(n + 1, nullptr);
+    for (auto &e : batch->events) zfr::hip_or_throw(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    hipStream_t s0 = batch->stream;
+    zfr::hip_or_throw(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      zfr::hip_or_throw(hipEventRecord(batch->events[n], s0), "hipEventRecord");
+      for (int i = 0; i < n; i++) {
+        zfr::FRSolver &s = *batch->solvers[i];
+        zfr::hip_or_throw(hipStreamWaitEvent(s.stream, batch->events[n], 0), "hipStreamWaitEvent");
+        s.compute_residual(0);
+        zfr::hip_or_throw(hipEventRecord(batch->events[i], s.stream), "hipEventRecord");
+        zfr::hip_or_throw(hipStreamWaitEvent(s0, batch->events[i], 0), "hipStreamWaitEvent");
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s0, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    zfr::hip_or_throw(hipStreamEndCapture(s0, &batch->graph), "hipStreamEndCapture");
+    zfr::hip_or_throw(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    b = batch.release();
+  });
+  return rc ? nullptr : b;
+}
+
+int zefr_batch_residual(zefr_batch_t *b) {
+  return zfr::guard([&] {
+    if (!b) zfr::raise("batch_residual: null batch");
+    zfr::hip_or_throw(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+  });
+}
+
+int zefr_batch_synchronize(zefr_batch_t *b) {
+  return zfr::guard([&] {
+    if (!b) zfr::raise("batch_synchronize: null batch");
+    zfr::hip_or_throw(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+  });
+}
+
+void zefr_batch_destroy(zefr_batch_t *b) { delete b; }
+
 // ---------------------------------------------------------------- singleton
 int zefr_initialize(const char *input_file) {
   return zfr::guard([&] {

@@ -330,6 +330,34 @@ class Zefr:
         return buf
 
 
+class Batch:
+    """Residuals of several set-up solvers as one HIP graph launch (zefr_batch_*): each
+    solver's compute_residual(0) is one branch of the graph.  Recreate after set_tile."""
+
+    def __init__(self, solvers):
+        self._solvers = list(solvers)  # kept alive as long as the graph refers to them
+        arr = (C.c_void_p * len(self._solvers))(*[z._h for z in self._solvers])
+        self._h = lib().zefr_batch_create(arr, len(self._solvers))
+        if not self._h:
+            raise RuntimeError(last_error())
+
+    def residual(self, sync=True):
+        for z in self._solvers:
+            z.synchronize()
+        check(lib().zefr_batch_residual(self._h))
+        if sync:
+            self.synchronize()
+
+    def synchronize(self):
+        check(lib().zefr_batch_synchronize(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().zefr_batch_destroy(h)
+            self._h = None
+
+
 # ---------------------------------------------------------------- module-level (zefr namespace)
 _singleton = None
 
