@@ -166,7 +166,9 @@ int zefr_set_tile(zefr_t *z, int eles_per_block);
 /* How FRSolver::step_RK (src/solver.cpp:2865-3097) runs: fuse_rk folds each stage's RK
  * update into the residual's last kernel (default on); use_graph captures one whole time
  * step (the U_ini copy and every launch) as a HIP graph and replays it (default off;
- * recaptured when dt or the tile change).  Results are identical either way. */
+ * recaptured when dt or the tile change).  Under p-multigrid the settings apply to every
+ * level, and use_graph captures a whole V-cycle (all levels, transfers and updates) as one
+ * graph instead.  Results are identical either way. */
 int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph);
 /* Run `reps` residuals (stage 0) with HIP events around each kernel on the solver's
  * stream; ms_out[0..2] = average duration of the extrapolate / gradient / divergence

@@ -394,7 +394,13 @@ int zefr_set_tile(zefr_t *z, int eles_per_block) {
 }
 
 int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph) {
-  return zfr::guard([&] { z->s().set_step_mode(fuse_rk != 0, use_graph != 0); });
+  return zfr::guard([&] {
+    z->s().set_step_mode(fuse_rk != 0, use_graph != 0);
+    if (z->pmg) {
+      for (int n = 1; n < z->pmg->nLevels; n++) z->pmg->grids[n]->set_step_mode(fuse_rk != 0, false);
+      z->pmg->use_graph = use_graph != 0;
+    }
+  });
 }
 
 // ---------------------------------------------------------------- batches

@@ -231,9 +231,20 @@ class PMGrid {
   // oppPro maps level n to n-1 (src/quads.cpp:363-465, src/hexas.cpp:586-704)
   std::vector<std::vector<double>> oppRes, oppPro;
   std::deque<DevBuf> oppRes_d, oppPro_d;
+  // V-cycles replayed from one captured HIP graph (every level, transfer and update of
+  // the cycle); recaptured when dt, rel_fac, a tile or the fusion mode change
+  bool use_graph = false;
+  ~PMGrid();
 
  private:
   FRSolver &level(FRSolver &fine, int n) { return n == 0 ? fine : *grids[n]; }
+  void run_v(FRSolver &solver);
+  void drop_graph();
+  std::vector<double> graph_key(const FRSolver &solver) const;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<double> key;
+  std::vector<int> iters_per_cycle;  // updates of each level in one cycle (counters on replay)
   void transfer(const FRSolver &from, const FRSolver &to, const DevBuf &A, double alpha, const double *in,
                 double beta, double *out);
 };

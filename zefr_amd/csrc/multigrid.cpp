@@ -204,13 +204,83 @@ void PMGrid::v_cycle(FRSolver &solver, int lvl) {
   if (lvl != nLevels - 1) prolong_err(*grids[lvl + 1], correctionsBT[lvl + 1].as<double>(), solver);
 }
 
+PMGrid::~PMGrid() { drop_graph(); }
+
+void PMGrid::drop_graph() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+}
+
+std::vector<double> PMGrid::graph_key(const FRSolver &solver) const {
+  std::vector<double> k = {input->dt, input->rel_fac, (double)solver.eles->tile, (double)solver.fuse_rk};
+  for (int n = 1; n < nLevels; n++) {
+    k.push_back((double)grids[n]->eles->tile);
+    k.push_back((double)grids[n]->fuse_rk);
+  }
+  for (int n = 0; n < nLevels; n++) k.push_back((double)input->mg_steps[n]);
+  return k;
+}
+
+// One V-cycle: mg_steps[0] fine steps, then v_cycle from the fine level.
+void PMGrid::run_v(FRSolver &solver) {
+  for (unsigned s = 0; s < input->mg_steps[0]; s++) solver.update();
+  v_cycle(solver, 0);
+}
+
 // PMGrid::cycle (src/multigrid.cpp:263-333).  No history or solution files are written.
+// With use_graph a V-cycle is captured once on the fine stream (every level runs on it)
+// and replayed; the iteration counters and times the captured updates would have
+// advanced are advanced on the host for each replay.
 void PMGrid::cycle(FRSolver &solver) {
   if (!solver.device_ready) raise("solver is not set up on the device (call setup_solver)");
   const std::string type = input->mg_cycle;
+  if (type == "V" && use_graph) {
+    const auto k = graph_key(solver);
+    if (!graph_exec || k != key) {
+      drop_graph();
+      std::vector<FRSolver *> lv = {&solver};
+      for (int n = 1; n < nLevels; n++) lv.push_back(grids[n].get());
+      std::vector<bool> saved;
+      std::vector<int> before;
+      for (auto *g : lv) {
+        saved.push_back(g->use_graph);
+        g->use_graph = false;  // the levels' own step graphs are not nested into this one
+        before.push_back(g->current_iter);
+      }
+      hip_or_throw(hipStreamBeginCapture(solver.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        run_v(solver);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(solver.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        for (size_t i = 0; i < lv.size(); i++) lv[i]->use_graph = saved[i];
+        throw;
+      }
+      hip_or_throw(hipStreamEndCapture(solver.stream, &graph), "hipStreamEndCapture");
+      iters_per_cycle.clear();
+      for (size_t i = 0; i < lv.size(); i++) {
+        lv[i]->use_graph = saved[i];
+        iters_per_cycle.push_back(lv[i]->current_iter - before[i]);
+        lv[i]->current_iter = before[i];  // the capture ran nothing; the replay below does
+        lv[i]->flow_time -= iters_per_cycle.back() * input->dt;
+      }
+      hip_or_throw(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      key = k;
+    }
+    hip_or_throw(hipGraphLaunch(graph_exec, solver.stream), "hipGraphLaunch");
+    for (int n = 0; n < nLevels; n++) {
+      FRSolver &g = level(solver, n);
+      g.prev_time = g.flow_time;
+      g.current_iter += iters_per_cycle[n];
+      g.flow_time += iters_per_cycle[n] * input->dt;
+    }
+    return;
+  }
   if (type == "V") {
-    for (unsigned s = 0; s < input->mg_steps[0]; s++) solver.update();
-    v_cycle(solver, 0);
+    run_v(solver);
   } else if (type == "FMG") {
     for (int n = nLevels - 1; n > 0; n--) {
       for (unsigned c = 0; c < input->FMG_vcycles; c++) {
