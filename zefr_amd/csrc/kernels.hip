@@ -1023,6 +1023,13 @@ constexpr size_t kMaxLds = 64 * 1024;
 // the kernel needs per element (per_ele doubles).
 inline int eles_per_block(const zfr_ele_desc_t *d, int defEB, size_t fixed, size_t per_ele) {
   int eb = d->eles_per_block > 0 ? d->eles_per_block : defEB;
+  if (d->eles_per_block <= 0) {
+    // small meshes (coarse multigrid levels, small cases): keep >= 1024 workgroups, four
+    // per CU, so that every SIMD holds a wave; the kernels are latency-bound there, and
+    // partly filled waves cost less than idle SIMDs
+    const long per_cu4 = ((long)d->nEles + 1023) / 1024;
+    if (per_cu4 < eb) eb = per_cu4 < 1 ? 1 : (int)per_cu4;
+  }
   while (eb > 1 && (eb * d->nSpts > kMaxThreads || sizeof(double) * (fixed + eb * per_ele) > kMaxLds)) eb--;
   return eb < 1 ? 1 : eb;
 }
