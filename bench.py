@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--n", type=int, default=64, help="elements per direction")
     p.add_argument("--order", type=int, default=3)
     p.add_argument("--tile", type=int, default=-1, help="elements per workgroup (-1: autotune)")
+    p.add_argument("--tiles", default=None,
+                   help="per-launch tiles 'trace,flux,correction' (overrides --tile; default: autotune)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-vjp", action="store_true", help="skip the VJP timing")
     p.add_argument("--no-steps", action="store_true", help="skip the RK4 step timings")
@@ -402,7 +404,7 @@ def batch_timing(workdir, count=64, reps=20, seed=2024):
             "graph_Gdof_res_per_s": round(dof / (out["graph_ms"] * 1e-3) / 1e9, 3)}
 
 
-def roe_timing(workdir, mesh, order, steps, tile):
+def roe_timing(workdir, mesh, order, steps):
     """The Roe+LDG variant of the same Taylor-Green workload (the reference has no Roe
     flux; it is timed separately from the Rusanov+LDG headline): residual throughput and
     the three kernels' average durations, HIP events on the solver's stream."""
@@ -412,7 +414,7 @@ def roe_timing(workdir, mesh, order, steps, tile):
     z = zefr.Zefr()
     z.read_input(deck)
     z.setup_solver()
-    z.set_tile(tile)
+    z.autotune(3)
     d = z.dims()
     dof = d["nEles"] * d["nSpts"] * d["nVars"]
     for _ in range(3):
@@ -520,7 +522,13 @@ def main():
     E, nS, V = d["nEles"], d["nSpts"], d["nVars"]
     dof = E * nS * V
 
-    tile = z.autotune(3) if args.tile < 0 else (z.set_tile(args.tile) or args.tile)
+    if args.tiles:
+        z.set_tiles([int(x) for x in args.tiles.split(",")])
+    elif args.tile >= 0:
+        z.set_tile(args.tile)
+    else:
+        z.autotune(3)
+    tile = z.tiles()
 
     for _ in range(args.warmup):
         z.compute_residual(0)
@@ -553,7 +561,7 @@ def main():
     if not args.no_steps:
         rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
     pmg = None if args.no_pmg else pmg_timing(workdir)
-    roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps, tile)
+    roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps)
     batch = None if args.no_batch else batch_timing(workdir)
 
     rec = {
@@ -571,7 +579,7 @@ def main():
         "data": "synthetic: generated periodic hex mesh, Taylor-Green initial state",
         "config": {"workload": f"3D Navier-Stokes Taylor-Green residual, {n}^3 hexes on [0,2pi]^3, p={order}",
                    "elements": E, "spts_per_element": nS, "vars": V, "dof": dof, "Re": 1600, "Ma": 0.1,
-                   "Pr": 0.71, "flux": "Rusanov+LDG(beta=0.5,tau=0.1)", "tile_eles_per_block": tile,
+                   "Pr": 0.71, "flux": "Rusanov+LDG(beta=0.5,tau=0.1)", "tiles_eles_per_block": tile,
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

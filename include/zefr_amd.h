@@ -159,10 +159,16 @@ int zefr_get_fpt_maps(zefr_t *z, int *fpt2gfpt, signed char *slot);
  * "weights_spts", "ops1d", "U_init".  Returns the count; with out == NULL only the
  * count is returned. */
 long zefr_get_host_array(zefr_t *z, const char *name, double *out, long n);
-/* Pick the elements-per-workgroup tile for this solver's kernels by timing the
- * candidates on the device (0 = keep the built-in default). Returns the chosen value. */
+/* Pick the elements-per-workgroup tiles for this solver's kernels by timing the
+ * candidates on the device, each of a residual's three launches separately (see
+ * zefr_set_tiles).  Returns the tile of the middle launch. */
 int zefr_autotune(zefr_t *z, int reps);
 int zefr_set_tile(zefr_t *z, int eles_per_block);
+/* Per-launch tiles of a residual: tiles[0] the trace launch, [1] the gradient / one-sided
+ * flux launch, [2] the divergence / face-correction launch (what zefr_autotune picks, each
+ * by its own timing; 0 = the solver-wide tile).  get returns the tiles in effect. */
+int zefr_set_tiles(zefr_t *z, const int *tiles);
+int zefr_get_tiles(zefr_t *z, int *tiles);
 /* How FRSolver::step_RK (src/solver.cpp:2865-3097) runs: fuse_rk folds each stage's RK
  * update into the residual's last kernel (default on); use_graph captures one whole time
  * step (the U_ini copy and every launch) as a HIP graph and replays it (default off;

@@ -112,6 +112,8 @@ SIGNATURES = {
     "zefr_get_host_array": (C.c_long, [_P, _CP, _DP, C.c_long]),
     "zefr_autotune": (_I, [_P, _I]),
     "zefr_set_tile": (_I, [_P, _I]),
+    "zefr_set_tiles": (_I, [_P, _IP]),
+    "zefr_get_tiles": (_I, [_P, _IP]),
     "zefr_set_step_mode": (_I, [_P, _I, _I]),
     "zefr_time_residual_kernels": (_I, [_P, _I, _DP]),
     "zefr_residual_at": (_I, [_P, _P, _P, _P]),

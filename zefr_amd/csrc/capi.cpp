@@ -390,6 +390,21 @@ int zefr_set_tile(zefr_t *z, int eles_per_block) {
   return zfr::guard([&] {
     if (eles_per_block < 0) zfr::raise("tile must be >= 0");
     z->el().tile = eles_per_block;
+    for (int &t : z->el().tile_phase) t = 0;
+  });
+}
+
+int zefr_set_tiles(zefr_t *z, const int *tiles) {
+  return zfr::guard([&] {
+    for (int k = 0; k < 3; k++)
+      if (tiles[k] < 0) zfr::raise("tiles must be >= 0");
+    for (int k = 0; k < 3; k++) z->el().tile_phase[k] = tiles[k];
+  });
+}
+
+int zefr_get_tiles(zefr_t *z, int *tiles) {
+  return zfr::guard([&] {
+    for (int k = 0; k < 3; k++) tiles[k] = z->el().tile_for(k);
   });
 }
 

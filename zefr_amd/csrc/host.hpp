@@ -115,7 +115,12 @@ class Elements {
   static double adjugate(int D, const double J[3][3], double A[3][3]);
   std::vector<double> initial_state(const zefr_input_t &in) const;  // [E][V][nS]
   std::vector<double> export_array(const std::string &name) const;  // reference layouts
-  zfr_ele_desc_t desc(const zefr_input_t &in) const;
+  // phase: 0 trace, 1 gradient / one-sided flux, 2 divergence / face correction (the
+  // autotuner's per-launch tiles); -1 = the solver-wide tile
+  zfr_ele_desc_t desc(const zefr_input_t &in, int phase = -1) const;
+  int tile_phase[3] = {0, 0, 0};  // per-launch tiles picked by FRSolver::autotune (0 = tile)
+  int tile_for(int phase) const { return (phase >= 0 && phase < 3 && tile_phase[phase] > 0) ? tile_phase[phase] : tile; }
+  int tile_key() const { return tile + 64 * (tile_phase[0] + 64 * (tile_phase[1] + 64 * tile_phase[2])); }
   size_t state_size() const { return (size_t)nEles * nVars * nSpts; }
 };
 

@@ -207,12 +207,12 @@ std::vector<double> Elements::initial_state(const zefr_input_t &in) const {
   return U;
 }
 
-zfr_ele_desc_t Elements::desc(const zefr_input_t &in) const {
+zfr_ele_desc_t Elements::desc(const zefr_input_t &in, int phase) const {
   zfr_ele_desc_t d;
   std::memset(&d, 0, sizeof(d));
   d.nDims = nDims; d.order = order; d.nVars = nVars; d.nEles = nEles; d.nSpts = nSpts; d.nFpts = nFpts;
   d.viscous = in.viscous;
-  d.eles_per_block = tile;
+  d.eles_per_block = tile_for(phase);
   d.gamma = in.gamma; d.prandtl = in.prandtl; d.mu = in.mu; d.rt = in.rt; d.c_sth = in.c_sth;
   d.fix_vis = in.fix_vis;
   d.rus_k = in.rus_k; d.ldg_b = in.ldg_b; d.ldg_tau = in.ldg_tau;
@@ -373,7 +373,7 @@ bool FRSolver::onesided() const {
 void FRSolver::compute_residual_start(unsigned, int color) {
   need_device(*this);
   if (color > -1) raise("multi-colour (MCGS) residuals are not supported");
-  const auto d = eles->desc(*input);
+  const auto d = eles->desc(*input, 0);
   if (onesided()) status_or_throw(zfr_extrapolate_partner(&d, eles->U_spts_d.as<double>(), stream));
   else status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
 }
@@ -381,7 +381,7 @@ void FRSolver::compute_residual_start(unsigned, int color) {
 void FRSolver::compute_residual_mid(unsigned stage, int) {
   if (!input->viscous) return;
   if (stage >= input->nStages) raise("stage out of range");
-  const auto d = eles->desc(*input);
+  const auto d = eles->desc(*input, 1);
   if (onesided()) {
     double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
     status_or_throw(zfr_gradient_flux(&d, eles->U_spts_d.as<double>(), divF, stream));
@@ -392,7 +392,7 @@ void FRSolver::compute_residual_mid(unsigned stage, int) {
 
 void FRSolver::compute_residual_finish(unsigned stage, int) {
   if (stage >= input->nStages) raise("stage out of range");
-  const auto d = eles->desc(*input);
+  const auto d = eles->desc(*input, 2);
   double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
   if (onesided()) status_or_throw(zfr_face_correction(&d, divF, stream));
   else status_or_throw(zfr_divergence(&d, eles->U_spts_d.as<double>(), divF, stream));
@@ -407,15 +407,15 @@ void FRSolver::compute_residual(unsigned stage, int color) {
 
 void FRSolver::residual_at(const double *U, double *divF, hipStream_t st) {
   need_device(*this);
-  const auto d = eles->desc(*input);
+  const auto d0 = eles->desc(*input, 0), d1 = eles->desc(*input, 1), d2 = eles->desc(*input, 2);
   if (onesided()) {
-    status_or_throw(zfr_extrapolate_partner(&d, U, st));
-    status_or_throw(zfr_gradient_flux(&d, U, divF, st));
-    status_or_throw(zfr_face_correction(&d, divF, st));
+    status_or_throw(zfr_extrapolate_partner(&d0, U, st));
+    status_or_throw(zfr_gradient_flux(&d1, U, divF, st));
+    status_or_throw(zfr_face_correction(&d2, divF, st));
   } else {
-    status_or_throw(zfr_extrapolate(&d, U, st));
-    if (input->viscous) status_or_throw(zfr_gradient(&d, U, nullptr, st));
-    status_or_throw(zfr_divergence(&d, U, divF, st));
+    status_or_throw(zfr_extrapolate(&d0, U, st));
+    if (input->viscous) status_or_throw(zfr_gradient(&d1, U, nullptr, st));
+    status_or_throw(zfr_divergence(&d2, U, divF, st));
   }
 }
 
@@ -464,7 +464,7 @@ void FRSolver::rk_final_update() {
 
 // compute_residual_finish with the RK update of this stage done by the same kernel.
 void FRSolver::finish_with_rk(int stage) {
-  const auto d = eles->desc(*input);
+  const auto d = eles->desc(*input, 2);
   const int ns = (int)input->nStages;
   const size_t st = eles->state_size();
   zfr_rk_args_t r{};
@@ -527,7 +527,7 @@ void FRSolver::step_RK() {
     step_RK_eager();
     return;
   }
-  if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile || graph_fused != fuse_rk ||
+  if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile_key() || graph_fused != fuse_rk ||
       graph_source != rk_source) {
     drop_graph();
     hip_or_throw(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -542,7 +542,7 @@ void FRSolver::step_RK() {
     hip_or_throw(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
     hip_or_throw(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     graph_dt = input->dt;
-    graph_tile = eles->tile;
+    graph_tile = eles->tile_key();
     graph_fused = fuse_rk;
     graph_source = rk_source;
   }
@@ -625,6 +625,10 @@ void FRSolver::report_residuals(double *res) {
 }
 
 // Time the residual with each feasible tile and keep the fastest.
+// Workgroup tiles by measurement: every candidate (1, 2, 4, ... elements per workgroup up
+// to 256 threads, and the kernels' default) is timed per launch with time_kernels, twice
+// in alternation; each of the residual's three launches keeps its own fastest tile.
+// Returns the tile of the middle (dominant) launch.
 int FRSolver::autotune(int reps) {
   need_device(*this);
   Elements &el = *eles;
@@ -632,33 +636,29 @@ int FRSolver::autotune(int reps) {
   for (int eb = 1; eb * el.nSpts <= 256; eb *= 2) cands.push_back(eb);
   const int def = std::max(1, 256 / el.nSpts);
   if (std::find(cands.begin(), cands.end(), def) == cands.end()) cands.push_back(def);
-  hipEvent_t a, b;
-  hip_or_throw(hipEventCreate(&a), "hipEventCreate");
-  hip_or_throw(hipEventCreate(&b), "hipEventCreate");
-  double best = 1e300;
-  int best_eb = el.tile;
-  for (int eb : cands) {
-    el.tile = eb;
-    compute_residual(0);
-    hip_or_throw(hipEventRecord(a, stream), "hipEventRecord");
-    for (int r = 0; r < reps; r++) compute_residual(0);
-    hip_or_throw(hipEventRecord(b, stream), "hipEventRecord");
-    hip_or_throw(hipEventSynchronize(b), "hipEventSynchronize");
-    float ms = 0.f;
-    hip_or_throw(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-    if (ms < best) { best = ms; best_eb = eb; }
-  }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  el.tile = best_eb;
-  return best_eb;
+  el.tile_phase[0] = el.tile_phase[1] = el.tile_phase[2] = 0;
+  double best[3] = {1e300, 1e300, 1e300};
+  int best_eb[3] = {el.tile, el.tile, el.tile};
+  for (int pass = 0; pass < 2; pass++)
+    for (int eb : cands) {
+      el.tile = eb;
+      double ms[3];
+      time_kernels(1, ms);  // warm-up at this shape
+      time_kernels(std::max(reps, 1), ms);
+      for (int k = 0; k < 3; k++)
+        if (ms[k] > 0.0 && ms[k] < best[k]) { best[k] = ms[k]; best_eb[k] = eb; }
+    }
+  if (best[1] == 1e300) best_eb[1] = best_eb[2];  // inviscid: no middle launch
+  el.tile = best_eb[1];
+  for (int k = 0; k < 3; k++) el.tile_phase[k] = best_eb[k];
+  return best_eb[1];
 }
 
 // Residuals with an event pair around every launch (benchmark instrumentation).
 void FRSolver::time_kernels(int reps, double *ms_out) {
   need_device(*this);
   Elements &el = *eles;
-  const auto d = el.desc(*input);
+  const auto d0 = el.desc(*input, 0), d1 = el.desc(*input, 1), d2 = el.desc(*input, 2);
   reps = std::max(reps, 1);
   // one event before each launch and one after the last: residuals run back to back
   std::vector<hipEvent_t> ev(3 * reps + 1);
@@ -666,14 +666,14 @@ void FRSolver::time_kernels(int reps, double *ms_out) {
   double *U = el.U_spts_d.as<double>(), *divF = el.divF_d.as<double>();
   for (int r = 0; r < reps; r++) {
     hip_or_throw(hipEventRecord(ev[3 * r], stream), "hipEventRecord");
-    if (onesided()) status_or_throw(zfr_extrapolate_partner(&d, U, stream));
-    else status_or_throw(zfr_extrapolate(&d, U, stream));
+    if (onesided()) status_or_throw(zfr_extrapolate_partner(&d0, U, stream));
+    else status_or_throw(zfr_extrapolate(&d0, U, stream));
     hip_or_throw(hipEventRecord(ev[3 * r + 1], stream), "hipEventRecord");
-    if (onesided()) status_or_throw(zfr_gradient_flux(&d, U, divF, stream));
-    else if (input->viscous) status_or_throw(zfr_gradient(&d, U, nullptr, stream));
+    if (onesided()) status_or_throw(zfr_gradient_flux(&d1, U, divF, stream));
+    else if (input->viscous) status_or_throw(zfr_gradient(&d1, U, nullptr, stream));
     hip_or_throw(hipEventRecord(ev[3 * r + 2], stream), "hipEventRecord");
-    if (onesided()) status_or_throw(zfr_face_correction(&d, divF, stream));
-    else status_or_throw(zfr_divergence(&d, U, divF, stream));
+    if (onesided()) status_or_throw(zfr_face_correction(&d2, divF, stream));
+    else status_or_throw(zfr_divergence(&d2, U, divF, stream));
   }
   hip_or_throw(hipEventRecord(ev[3 * reps], stream), "hipEventRecord");
   hip_or_throw(hipEventSynchronize(ev[3 * reps]), "hipEventSynchronize");

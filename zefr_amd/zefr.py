@@ -187,6 +187,16 @@ class Zefr:
     def set_tile(self, eles_per_block):
         check(lib().zefr_set_tile(self._h, eles_per_block))
 
+    def set_tiles(self, tiles):
+        """Per-launch workgroup tiles (trace, gradient/one-sided, divergence/face correction)."""
+        arr = (C.c_int * 3)(*tiles)
+        check(lib().zefr_set_tiles(self._h, arr))
+
+    def tiles(self):
+        arr = (C.c_int * 3)()
+        check(lib().zefr_get_tiles(self._h, arr))
+        return list(arr)
+
     def set_step_mode(self, fuse_rk=True, use_graph=False):
         """RK update fused into each residual's last kernel; whole step as a HIP graph."""
         check(lib().zefr_set_step_mode(self._h, int(fuse_rk), int(use_graph)))
