@@ -209,6 +209,9 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
   const int cnt = __popc(mask) * S::nFF;
 #pragma unroll
   for (int kk = 0; kk < S::FPT; kk++) {
+    // viscous runs use one element per workgroup, so cnt is the same for every thread and
+    // a pass with no owned point left (and its barriers) is skipped by all of them
+    if (VISC && kk > 0 && kk * nS >= cnt) break;
     const int k = t + kk * nS;
     const bool has = valid && k < cnt;
     int f = 0, dim = 0, side = 0, base = 0, str = 0;
