@@ -404,10 +404,11 @@ def batch_timing(workdir, count=64, reps=20, seed=2024):
             "graph_Gdof_res_per_s": round(dof / (out["graph_ms"] * 1e-3) / 1e9, 3)}
 
 
-def roe_timing(workdir, mesh, order, steps):
+def roe_timing(workdir, mesh, order, steps, with_vjp=True):
     """The Roe+LDG variant of the same Taylor-Green workload (the reference has no Roe
     flux; it is timed separately from the Rusanov+LDG headline): residual throughput and
-    the three kernels' average durations, HIP events on the solver's stream."""
+    the three kernels' average durations, HIP events on the solver's stream, and the Roe
+    residual's vector-Jacobian product against its forward."""
     from zefr_amd import zefr
     from zefr_amd._lib import lib
     deck = write_deck(os.path.join(workdir, "tgv_roe.deck"), mesh, order, fconv="Roe")
@@ -427,12 +428,14 @@ def roe_timing(workdir, mesh, order, steps):
     wall = time.perf_counter() - t
     if rc != 0:
         raise RuntimeError(lib().zefr_last_error().decode())
+    vjp = vjp_timing(z, steps) if with_vjp else None
     del z
     return {"workload": "3D Navier-Stokes Taylor-Green residual with Roe+LDG (Harten entropy fix 0.1 a), same mesh",
             "value": round(dof / (wall / steps) / 1e9, 4), "unit": "Gdof-residuals/s",
             "ms_per_step": round(wall / steps * 1e3, 4),
             "kernels_ms": {"extrapolate": round(ms3[0], 4), "gradient_flux": round(ms3[1], 4),
-                           "face_correction": round(ms3[2], 4)}}
+                           "face_correction": round(ms3[2], 4)},
+            "vjp": vjp}
 
 
 def vjp_timing(z, reps):
@@ -561,7 +564,7 @@ def main():
     if not args.no_steps:
         rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
     pmg = None if args.no_pmg else pmg_timing(workdir)
-    roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps)
+    roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps, not args.no_vjp)
     batch = None if args.no_batch else batch_timing(workdir)
 
     rec = {

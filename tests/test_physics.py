@@ -25,8 +25,9 @@ def dp(a):
     return a.ctypes.data_as(DP)
 
 
-def phys(gamma=1.4, prandtl=0.72, mu=2e-2, rt=0.7, c_sth=0.3, fix_vis=1, rus_k=0.0, ldg_b=0.5, ldg_tau=0.1):
-    return _lib.ZfrPhys(gamma, prandtl, mu, rt, c_sth, fix_vis, rus_k, ldg_b, ldg_tau)
+def phys(gamma=1.4, prandtl=0.72, mu=2e-2, rt=0.7, c_sth=0.3, fix_vis=1, rus_k=0.0, ldg_b=0.5, ldg_tau=0.1,
+         fconv_type=0):
+    return _lib.ZfrPhys(gamma, prandtl, mu, rt, c_sth, fix_vis, rus_k, ldg_b, ldg_tau, fconv_type)
 
 
 def state(D, rng):
@@ -155,10 +156,13 @@ def test_point_flux_vjp_matches_finite_differences(D, viscous, fix_vis):
 
 
 @pytest.mark.parametrize("D", [2, 3])
-@pytest.mark.parametrize("visc,beta,fix_vis", [(False, 0.5, 1), (True, 0.5, 1), (True, -0.5, 0), (True, 0.3, 1)])
-def test_common_flux_vjp_matches_finite_differences(D, visc, beta, fix_vis):
-    rng = np.random.default_rng(11 * D + int(10 * beta))
-    a = phys(ldg_b=beta, fix_vis=fix_vis, rus_k=0.1)
+@pytest.mark.parametrize("visc,beta,fix_vis,fconv", [(False, 0.5, 1, 0), (True, 0.5, 1, 0), (True, -0.5, 0, 0),
+                                                     (True, 0.3, 1, 0), (False, 0.5, 1, 2), (True, 0.5, 1, 2),
+                                                     (True, -0.5, 0, 2)])
+def test_common_flux_vjp_matches_finite_differences(D, visc, beta, fix_vis, fconv):
+    """Rusanov (fconv 0) and Roe (fconv 2, the reference has none) common fluxes."""
+    rng = np.random.default_rng(11 * D + int(10 * beta) + fconv)
+    a = phys(ldg_b=beta, fix_vis=fix_vis, rus_k=0.1, fconv_type=fconv)
     V = D + 2
     for trial in range(6):
         UL, UR = state(D, rng), state(D, rng)
@@ -182,6 +186,34 @@ def test_common_flux_vjp_matches_finite_differences(D, visc, beta, fix_vis):
             assert abs(Fb @ col - grad[k]) < 1e-8 * max(1.0, abs(grad[k])), (trial, k)
         d = rng.standard_normal(x0.size)
         assert abs(Fb @ fd4(f, x0, d, 1e-4) - grad @ d) < 1e-8 * max(1.0, abs(grad @ d))
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_roe_vjp_inside_the_entropy_fix(D):
+    """Transonic states put an acoustic speed inside Harten's fix (|lambda| < 0.1 a), where
+    the flux is smooth but follows the parabola: the VJP still matches differences."""
+    rng = np.random.default_rng(70 + D)
+    a = phys(fconv_type=2)
+    V = D + 2
+    for _ in range(4):
+        n = rng.standard_normal(D)
+        n /= np.linalg.norm(n)
+        c = np.sqrt(1.4)
+        UL = np.concatenate([[1.0], (0.995 * c) * n, [1.0 / 0.4 + 0.5 * 0.995 ** 2 * 1.4]])
+        UR = np.concatenate([[1.01], 1.01 * (1.002 * c) * n, [1.005 / 0.4 + 0.5 * 1.01 * 1.002 ** 2 * 1.4]])
+        Fb = rng.standard_normal(V)
+        z = np.zeros((V, D))
+        ULb, URb, _, _ = host_common_vjp(D, a, False, UL, UR, z, z, n, 1.0, Fb)
+        x0 = np.concatenate([UL, UR])
+
+        def f(x):
+            return host_common(D, a, False, x[:V], x[V:], z, z, n, 1.0)
+        grad = np.concatenate([ULb, URb])
+        for k in range(2 * V):
+            e = np.zeros(2 * V)
+            e[k] = 1.0
+            col = fd4(f, x0, e, 1e-6)
+            assert abs(Fb @ col - grad[k]) < 1e-7 * max(1.0, abs(grad[k])), k
 
 
 def test_vjp_check_rejects_a_planted_sign_error():

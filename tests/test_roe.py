@@ -176,12 +176,3 @@ def test_product_roe_textbook_properties(D):
     F = _product_roe(D, UL, UR, n)
     Fr = _product_roe(D, rot(UL), rot(UR), Rm @ n)
     assert np.abs(Fr - rot(F)).max() <= 1e-13 * np.abs(F).max()
-
-
-def test_roe_vjp_is_rejected():
-    from zefr_amd import _lib
-    phys = _lib.ZfrPhys(gamma=GAMMA, prandtl=0.72, fix_vis=1, ldg_b=0.5, ldg_tau=0.1, fconv_type=2)
-    z = np.zeros(5)
-    p = z.ctypes.data_as(C.POINTER(C.c_double))
-    rc = _lib.lib().zfr_host_common_flux_vjp(3, C.byref(phys), 0, p, p, None, None, p, 1.0, p, p, p, None, None)
-    assert rc != 0 and "Rusanov" in _lib.last_error()

@@ -91,7 +91,7 @@ struct AdjLds {
   static constexpr int per_ele = S::V * S::nS + R + GB + S::V * S::nF + 1;  // doubles
 };
 
-template <int D, int N, bool VISC, int BM>
+template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
 void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
                 double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
@@ -236,8 +236,8 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
           Fb[v] = Wtr[(e * V + v) * nF + f] + Wtr[(ep * V + v) * nF + fp];
           Uob[v] = Upb[v] = Gb[v] = 0.0;
         }
-        if (owner == 0) common_flux_vjp<D, VISC, BM>(Uo, Up, n, 1.0, a, Fb, Uob, Upb, Gb, nullptr);
-        else common_flux_vjp<D, VISC, BM>(Up, Uo, n, 1.0, a, Fb, Upb, Uob, Gb, nullptr);
+        if (owner == 0) common_flux_vjp<D, VISC, BM, FC>(Uo, Up, n, 1.0, a, Fb, Uob, Upb, Gb, nullptr);
+        else common_flux_vjp<D, VISC, BM, FC>(Up, Uo, n, 1.0, a, Fb, Upb, Uob, Gb, nullptr);
 #pragma unroll
         for (int v = 0; v < V; v++) Xtr[(e * V + v) * nF + f] = Upb[v];
       }

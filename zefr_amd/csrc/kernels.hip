@@ -1066,6 +1066,18 @@ struct AdjPtrs {
   const KArgs::Rk *rk = nullptr;  // fused RK update for Div / FaceCorr
 };
 
+// k_adj_main for one convective flux (FC: 0 Rusanov, 1 Roe) and the LDG beta branch.
+template <int D, int N, int FC>
+void launch_adj_main(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds, hipStream_t st, const double *U,
+                     const AdjPtrs &j, double *out, const KArgs &a) {
+  if (!d->viscous)
+    hipLaunchKernelGGL((k_adj_main<D, N, false, 0, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
+  else if (d->ldg_b == 0.5)
+    hipLaunchKernelGGL((k_adj_main<D, N, true, 1, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
+  else
+    hipLaunchKernelGGL((k_adj_main<D, N, true, 2, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
+}
+
 template <int D, int N>
 int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j = AdjPtrs()) {
   using S = Shape<D, N>;
@@ -1093,7 +1105,6 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   const bool adj = op == Op::AdjPrep || op == Op::AdjMain || op == Op::AdjGather;
   if (adj && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5)
     return zfr::fail("the residual VJP supports LDG beta = +-0.5 (and Euler)");
-  if (adj && d->fconv_type != 0) return zfr::fail("the residual VJP supports the Rusanov flux only");
   if (d->fconv_type != 0 && d->fconv_type != 2) return zfr::fail("fconv_type must be 0 (Rusanov) or 2 (Roe)");
   if ((op == Op::OneSided || op == Op::FaceCorr) && d->ldg_b != 0.5 && d->ldg_b != -0.5)
     return zfr::fail("one-sided LDG kernels need ldg_b = +-0.5");
@@ -1141,9 +1152,8 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       hipLaunchKernelGGL((k_adj_prep<D, N>), grid, block, lds, st, U, j.w, j.Wtr, a);
       break;
     case Op::AdjMain:
-      if (!d->viscous) hipLaunchKernelGGL((k_adj_main<D, N, false, 0>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-      else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_adj_main<D, N, true, 1>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-      else hipLaunchKernelGGL((k_adj_main<D, N, true, 2>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
+      if (d->fconv_type == 2) launch_adj_main<D, N, 1>(d, grid, block, lds, st, U, j, out, a);
+      else launch_adj_main<D, N, 0>(d, grid, block, lds, st, U, j, out, a);
       break;
     case Op::AdjGather:
       if (d->viscous && d->ldg_b == -0.5) hipLaunchKernelGGL((k_adj_gather<D, N, 2>), grid, block, lds, st, j.Xtr, out, a);

@@ -412,6 +412,130 @@ ZFR_HD void roe_dissipation(const double *UL, const double *UR, const Prim<D> &p
   diss[V - 1] = w1 * (H - a * un) + w2 * 0.5 * q2 + w3 * (H + a * un) + l2 * rho * ut_du;
 }
 
+// Reverse of roe_dissipation: ULb += (d diss / d UL)^T db, URb likewise.  Derived by hand
+// from the forward statements above, in reverse order; the kinks (|lambda| at the entropy
+// fix threshold and at 0) take the branch the forward evaluation took.
+template <int D>
+ZFR_HD void roe_dissipation_vjp(const double *UL, const double *UR, const Prim<D> &pL, const Prim<D> &pR,
+                                const double *n, double gamma, const double *db, double *ULb, double *URb) {
+  constexpr int V = D + 2;
+  // forward recompute
+  const double sL = sqrt(UL[0]), sR = sqrt(UR[0]);
+  const double wden = 1.0 / (sL + sR);
+  double u[D], un = 0.0, q2 = 0.0, unL = 0.0, unR = 0.0;
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++) {
+    u[d] = (sL * pL.vel[d] + sR * pR.vel[d]) * wden;
+    un += u[d] * n[d];
+    q2 += u[d] * u[d];
+    unL += pL.vel[d] * n[d];
+    unR += pR.vel[d] * n[d];
+  }
+  const double H = (sL * pL.H + sR * pR.H) * wden;
+  const double rho = sL * sR;
+  const double a2 = (gamma - 1.0) * (H - 0.5 * q2);
+  const double a = sqrt(a2);
+  const double dr = UR[0] - UL[0], dp = pR.P - pL.P, dun = unR - unL;
+  const double delta = kRoeEntropyFix * a;
+  const double lam1 = un - a, lam3 = un + a;
+  const double l1 = harten_abs(lam1, delta), l2 = fabs(un), l3 = harten_abs(lam3, delta);
+  const double A1 = 0.5 * (dp - rho * a * dun) / a2, A3 = 0.5 * (dp + rho * a * dun) / a2;
+  const double A2 = dr - dp / a2;
+  const double w1 = l1 * A1, w3 = l3 * A3, w2 = l2 * A2;
+  double dt[D], ut = 0.0;
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++) {
+    dt[d] = (pR.vel[d] - pL.vel[d]) - dun * n[d];
+    ut += u[d] * dt[d];
+  }
+  // reverse: outputs
+  const double dbE = db[V - 1];
+  double bw1 = db[0] + dbE * (H - a * un), bw2 = db[0] + dbE * 0.5 * q2, bw3 = db[0] + dbE * (H + a * un);
+  double sh = dbE * ut, bu[D], bdt[D];
+  double ba = dbE * (w3 - w1) * un, bun = dbE * (w3 - w1) * a, bH = dbE * (w1 + w3), bq2 = dbE * 0.5 * w2;
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++) {
+    const double b = db[d + 1];
+    bw1 += b * (u[d] - a * n[d]);
+    bw2 += b * u[d];
+    bw3 += b * (u[d] + a * n[d]);
+    sh += b * dt[d];
+    ba += b * (w3 - w1) * n[d];
+    bu[d] = b * (w1 + w2 + w3) + dbE * l2 * rho * dt[d];
+    bdt[d] = l2 * rho * (b + dbE * u[d]);
+  }
+  double bl2 = rho * sh, brho = l2 * sh;
+  // w = l A
+  const double bl1 = bw1 * A1, bA1 = bw1 * l1, bl3 = bw3 * A3, bA3 = bw3 * l3;
+  bl2 += bw2 * A2;
+  const double bA2 = bw2 * l2;
+  // wave strengths
+  double bdp = 0.5 * (bA1 + bA3) / a2 - bA2 / a2;
+  double bdun = 0.5 * rho * a * (bA3 - bA1) / a2;
+  brho += 0.5 * a * dun * (bA3 - bA1) / a2;
+  ba += 0.5 * rho * dun * (bA3 - bA1) / a2;
+  double ba2 = -(A1 * bA1 + A3 * bA3) / a2 + dp * bA2 / (a2 * a2);
+  const double bdr = bA2;
+  // |lambda| with the entropy fix
+  auto dabs = [&](double lam, double &bdel) {  // d habs / d lam; accumulates d / d delta
+    if (fabs(lam) < delta) {
+      bdel += (delta * delta - lam * lam) / (2.0 * delta * delta);
+      return lam / delta;
+    }
+    return lam >= 0.0 ? 1.0 : -1.0;
+  };
+  double dd1 = 0.0, dd3 = 0.0;
+  const double g1 = dabs(lam1, dd1), g3 = dabs(lam3, dd3);
+  bun += bl1 * g1 + bl3 * g3 + bl2 * (un >= 0.0 ? 1.0 : -1.0);
+  ba += -bl1 * g1 + bl3 * g3 + kRoeEntropyFix * (bl1 * dd1 + bl3 * dd3);
+  // shear jumps
+  double bvL[D], bvR[D];
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++) {
+    bvR[d] = bdt[d];
+    bvL[d] = -bdt[d];
+    bdun -= bdt[d] * n[d];
+  }
+  // a, a2, H, q2, un, u
+  ba2 += ba / (2.0 * a);
+  bH += (gamma - 1.0) * ba2;
+  bq2 -= 0.5 * (gamma - 1.0) * ba2;
+  double bsL = sR * brho, bsR = sL * brho;
+  double bw = (sL * pL.H + sR * pR.H) * bH;
+  const double bHL = sL * wden * bH, bHR = sR * wden * bH;
+  bsL += pL.H * wden * bH;
+  bsR += pR.H * wden * bH;
+  ZFR_UNROLL
+  for (int d = 0; d < D; d++) {
+    bu[d] += 2.0 * u[d] * bq2 + n[d] * bun;
+    bvL[d] += n[d] * (-bdun) + sL * wden * bu[d];
+    bvR[d] += n[d] * bdun + sR * wden * bu[d];
+    bsL += pL.vel[d] * wden * bu[d];
+    bsR += pR.vel[d] * wden * bu[d];
+    bw += (sL * pL.vel[d] + sR * pR.vel[d]) * bu[d];
+  }
+  bsL -= wden * wden * bw;
+  bsR -= wden * wden * bw;
+  // primitives -> conservative, each side: rho, v = m/rho, P = (g-1)(E - |m|^2/(2 rho)), H = (E+P)/rho
+  auto side = [&](const double *U, const Prim<D> &p, double bs, double brho_dir, double bP, const double *bv,
+                  double bHs, double *Ub) {
+    const double bPt = bP + bHs * p.invrho;
+    double vv = 0.0;
+    ZFR_UNROLL
+    for (int d = 0; d < D; d++) vv += p.vel[d] * p.vel[d];
+    double br = brho_dir + bs / (2.0 * sqrt(U[0])) - bHs * p.H * p.invrho + bPt * (gamma - 1.0) * 0.5 * vv;
+    ZFR_UNROLL
+    for (int d = 0; d < D; d++) {
+      br -= bv[d] * p.vel[d] * p.invrho;
+      Ub[d + 1] += bv[d] * p.invrho - bPt * (gamma - 1.0) * p.vel[d];
+    }
+    Ub[0] += br;
+    Ub[V - 1] += bHs * p.invrho + bPt * (gamma - 1.0);
+  };
+  side(UL, pL, bsL, -bdr, -bdp, bvL, bHL, ULb);
+  side(UR, pR, bsR, bdr, bdp, bvR, bHR, URb);
+}
+
 // LDG beta branches of src/faces.cpp:1520-1545, fixed at compile time: BM = 0 any beta
 // (both sides' gradients), 1 beta = +1/2 (left side only), 2 beta = -1/2 (right only).
 // Common normal flux for this element's side of a flux point: Rusanov
@@ -491,10 +615,31 @@ ZFR_HD void common_flux(const double *UL, const double *UR, const double *G1, co
   }
 }
 
+// Reverse of the LDG part of common_flux for the area-scaled cotangent Fs.
+template <int D, int BM, typename P>
+ZFR_HD void ldg_vjp_add(const double *Fs, const P &a, double *ULb, double *URb, double *G1b, double *G2b) {
+  constexpr int V = D + 2;
+  const double beta = a.ldg_b, tau = a.ldg_tau;
+  ZFR_UNROLL
+  for (int v = 0; v < V; v++) {
+    ULb[v] += tau * Fs[v];
+    URb[v] -= tau * Fs[v];
+    const double GLb = (0.5 + beta) * Fs[v], GRb = (0.5 - beta) * Fs[v];
+    if (BM == 0) {
+      G1b[v] += GLb;
+      G2b[v] += GRb;
+    } else if (BM == 1) {
+      G1b[v] += GLb;
+    } else {
+      G1b[v] += GRb;
+    }
+  }
+}
+
 // Reverse of common_flux for the cotangent Fb of Fout: adds to ULb, URb and, when
 // viscous, to G1b (and G2b for BM 0).  The wave-speed maximum passes its cotangent to
 // the larger side (the left one on a tie, as fmax keeps its first argument).
-template <int D, bool VISC, int BM, typename P>
+template <int D, bool VISC, int BM, int FC = 0, typename P>
 ZFR_HD void common_flux_vjp(const double *UL, const double *UR, const double *n, double dAs, const P &a,
                             const double *Fb, double *ULb, double *URb, double *G1b, double *G2b) {
   constexpr int V = D + 2;
@@ -507,6 +652,14 @@ ZFR_HD void common_flux_vjp(const double *UL, const double *UR, const double *n,
   }
   conv_proj_vjp<D>(UL, pL, n, Fh, a.gamma, ULb);
   conv_proj_vjp<D>(UR, pR, n, Fh, a.gamma, URb);
+  if (FC == 1) {
+    double db[V];
+    ZFR_UNROLL
+    for (int v = 0; v < V; v++) db[v] = -Fh[v];
+    roe_dissipation_vjp<D>(UL, UR, pL, pR, n, a.gamma, db, ULb, URb);
+    if (VISC) ldg_vjp_add<D, BM>(Fs, a, ULb, URb, G1b, G2b);
+    return;
+  }
   double mnL = UL[1] * n[0], mnR = UR[1] * n[0];
   ZFR_UNROLL
   for (int d = 1; d < D; d++) {
@@ -527,23 +680,7 @@ ZFR_HD void common_flux_vjp(const double *UL, const double *UR, const double *n,
   const double eigb = 0.5 * (1.0 - a.rus_k) * dissb;
   if (lamL >= lamR) wavespeed_vjp<D>(UL, pL, n, a.gamma, eigb, ULb);
   else wavespeed_vjp<D>(UR, pR, n, a.gamma, eigb, URb);
-  if (VISC) {
-    const double beta = a.ldg_b, tau = a.ldg_tau;
-    ZFR_UNROLL
-    for (int v = 0; v < V; v++) {
-      ULb[v] += tau * Fs[v];
-      URb[v] -= tau * Fs[v];
-      const double GLb = (0.5 + beta) * Fs[v], GRb = (0.5 - beta) * Fs[v];
-      if (BM == 0) {
-        G1b[v] += GLb;
-        G2b[v] += GRb;
-      } else if (BM == 1) {
-        G1b[v] += GLb;
-      } else {
-        G1b[v] += GRb;
-      }
-    }
-  }
+  if (VISC) ldg_vjp_add<D, BM>(Fs, a, ULb, URb, G1b, G2b);
 }
 
 }  // namespace zphys

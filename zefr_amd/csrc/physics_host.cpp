@@ -73,18 +73,18 @@ void common(const zfr_phys_t &a, int viscous, const double *UL, const double *UR
   else common_flux<D, true, 2, FC>(UL, UR, GR, nullptr, n, dAs, a, F);
 }
 
-template <int D, int BM>
+template <int D, int BM, int FC>
 void common_vjp(const zfr_phys_t &a, int viscous, const double *UL, const double *UR, const double *dUL,
                 const double *dUR, const double *n, double dAs, const double *Fb, double *ULb, double *URb,
                 double *dULb, double *dURb) {
   constexpr int V = D + 2;
   for (int v = 0; v < V; v++) ULb[v] = URb[v] = 0.0;
   if (!viscous) {
-    common_flux_vjp<D, false, 0>(UL, UR, n, dAs, a, Fb, ULb, URb, nullptr, nullptr);
+    common_flux_vjp<D, false, 0, FC>(UL, UR, n, dAs, a, Fb, ULb, URb, nullptr, nullptr);
     return;
   }
   double G1b[V] = {}, G2b[V] = {};
-  common_flux_vjp<D, true, BM>(UL, UR, n, dAs, a, Fb, ULb, URb, G1b, G2b);
+  common_flux_vjp<D, true, BM, FC>(UL, UR, n, dAs, a, Fb, ULb, URb, G1b, G2b);
   // G = -T . n for the side whose gradient enters: cotangent on T[v][j] is -Gb[v] n_j
   auto side = [&](const double *U, const double *dU, const double *Gb, double *Ub, double *dUb) {
     const Prim<D> p = primitives<D>(U, a.gamma);
@@ -147,19 +147,18 @@ int zfr_host_common_flux_vjp(int nDims, const zfr_phys_t *a, int viscous, const 
                              const double *dUL, const double *dUR, const double *n, double dAs, const double *Fb,
                              double *ULb, double *URb, double *dULb, double *dURb) {
   return zfr::guard([&] {
-    if (a->fconv_type != 0) zfr::raise("the common-flux VJP supports the Rusanov flux only");
+    if (nDims != 2 && nDims != 3) zfr::raise("nDims must be 2 or 3");
+    if (a->fconv_type != 0 && a->fconv_type != 2) zfr::raise("fconv_type must be 0 (Rusanov) or 2 (Roe)");
     const int bm = bm_of(a->ldg_b);
-    if (nDims == 2) {
-      if (bm == 0) common_vjp<2, 0>(*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb, dURb);
-      else if (bm == 1) common_vjp<2, 1>(*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb, dURb);
-      else common_vjp<2, 2>(*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb, dURb);
-    } else if (nDims == 3) {
-      if (bm == 0) common_vjp<3, 0>(*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb, dURb);
-      else if (bm == 1) common_vjp<3, 1>(*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb, dURb);
-      else common_vjp<3, 2>(*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb, dURb);
-    } else {
-      zfr::raise("nDims must be 2 or 3");
-    }
+    using Fn = void (*)(const zfr_phys_t &, int, const double *, const double *, const double *, const double *,
+                        const double *, double, const double *, double *, double *, double *, double *);
+    static const Fn table[2][2][3] = {
+        {{common_vjp<2, 0, 0>, common_vjp<2, 1, 0>, common_vjp<2, 2, 0>},
+         {common_vjp<2, 0, 1>, common_vjp<2, 1, 1>, common_vjp<2, 2, 1>}},
+        {{common_vjp<3, 0, 0>, common_vjp<3, 1, 0>, common_vjp<3, 2, 0>},
+         {common_vjp<3, 0, 1>, common_vjp<3, 1, 1>, common_vjp<3, 2, 1>}}};
+    table[nDims - 2][a->fconv_type == 2 ? 1 : 0][bm](*a, viscous, UL, UR, dUL, dUR, n, dAs, Fb, ULb, URb, dULb,
+                                                     dURb);
   });
 }
 
