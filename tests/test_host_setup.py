@@ -169,6 +169,21 @@ def test_deck_rejects_unsupported_options(extra, msg):
         _read(extra, "bad.deck")
 
 
+def test_triangle_mesh_is_rejected_with_the_reason():
+    """Two triangles splitting the unit square: the setup refuses them and names the
+    reference's missing triangle volume divergence (src/elements.cpp:591-730)."""
+    from zefr_amd import zefr
+    mesh = _deck_lines(tmp("tri.msh"), [
+        "$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", "2", '1 1 "PERIODIC"', '2 2 "FLUID"',
+        "$EndPhysicalNames", "$Nodes", "4", "1 0 0 0", "2 1 0 0", "3 1 1 0", "4 0 1 0", "$EndNodes",
+        "$Elements", "2", "1 2 2 2 2 1 2 3", "2 2 2 2 2 1 3 4", "$EndElements"])
+    path = _deck_lines(tmp("tri.deck"), [l.format(mesh=mesh) for l in BASE])
+    z = zefr.Zefr()
+    z.read_input(path)
+    with pytest.raises(RuntimeError, match="triangular elements.*volume divergence"):
+        z.setup_host()
+
+
 def test_deck_rejects_missing_required_key():
     from zefr_amd import zefr
     path = _deck_lines(tmp("missing.deck"), ["nDims 2", "order 2"])

@@ -178,6 +178,13 @@ Mesh read_gmsh(const std::string &file, const zefr_input_t &in) {
       if (it == phys.end()) raise("boundary face with an unknown physical tag");
       if (it->second != 1) raise("boundary face tagged as fluid");
       m.per_faces.push_back(make_key(nd, nBndNodes));
+    } else if (type == 2 && m.nDims == 2) {
+      // The reference's triangle operators (src/elements.cpp:555-744) leave oppDiv without
+      // its volume derivative for every non-tensor element (it is filled only for quads and
+      // hexes, :591-621, and becomes -M3 M2 at :728-730): its triangle residual is the
+      // interface correction alone, so there is no triangle result to reproduce.
+      raise("triangular elements (Gmsh type 2) are not supported: the reference solver's "
+            "triangle residual has no volume divergence term (src/elements.cpp:591-730)");
     } else {
       raise("Gmsh element type " + std::to_string(type) +
             " is not supported (linear quads / hexes and their boundary faces)");
