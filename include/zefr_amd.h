@@ -210,18 +210,19 @@ int zefr_write_solution(zefr_t *z);
 int zefr_restart_solution(zefr_t *z);
 
 /* Batched residuals of independent solvers (new; no reference counterpart): the
- * compute_residual(0) launches of n set-up solvers captured once as one HIP graph, one
- * branch per solver (fork and join through events on the batch's own stream), so that one
- * graph launch evaluates all of them and the runtime overlaps the branches.  Meant for many
- * small cases whose residuals are too small to fill the GPU or to outlast the launch cost.
- * zefr_batch_residual launches the graph on the batch's stream and returns;
- * zefr_batch_synchronize waits for it.  The member solvers must have no work pending when
- * the batch is launched, and the batch must be recreated after a solver's tile changes
- * (the graph holds the launch shapes).  Results land in each solver's divF(0). */
+ * compute_residual(0) of n set-up solvers as one multi-case plan (zfr_multi_create: one
+ * launch per kernel, shape and flux branch over all members), captured once as a HIP graph
+ * on the batch's own stream, so that one graph launch evaluates all of them.  Meant for
+ * many small cases whose residuals are too small to fill the GPU or to outlast the launch
+ * cost.  zefr_batch_residual launches the graph on the batch's stream and returns;
+ * zefr_batch_synchronize waits for it; zefr_batch_launches returns the kernel launches in
+ * the graph.  The member solvers must have no work pending when the batch is launched.
+ * Results land in each solver's divF(0), bit-identical to its own compute_residual(0). */
 typedef struct zefr_batch_s zefr_batch_t;
 zefr_batch_t *zefr_batch_create(zefr_t **solvers, int n);
 int zefr_batch_residual(zefr_batch_t *b);
 int zefr_batch_synchronize(zefr_batch_t *b);
+int zefr_batch_launches(zefr_batch_t *b);
 void zefr_batch_destroy(zefr_batch_t *b);
 
 /* ---------------------------------------------------------------- singleton ------ */
@@ -330,6 +331,25 @@ int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, co
  * Wtr and Xtr are [E][V][nF] scratch buffers. */
 int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, double *v, double *Wtr,
                      double *Xtr, void *stream);
+/* Multi-case residual plan (new; no reference counterpart): the residuals of n independent
+ * element blocks (descs[i] with its own arrays, state U[i] and output divF[i], each
+ * [E_i][V][nS_i]) in one launch per kernel, shape and flux branch instead of one launch
+ * per case and kernel.  Cases are grouped by (kernel, nDims, order, LDG branch, convective
+ * flux); each group's workgroups find their case by a binary search over the cases' first
+ * workgroups.  Every case takes the kernels its own descriptor selects (Euler: extrapolate
+ * -> divergence; LDG beta = +-1/2: partner trace -> gradient_flux -> face_correction;
+ * other beta: extrapolate -> gradient -> divergence), with the same arithmetic as the
+ * single-case entry points, so the results are bit-identical to them.  The descriptors'
+ * tiles are ignored: each group takes one tile for its total element count.
+ * zfr_multi_create copies the case tables to the device (hipMalloc, blocking copies);
+ * zfr_multi_residual only launches, so it may be captured into a HIP graph;
+ * zfr_multi_launches returns the number of launches of one residual. */
+typedef struct zfr_multi_s zfr_multi_t;
+int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double *const *divF, int n,
+                     zfr_multi_t **plan);
+int zfr_multi_residual(zfr_multi_t *plan, void *stream);
+int zfr_multi_launches(const zfr_multi_t *plan);
+void zfr_multi_destroy(zfr_multi_t *plan);
 /* Plain device copy dst[i] = src[i], i < n (bandwidth calibration for the benchmark; no
  * counterpart in the reference). */
 int zfr_stream_copy(double *dst, const double *src, long n, void *stream);

@@ -1,6 +1,8 @@
-"""Batched residuals (zefr_batch_*): the compute_residual launches of independent solvers
-captured as one HIP graph.  Each replay must reproduce every member's eager residual bit for
-bit; the graph is replayed several times with eager work and synchronisation in between."""
+"""Batched residuals (zefr_batch_*): the residuals of independent solvers as one multi-case
+plan (zfr_multi_*: one launch per kernel, shape and flux branch over all members) captured as
+one HIP graph.  The members cover Euler, the one-sided LDG path, general beta and Roe on quads
+and hexes of several orders.  Each replay must reproduce every member's eager residual bit
+for bit; the graph is replayed several times with eager work and synchronisation in between."""
 
 from __future__ import annotations
 
@@ -37,6 +39,7 @@ def test_batch_graph_matches_eager_residuals():
         refs.append(z.state("divF", 0, layout="device").copy())
         zs.append(z)
     b = Batch(zs)
+    assert 0 < b.launches < 2 * len(zs)  # grouped: fewer launches than one per member and kernel
     for rep in range(3):
         for z in zs:  # clear every residual eagerly between replays
             d = z.dims()

@@ -122,6 +122,7 @@ SIGNATURES = {
     "zefr_batch_create": (_P, [C.POINTER(_P), _I]),
     "zefr_batch_residual": (_I, [_P]),
     "zefr_batch_synchronize": (_I, [_P]),
+    "zefr_batch_launches": (_I, [_P]),
     "zefr_batch_destroy": (None, [_P]),
     "zefr_write_solution": (_I, [_P]),
     "zefr_restart_solution": (_I, [_P]),
@@ -148,6 +149,10 @@ SIGNATURES = {
     "zfr_face_correction_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_divergence_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P]),
     "zfr_stream_copy": (_I, [_P, _P, C.c_long, _P]),
+    "zfr_multi_create": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _I, C.POINTER(_P)]),
+    "zfr_multi_residual": (_I, [_P, _P]),
+    "zfr_multi_launches": (_I, [_P]),
+    "zfr_multi_destroy": (None, [_P]),
     "zfr_rk_stage_source": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, C.c_double, C.c_double, _P]),
     "zfr_rk_final_source": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_pmg_transfer": (_I, [_I, _I, C.c_long, _I, _I, _P, C.c_double, _P, C.c_double, _P, _P]),

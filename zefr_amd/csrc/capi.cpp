@@ -419,20 +419,20 @@ int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph) {
 }
 
 // ---------------------------------------------------------------- batches
-// Residuals of independent solvers as one HIP graph: captured once on the batch's own
-// stream, forking to every solver's stream (each solver's launches form one branch) and
-// joining back, so one graph launch runs all of them and the runtime overlaps branches.
+// Residuals of independent solvers: a multi-case plan over their element blocks (one
+// launch per kernel, shape and flux branch; zfr_multi_create), captured once as a HIP
+// graph on the batch's own stream, so one graph launch evaluates every member.
 struct zefr_batch_s {
   std::vector<zfr::FRSolver *> solvers;
   hipStream_t stream = nullptr;
-  std::vector<hipEvent_t> events;
+  zfr_multi_t *plan = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
   ~zefr_batch_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (auto e : events) (void)hipEventDestroy(e);
+    if (plan) zfr_multi_destroy(plan);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -442,39 +442,39 @@ zefr_batch_t *zefr_batch_create(zefr_t **zs, int n) {
   const int rc = zfr::guard([&] {
     if (!zs || n < 1) zfr::raise("batch_create: need at least one solver");
     auto batch = std::make_unique<zefr_batch_s>();
+    std::vector<zfr_ele_desc_t> descs;
+    std::vector<const double *> U;
+    std::vector<double *> divF;
     for (int i = 0; i < n; i++) {
       if (!zs[i]) zfr::raise("batch_create: null solver");
       zfr::FRSolver &s = zs[i]->s();
       if (!s.device_ready) zfr::raise("batch_create: every solver must be set up on the device");
       s.synchronize();
       batch->solvers.push_back(&s);
+      descs.push_back(s.eles->desc(*s.input));
+      U.push_back(s.eles->U_spts_d.as<double>());
+      divF.push_back(s.eles->divF_d.as<double>());
     }
+    zfr::status_or_throw(zfr_multi_create(descs.data(), U.data(), divF.data(), n, &batch->plan));
     zfr::hip_or_throw(hipStreamCreateWithFlags(&batch->stream, hipStreamNonBlocking), "hipStreamCreate");
-    batch->events.resize(n + 1, nullptr);
-    for (auto &e : batch->events) zfr::hip_or_throw(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     hipStream_t s0 = batch->stream;
     zfr::hip_or_throw(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    try {
-      zfr::hip_or_throw(hipEventRecord(batch->events[n], s0), "hipEventRecord");
-      for (int i = 0; i < n; i++) {
-        zfr::FRSolver &s = *batch->solvers[i];
-        zfr::hip_or_throw(hipStreamWaitEvent(s.stream, batch->events[n], 0), "hipStreamWaitEvent");
-        s.compute_residual(0);
-        zfr::hip_or_throw(hipEventRecord(batch->events[i], s.stream), "hipEventRecord");
-        zfr::hip_or_throw(hipStreamWaitEvent(s0, batch->events[i], 0), "hipStreamWaitEvent");
-      }
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s0, &g);
+    const int lrc = zfr_multi_residual(batch->plan, s0);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s0, &g);
+    if (lrc) {
       if (g) (void)hipGraphDestroy(g);
-      throw;
+      zfr::status_or_throw(lrc);
     }
-    zfr::hip_or_throw(hipStreamEndCapture(s0, &batch->graph), "hipStreamEndCapture");
+    zfr::hip_or_throw(ec, "hipStreamEndCapture");
+    batch->graph = g;
     zfr::hip_or_throw(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     b = batch.release();
   });
   return rc ? nullptr : b;
 }
+
+int zefr_batch_launches(zefr_batch_t *b) { return b ? zfr_multi_launches(b->plan) : -1; }
 
 int zefr_batch_residual(zefr_batch_t *b) {
   return zfr::guard([&] {

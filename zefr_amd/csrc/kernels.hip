@@ -30,7 +30,10 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "zefr_amd.h"
 #include "common.hpp"
@@ -83,6 +86,27 @@ struct KArgs {
     int nStages, stage, mode;
   } rk;
 };
+
+// One case of a multi-case launch: its kernel arguments and arrays.  A multi-case launch
+// runs the same kernel over several independent solvers of one shape in one grid:
+// workgroup b does block b - starts[c] of case c, where starts[c] <= b < starts[c + 1].
+struct MultiCase {
+  KArgs a;
+  const double *U;
+  double *out;
+};
+
+// The case of workgroup b: binary search over the cases' first workgroups (uniform across
+// the workgroup, so the loads are scalar).
+__device__ __forceinline__ int find_case(const long *__restrict__ starts, int nc, long b) {
+  int lo = 0, hi = nc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 // Flux point f of the reference ordering (src/quads.cpp:86-123, src/hexas.cpp:133-194):
 // normal direction, side (0: the -1 end, 1: the +1 end), first spt of the line through
@@ -214,13 +238,13 @@ __device__ __forceinline__ int owned_fpt(unsigned mask, int k) {
 // trace_sel = 1 only the points of faces the partner owns are written: on the one-sided
 // LDG path the owner reads the other side's trace and nothing reads its own.
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__restrict__ U, KArgs a) {
+__device__ __forceinline__ void extrapolate_block(const double *__restrict__ U, const KArgs &a, long bid) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   extern __shared__ double lds[];
   double *lops = lds;
   double *lU = lds + S::NOPS_PAD;
-  const long e0 = (long)blockIdx.x * a.EB;
+  const long e0 = bid * a.EB;
   const long neb = min((long)a.EB, (long)a.E - e0);
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
@@ -248,6 +272,17 @@ __global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__res
       for (int v = 0; v < V; v++) a.Utr[(e * V + v) * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
     }
   }
+}
+
+template <int D, int N>
+__global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__restrict__ U, KArgs a) {
+  extrapolate_block<D, N>(U, a, blockIdx.x);
+}
+
+template <int D, int N>
+__global__ __launch_bounds__(kMaxThreads) void k_extrapolate_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  extrapolate_block<D, N>(cs[c].U, cs[c].a, (long)blockIdx.x - starts[c]);
 }
 
 // LDG common solution at this thread's flux points (src/faces.cpp:1366-1474):
@@ -335,15 +370,15 @@ __device__ __forceinline__ void load_metric(const KArgs &a, long e, int t, int n
 // the flux points whose gradient the partner element needs.  LDS per element holds U and
 // Ucomm, then (aliased) the physical gradient.
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restrict__ U, double *__restrict__ dU_out,
-                                                          KArgs a) {
+__device__ __forceinline__ void gradient_block(const double *__restrict__ U, double *__restrict__ dU_out,
+                                               const KArgs &a, long bid) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   constexpr int W = (V * nS + V * nF > D * V * nS) ? V * nS + V * nF : D * V * nS;
   extern __shared__ double lds[];
   double *lops = lds;
   double *lW = lds + S::NOPS_PAD;
-  const long e0 = (long)blockIdx.x * a.EB;
+  const long e0 = bid * a.EB;
   const long neb = min((long)a.EB, (long)a.E - e0);
   stage_U<D, N>(U, a, lops, lW, e0, neb, W);
   __syncthreads();
@@ -395,13 +430,25 @@ __global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restri
   }
 }
 
+template <int D, int N>
+__global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restrict__ U, double *__restrict__ dU_out,
+                                                          KArgs a) {
+  gradient_block<D, N>(U, dU_out, a, blockIdx.x);
+}
+
+template <int D, int N>
+__global__ __launch_bounds__(kMaxThreads) void k_gradient_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  gradient_block<D, N>(cs[c].U, nullptr, cs[c].a, (long)blockIdx.x - starts[c]);
+}
+
 // ---------------------------------------------------------------- kernel 3: divergence
 // compute_F + compute_divF_spts + compute_common_F + compute_divF_fpts.
 // Phases (viscous): U -> LDS | Ucomm -> lC | gradient -> lG | common flux -> lC |
 // transformed flux -> lG | divergence.  Inviscid: U -> LDS | flux + common flux | divergence.
 template <int D, int N, bool VISC, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__restrict__ U, double *__restrict__ divF,
-                                                            KArgs a) {
+__device__ __forceinline__ void divergence_block(const double *__restrict__ U, double *__restrict__ divF,
+                                                 const KArgs &a, long bid) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   extern __shared__ double lds[];
@@ -410,7 +457,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
   double *lU = lops + S::NOPS_PAD;
   double *lC = lU + EB * V * nS;
   double *lG = lC + EB * V * nF;  // gradient (viscous), then the transformed flux
-  const long e0 = (long)blockIdx.x * EB;
+  const long e0 = bid * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
   __syncthreads();
@@ -562,6 +609,18 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__rest
   rk_epilogue<D>(a, e, t, nS, acc);
 }
 
+template <int D, int N, bool VISC, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__restrict__ U, double *__restrict__ divF,
+                                                            KArgs a) {
+  divergence_block<D, N, VISC, BM, FC>(U, divF, a, blockIdx.x);
+}
+
+template <int D, int N, bool VISC, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) void k_divergence_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  divergence_block<D, N, VISC, BM, FC>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
+}
+
 // ---------------------------------------------------------------- one-sided LDG path
 // With LDG beta = +1/2 (BM 1) the common viscous flux uses only the left (slot-0)
 // side's gradient, with beta = -1/2 (BM 2) only the right side's (src/faces.cpp:1520-1545),
@@ -583,8 +642,8 @@ struct OneSidedWaves {
 };
 
 template <int D, int N, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
-void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
+__device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U, double *__restrict__ divF,
+                                                    const KArgs &a, long bid) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   constexpr int owner = (BM == 1) ? 0 : 1;  // slot whose gradient enters the common flux
@@ -597,7 +656,7 @@ void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KA
   double *lU = lops + S::NOPS_PAD;
   double *lW = lU + EB * V * nS;
   unsigned *lmask = reinterpret_cast<unsigned *>(lW + EB * W);  // owned faces per element
-  const long e0 = (long)blockIdx.x * EB;
+  const long e0 = bid * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
@@ -778,10 +837,23 @@ void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KA
   for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
 }
 
+template <int D, int N, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
+void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
+  visc_onesided_block<D, N, BM, FC>(U, divF, a, blockIdx.x);
+}
+
+template <int D, int N, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
+void k_visc_onesided_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  visc_onesided_block<D, N, BM, FC>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
+}
+
 // divF += oppDiv_fpts Fcomm (src/elements.cpp:1476-1501) with Fcomm = Ftr(owner) * signed
 // dA of this side; the owner's Ftr is this element's own or its partner's.
 template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restrict__ divF, KArgs a) {
+__device__ __forceinline__ void face_correction_block(double *__restrict__ divF, const KArgs &a, long bid) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   constexpr int owner = (BM == 1) ? 0 : 1;
@@ -791,7 +863,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restr
   double *lF = lops + S::NOPS_PAD;
   const int tid = threadIdx.x, NT = blockDim.x;
   for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const long e0 = (long)blockIdx.x * EB;
+  const long e0 = bid * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   const int el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
@@ -830,6 +902,17 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restr
 #pragma unroll
   for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
   rk_epilogue<D>(a, e, t, nS, acc);
+}
+
+template <int D, int N, int BM>
+__global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restrict__ divF, KArgs a) {
+  face_correction_block<D, N, BM>(divF, a, blockIdx.x);
+}
+
+template <int D, int N, int BM>
+__global__ __launch_bounds__(kMaxThreads) void k_face_correction_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  face_correction_block<D, N, BM>(cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
 }
 
 // ---------------------------------------------------------------- reverse mode
@@ -1078,14 +1161,12 @@ void launch_adj_main(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds,
     hipLaunchKernelGGL((k_adj_main<D, N, true, 2, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
 }
 
+// LDS doubles per element of each kernel (the owned-point list of the one-sided kernel is
+// nF + 1 ints).
 template <int D, int N>
-int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j = AdjPtrs()) {
+size_t lds_per_element(Op op, bool viscous) {
   using S = Shape<D, N>;
-  if (d->nSpts != S::nS || d->nFpts != S::nF || d->nVars != S::V)
-    return zfr::fail("kernel launch: element descriptor does not match (nDims, order)");
-  if (d->nEles <= 0) return 0;
   constexpr size_t kUC = (size_t)S::V * S::nS + S::V * S::nF, kG = (size_t)D * S::V * S::nS;
-  // LDS doubles per element (the owned-point list of the one-sided kernel is nF + 1 ints)
   size_t per_ele = 0;
   switch (op) {
     case Op::Extrap:
@@ -1098,10 +1179,20 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
     case Op::FaceCorr: per_ele = (size_t)S::V * S::nF; break;
     case Op::AdjPrep: per_ele = (size_t)2 * S::V * S::nS; break;
     case Op::AdjMain:
-      per_ele = d->viscous ? (size_t)AdjLds<D, N, true>::per_ele : (size_t)AdjLds<D, N, false>::per_ele;
+      per_ele = viscous ? (size_t)AdjLds<D, N, true>::per_ele : (size_t)AdjLds<D, N, false>::per_ele;
       break;
     case Op::AdjGather: per_ele = (size_t)S::V * S::nF; break;
   }
+  return per_ele;
+}
+
+template <int D, int N>
+int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j = AdjPtrs()) {
+  using S = Shape<D, N>;
+  if (d->nSpts != S::nS || d->nFpts != S::nF || d->nVars != S::V)
+    return zfr::fail("kernel launch: element descriptor does not match (nDims, order)");
+  if (d->nEles <= 0) return 0;
+  const size_t per_ele = lds_per_element<D, N>(op, d->viscous != 0);
   const bool adj = op == Op::AdjPrep || op == Op::AdjMain || op == Op::AdjGather;
   if (adj && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5)
     return zfr::fail("the residual VJP supports LDG beta = +-0.5 (and Euler)");
@@ -1185,6 +1276,81 @@ int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipSt
     }
   }
   return zfr::fail("kernel launch: supported shapes are quads and hexes with order 0..5");
+}
+
+// One launch of a multi-case plan: every case of one kernel, shape and flux branch.
+struct MultiGroup {
+  Op op = Op::Extrap;
+  int D = 0, N = 0, visc = 0, bm = 0, fc = 0;  // bm: LDG branch (0 general, 1 beta = 1/2, 2 beta = -1/2)
+  int EB = 1, nc = 0;
+  unsigned nblk = 0;
+  size_t lds = 0;
+  MultiCase *cases = nullptr;  // device [nc]
+  long *starts = nullptr;      // device [nc]
+};
+
+template <int D, int N>
+void launch_multi(const MultiGroup &g, hipStream_t st) {
+  const dim3 grid(g.nblk), block((unsigned)(g.EB * Shape<D, N>::nS));
+  const MultiCase *c = g.cases;
+  const long *s0 = g.starts;
+  switch (g.op) {
+    case Op::Extrap:
+    case Op::ExtrapPartner:
+      hipLaunchKernelGGL((k_extrapolate_multi<D, N>), grid, block, g.lds, st, c, s0, g.nc);
+      break;
+    case Op::Grad:
+      hipLaunchKernelGGL((k_gradient_multi<D, N>), grid, block, g.lds, st, c, s0, g.nc);
+      break;
+    case Op::Div:
+#define ZK_DIV(V_, B_, F_) hipLaunchKernelGGL((k_divergence_multi<D, N, V_, B_, F_>), grid, block, g.lds, st, c, s0, g.nc)
+      if (!g.visc) { if (g.fc) ZK_DIV(false, 0, 1); else ZK_DIV(false, 0, 0); }
+      else if (g.bm == 1) { if (g.fc) ZK_DIV(true, 1, 1); else ZK_DIV(true, 1, 0); }
+      else if (g.bm == 2) { if (g.fc) ZK_DIV(true, 2, 1); else ZK_DIV(true, 2, 0); }
+      else { if (g.fc) ZK_DIV(true, 0, 1); else ZK_DIV(true, 0, 0); }
+#undef ZK_DIV
+      break;
+    case Op::OneSided:
+      if (g.bm == 1) {
+        if (g.fc) hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 1, 1>), grid, block, g.lds, st, c, s0, g.nc);
+        else hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 1, 0>), grid, block, g.lds, st, c, s0, g.nc);
+      } else {
+        if (g.fc) hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 2, 1>), grid, block, g.lds, st, c, s0, g.nc);
+        else hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 2, 0>), grid, block, g.lds, st, c, s0, g.nc);
+      }
+      break;
+    case Op::FaceCorr:
+      if (g.bm == 1) hipLaunchKernelGGL((k_face_correction_multi<D, N, 1>), grid, block, g.lds, st, c, s0, g.nc);
+      else hipLaunchKernelGGL((k_face_correction_multi<D, N, 2>), grid, block, g.lds, st, c, s0, g.nc);
+      break;
+    default:
+      break;
+  }
+}
+
+template <int D>
+void dispatch_multi_n(const MultiGroup &g, hipStream_t st) {
+  switch (g.N) {
+    case 1: launch_multi<D, 1>(g, st); break;
+    case 2: launch_multi<D, 2>(g, st); break;
+    case 3: launch_multi<D, 3>(g, st); break;
+    case 4: launch_multi<D, 4>(g, st); break;
+    case 5: launch_multi<D, 5>(g, st); break;
+    case 6: launch_multi<D, 6>(g, st); break;
+  }
+}
+
+template <int D>
+size_t lds_per_element_n(int N, Op op, bool viscous) {
+  switch (N) {
+    case 1: return lds_per_element<D, 1>(op, viscous);
+    case 2: return lds_per_element<D, 2>(op, viscous);
+    case 3: return lds_per_element<D, 3>(op, viscous);
+    case 4: return lds_per_element<D, 4>(op, viscous);
+    case 5: return lds_per_element<D, 5>(op, viscous);
+    case 6: return lds_per_element<D, 6>(op, viscous);
+  }
+  return 0;
 }
 
 inline unsigned grid_for(long total) {
@@ -1368,6 +1534,136 @@ int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, 
   if (!rc) rc = zk::dispatch(zk::Op::AdjGather, d, U, v, st, j);
   return rc;
 }
+
+// Multi-case residual plan: cases grouped by (kernel, shape, flux branch), one launch per
+// group per phase; the case tables live on the device for the plan's lifetime.
+struct zfr_multi_s {
+  std::vector<zk::MultiGroup> groups;  // launch order: every phase-0 group, then phase 1, phase 2
+  std::vector<void *> allocs;
+  ~zfr_multi_s() {
+    for (void *q : allocs) (void)hipFree(q);
+  }
+};
+
+int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double *const *divF, int n,
+                     zfr_multi_t **plan) {
+  using zk::Op;
+  if (!plan) return zfr::fail("zfr_multi_create: null plan pointer");
+  *plan = nullptr;
+  if (!descs || !U || !divF || n < 1) return zfr::fail("zfr_multi_create: need at least one case");
+  // the kernels of each case's residual, by phase (the same choice as the per-solver path)
+  struct Key {
+    Op op; int D, N, visc, bm, fc;
+    bool operator<(const Key &o) const {
+      return std::tie(op, D, N, visc, bm, fc) < std::tie(o.op, o.D, o.N, o.visc, o.bm, o.fc);
+    }
+  };
+  std::map<Key, std::vector<int>> phase[3];
+  for (int i = 0; i < n; i++) {
+    const zfr_ele_desc_t *d = &descs[i];
+    const int D = d->nDims, N = d->order + 1;
+    if ((D != 2 && D != 3) || N < 1 || N > 6)
+      return zfr::fail("zfr_multi_create: supported shapes are quads and hexes with order 0..5");
+    long nS = 1, nFF = 1;
+    for (int k = 0; k < D; k++) nS *= N;
+    for (int k = 0; k < D - 1; k++) nFF *= N;
+    if (d->nSpts != nS || d->nFpts != 2 * D * nFF || d->nVars != D + 2)
+      return zfr::fail("zfr_multi_create: element descriptor " + std::to_string(i) + " does not match (nDims, order)");
+    if (d->fconv_type != 0 && d->fconv_type != 2) return zfr::fail("fconv_type must be 0 (Rusanov) or 2 (Roe)");
+    if (!U[i] || !divF[i] || !d->Utr) return zfr::fail("zfr_multi_create: missing buffer of case " + std::to_string(i));
+    const int fc = d->fconv_type == 2, visc = d->viscous != 0;
+    const int bm = !visc ? 0 : (d->ldg_b == 0.5 ? 1 : (d->ldg_b == -0.5 ? 2 : 0));
+    if (d->nEles <= 0) continue;
+    if (visc && bm != 0) {
+      if (!d->Ftr) return zfr::fail("zfr_multi_create: the one-sided path needs Ftr");
+      phase[0][{Op::ExtrapPartner, D, N, 0, bm, 0}].push_back(i);
+      phase[1][{Op::OneSided, D, N, 1, bm, fc}].push_back(i);
+      phase[2][{Op::FaceCorr, D, N, 1, bm, 0}].push_back(i);
+    } else {
+      if (visc && !d->Gtr) return zfr::fail("zfr_multi_create: the general LDG path needs Gtr");
+      phase[0][{Op::Extrap, D, N, 0, 0, 0}].push_back(i);
+      if (visc) phase[1][{Op::Grad, D, N, 1, 0, 0}].push_back(i);
+      phase[2][{Op::Div, D, N, visc, bm, fc}].push_back(i);
+    }
+  }
+  auto m = new zfr_multi_s();
+  for (int ph = 0; ph < 3; ph++) {
+    for (auto &kv : phase[ph]) {
+      const Key &k = kv.first;
+      const std::vector<int> &ids = kv.second;
+      zk::MultiGroup g;
+      g.op = k.op; g.D = k.D; g.N = k.N; g.visc = k.visc; g.bm = k.bm; g.fc = k.fc;
+      g.nc = (int)ids.size();
+      const size_t per_ele = (k.D == 2) ? zk::lds_per_element_n<2>(k.N, k.op, k.visc != 0)
+                                        : zk::lds_per_element_n<3>(k.N, k.op, k.visc != 0);
+      const int N = k.N;
+      const int nS = (k.D == 2) ? N * N : N * N * N;
+      const int nops = 2 * N + N * N + 2 * N, nops_pad = (nops + 7) & ~7;
+      // one tile for the group, chosen for its total element count as for one solver
+      zfr_ele_desc_t agg = descs[ids[0]];
+      agg.nEles = 0;
+      for (int i : ids) agg.nEles += descs[i].nEles;
+      agg.eles_per_block = 0;
+      const int defEB = (zk::kMaxThreads / nS) > 0 ? zk::kMaxThreads / nS : 1;
+      g.EB = zk::eles_per_block(&agg, defEB, nops_pad, per_ele);
+      g.lds = sizeof(double) * (nops_pad + g.EB * per_ele);
+      if (g.lds > zk::kMaxLds) {
+        delete m;
+        return zfr::fail("zfr_multi_create: one element needs more LDS than a workgroup may use");
+      }
+      std::vector<zk::MultiCase> hc(ids.size());
+      std::vector<long> hs(ids.size());
+      long blocks = 0;
+      for (size_t q = 0; q < ids.size(); q++) {
+        const int i = ids[q];
+        hc[q].a = zk::make_args(&descs[i], g.EB);
+        hc[q].a.trace_sel = (k.op == Op::ExtrapPartner) ? 1 : 0;
+        hc[q].U = U[i];
+        hc[q].out = (k.op == Op::Div || k.op == Op::OneSided || k.op == Op::FaceCorr) ? divF[i] : nullptr;
+        hs[q] = blocks;
+        blocks += (descs[i].nEles + g.EB - 1) / g.EB;
+      }
+      if (blocks > 0x7fffffffL) {
+        delete m;
+        return zfr::fail("zfr_multi_create: too many workgroups in one group");
+      }
+      g.nblk = (unsigned)blocks;
+      void *dc = nullptr, *ds = nullptr;
+      hipError_t e = hipMalloc(&dc, hc.size() * sizeof(zk::MultiCase));
+      if (e == hipSuccess) m->allocs.push_back(dc);
+      if (e == hipSuccess) e = hipMalloc(&ds, hs.size() * sizeof(long));
+      if (e == hipSuccess) m->allocs.push_back(ds);
+      if (e == hipSuccess) e = hipMemcpy(dc, hc.data(), hc.size() * sizeof(zk::MultiCase), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), hs.size() * sizeof(long), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        delete m;
+        return zfr::check_hip(e, "zfr_multi_create");
+      }
+      g.cases = static_cast<zk::MultiCase *>(dc);
+      g.starts = static_cast<long *>(ds);
+      m->groups.push_back(g);
+    }
+  }
+  *plan = m;
+  return 0;
+}
+
+int zfr_multi_residual(zfr_multi_t *plan, void *stream) {
+  if (!plan) return zfr::fail("zfr_multi_residual: null plan");
+  const hipStream_t st = (hipStream_t)stream;
+  for (const zk::MultiGroup &g : plan->groups) {
+    if (g.nblk == 0) continue;
+    if (g.D == 2) zk::dispatch_multi_n<2>(g, st);
+    else zk::dispatch_multi_n<3>(g, st);
+    const int rc = zfr::check_hip(hipGetLastError(), "zfr_multi_residual");
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int zfr_multi_launches(const zfr_multi_t *plan) { return plan ? (int)plan->groups.size() : -1; }
+
+void zfr_multi_destroy(zfr_multi_t *plan) { delete plan; }
 
 int zfr_stream_copy(double *dst, const double *src, long n, void *stream) {
   if (n <= 0) return 0;

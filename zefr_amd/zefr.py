@@ -341,8 +341,9 @@ class Zefr:
 
 
 class Batch:
-    """Residuals of several set-up solvers as one HIP graph launch (zefr_batch_*): each
-    solver's compute_residual(0) is one branch of the graph.  Recreate after set_tile."""
+    """Residuals of several set-up solvers as one HIP graph launch (zefr_batch_*): the
+    members' compute_residual(0) as a multi-case plan, one kernel launch per kernel, shape
+    and flux branch over all members (``launches``)."""
 
     def __init__(self, solvers):
         self._solvers = list(solvers)  # kept alive as long as the graph refers to them
@@ -360,6 +361,10 @@ class Batch:
 
     def synchronize(self):
         check(lib().zefr_batch_synchronize(self._h))
+
+    @property
+    def launches(self):
+        return lib().zefr_batch_launches(self._h)
 
     def __del__(self):
         h = getattr(self, "_h", None)
