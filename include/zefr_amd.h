@@ -312,7 +312,13 @@ int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type,
  * src/solver.cpp:2897-2941 or the final combination of :2997-3062, same arithmetic as
  * zfr_rk_stage / zfr_rk_final): after divF of `stage` is formed, mode 1 writes
  * U = U_ini - alpha dt/detJ divF, mode 2 writes U = U_ini - sum_s beta_s dt/detJ divF_s
- * with the other stages read from divF0 ([nStages][E][V][nS], stage 0 first). */
+ * with the other stages read from divF0 ([nStages][E][V][nS], stage 0 first).
+ * trace_out (mode 1 only; NULL for none): the same kernel also writes the new U's trace
+ * there ([E][V][nF]), as the next stage's extrapolation would (zfr_extrapolate_partner for
+ * zfr_face_correction_rk, zfr_extrapolate for zfr_divergence_rk), so that launch can be
+ * skipped.  zfr_face_correction_rk reads no trace and may write d->Utr itself;
+ * zfr_divergence_rk reads the partners' d->Utr while it writes, so its trace_out must be
+ * another buffer (the next stage then runs with Utr = trace_out). */
 typedef struct zfr_rk_args {
   double *U;
   const double *U_ini;
@@ -321,6 +327,7 @@ typedef struct zfr_rk_args {
   double alpha, dt;
   double beta[4];
   const double *source;  /* p-multigrid source term added to each divF_s, or NULL */
+  double *trace_out;     /* mode 1: also write the new state's trace here, or NULL  */
 } zfr_rk_args_t;
 int zfr_face_correction_rk(const zfr_ele_desc_t *d, double *divF, const zfr_rk_args_t *rk, void *stream);
 int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *rk,

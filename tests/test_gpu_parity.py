@@ -70,6 +70,25 @@ def test_rk4_steps(case):
     assert err <= RK_TOL, f"{case['name']}: relative L-inf after {nsteps} steps {err:.3e}"
 
 
+@pytest.mark.parametrize("order,viscous,tiles", [(5, True, (3, 2, 4)), (4, True, (2, 2, 3)), (5, False, (4, 4, 3))],
+                         ids=["q_ns_p5", "q_ns_p4", "q_euler_p5"])
+def test_rk4_steps_multi_element_tiles(order, viscous, tiles):
+    """Fused RK stages write the next stage's trace from the last kernel's LDS.  Quads at
+    p >= 4 have more solution than flux points per element; tiles of several elements per
+    workgroup check that the per-element blocks stay apart (the default tile is one element
+    per workgroup on meshes this small)."""
+    case = dict(name=f"q_tiled_p{order}", n=(6, 5), lo=(0.0, 0.0), hi=(TWO_PI, TWO_PI), order=order,
+                viscous=viscous, state="tgv2d")
+    z, o, _ = paired(case)
+    z.set_tiles(tiles)
+    nsteps = 10
+    for _ in range(nsteps):
+        z.do_step()
+    o.step_rk(nsteps)
+    err = rel_err(z.state("U"), o.U())
+    assert err <= RK_TOL, f"{case['name']}: relative L-inf after {nsteps} steps {err:.3e}"
+
+
 def test_vortex_100_rk4_steps():
     """The CPU-runnable benchmark case: 2D Euler vortex, 32x32 quads on [-5,5]^2, p=3,
     Rusanov, RK4 dt=1e-3, 100 steps."""

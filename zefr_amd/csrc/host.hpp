@@ -105,6 +105,7 @@ class Elements {
   std::vector<double> metric;                        // [E][D*D+1][nS]: adjugate(d1,d2), detJ
   // device data (element-major)
   DevBuf U_spts_d, U_ini_d, divF_d, dU_spts_d, metric_d, fgeo_d, nbr_d, ops_d, Utr_d, Gtr_d, Ftr_d;
+  DevBuf Utr2_d;  // second trace buffer: fused RK stages on the divergence path alternate with Utr_d
   std::vector<double> U_host, dU_host;  // host mirrors for get_u_spts / get_du_spts
   int tile = 0;                         // elements per workgroup (0 = kernel default)
 
@@ -192,6 +193,9 @@ class FRSolver {
   // source term [E][V][nS] added to every stage residual (nullptr: plain update)
   void update(const double *source);
   bool own_stream = true;  // false for p-multigrid levels, which run on the fine stream
+  // The trace buffer the residual phases read, when not Utr_d (fused RK steps on the
+  // divergence path alternate two buffers; see step_RK_eager)
+  double *trace_in = nullptr;
   // PyFR-format solution files (src/solver.cpp:4275-4837): write returns the file name
   std::string write_solution_pyfr(const std::string &prefix);
   void restart_pyfr(const std::string &restart_case, unsigned restart_iter);
@@ -200,7 +204,7 @@ class FRSolver {
  private:
   void rk_stage_update(int stage);
   void rk_final_update();
-  void finish_with_rk(int stage);
+  void finish_with_rk(int stage, double *trace_out = nullptr);
   void step_RK_eager();
   void drop_graph();
   hipGraph_t graph = nullptr;

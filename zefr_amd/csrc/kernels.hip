@@ -84,6 +84,7 @@ struct KArgs {
     long total;
     double alpha, dt, beta[4];
     int nStages, stage, mode;
+    double *trace;  // mode 1: also write the new state's trace here (the next stage's Utr), or null
   } rk;
 };
 
@@ -195,7 +196,8 @@ __device__ __forceinline__ double line_extrap(const double *lE, const double *bl
 // stage's divF at the spt; the other stages' come from memory.  With a source term
 // (src/solver.cpp:2946-2962, 3061-3073) every divF_s is replaced by divF_s + src.
 template <int D>
-__device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc) {
+__device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc,
+                                            double *unew = nullptr) {
   constexpr int V = D + 2, nM = D * D + 1;
   if (a.rk.mode == 0) return;
   const double detJ = a.metric[(e * nM + D * D) * nS + t];
@@ -204,7 +206,9 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
     const long i = (e * V + v) * nS + t;
     if (a.rk.mode == 1) {
       const double r = a.rk.src ? acc[v] + a.rk.src[i] : acc[v];
-      a.rk.U[i] = a.rk.Uini[i] - a.rk.alpha * a.rk.dt / detJ * r;
+      const double u = a.rk.Uini[i] - a.rk.alpha * a.rk.dt / detJ * r;
+      a.rk.U[i] = u;
+      if (unew) unew[v] = u;
     } else {
       double u = a.rk.Uini[i];
       for (int s = 0; s < a.rk.nStages; s++) {
@@ -214,6 +218,38 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
       }
       a.rk.U[i] = u;
     }
+  }
+}
+
+// After a fused RK stage update: the trace of the new state at this element's flux points,
+// written to a.rk.trace ([E][V][nF], the next stage's Utr), so that the next stage's
+// extrapolation launch is skipped.  owner < 0 writes every point (k_extrapolate with
+// trace_sel 0); otherwise only the points whose partner owns the face (trace_sel 1 with
+// that owner slot).  The new state goes through the element's LDS block w (V * nS
+// doubles), which no thread may still read; every thread of the workgroup calls this (it
+// holds barriers).
+template <int D, int N>
+__device__ __forceinline__ void rk_trace(const KArgs &a, const double *lops, double *w, long e, int t, bool valid,
+                                         const double *unew, int owner) {
+  using S = Shape<D, N>;
+  constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < V; v++) w[v * nS + t] = unew[v];
+  }
+  __syncthreads();
+  if (!valid) return;
+#pragma unroll
+  for (int k = 0; k < S::FPT; k++) {
+    const int f = t + k * nS;
+    if (f >= nF) continue;
+    if (owner >= 0 && (int)((a.nbr[e * nF + f] >> 8) & 1u) == owner) continue;
+    int dim, side, base, str;
+    fpt_info<D, N>(f, dim, side, base, str);
+    const double *lE = tab_lE<N>(lops, side);
+#pragma unroll
+    for (int v = 0; v < V; v++) a.rk.trace[(e * V + v) * nF + f] = line_extrap<N>(lE, w + v * nS, base, str);
   }
 }
 
@@ -572,41 +608,45 @@ __device__ __forceinline__ void divergence_block(const double *__restrict__ U, d
       for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[d][v];
   }
   __syncthreads();
-  if (!valid) return;
-
-  // divergence: oppDiv tF + oppDiv_fpts Fcomm (src/elements.cpp:1449-1501)
-  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-  double acc[V];
+  double unew[V];
+  if (valid) {
+    // divergence: oppDiv tF + oppDiv_fpts Fcomm (src/elements.cpp:1449-1501)
+    const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
+    double acc[V];
 #pragma unroll
-  for (int v = 0; v < V; v++) acc[v] = 0.0;
+    for (int v = 0; v < V; v++) acc[v] = 0.0;
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-    const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
-    const int base = t - id * str;
-    const double *Dr = tab_Dss<N>(lops, id);
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+      const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
+      const int base = t - id * str;
+      const double *Dr = tab_Dss<N>(lops, id);
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      double s = acc[v];
+      for (int v = 0; v < V; v++) {
+        double s = acc[v];
 #pragma unroll
-      for (int m = 0; m < N; m++) s += Dr[m] * g_l[(d * V + v) * nS + base + m * str];
-      acc[v] = s;
+        for (int m = 0; m < N; m++) s += Dr[m] * g_l[(d * V + v) * nS + base + m * str];
+        acc[v] = s;
+      }
     }
-  }
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-    const double *Df = tab_Dsf<N>(lops, id);
-    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+      const double *Df = tab_Dsf<N>(lops, id);
+      const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      acc[v] += (-Df[0]) * c[v * nF + f0];
-      acc[v] += Df[1] * c[v * nF + f1];
+      for (int v = 0; v < V; v++) {
+        acc[v] += (-Df[0]) * c[v * nF + f0];
+        acc[v] += Df[1] * c[v * nF + f1];
+      }
     }
-  }
 #pragma unroll
-  for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
-  rk_epilogue<D>(a, e, t, nS, acc);
+    for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+    rk_epilogue<D>(a, e, t, nS, acc, unew);
+  }
+  // the new state's full trace for the next stage, into a buffer other than the Utr this
+  // launch reads; the element's U block is no longer read
+  if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, lU + el * V * nS, e, t, valid, unew, -1);
 }
 
 template <int D, int N, bool VISC, int BM, int FC>
@@ -857,6 +897,9 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   constexpr int owner = (BM == 1) ? 0 : 1;
+  // per-element LDS block: the common fluxes (V * nF), then the new state (V * nS) when the
+  // RK update writes the next trace
+  constexpr int W = V * (nF > nS ? nF : nS);
   extern __shared__ double lds[];
   const int EB = a.EB;
   double *lops = lds;
@@ -868,7 +911,7 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
   const int el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   const bool valid = el < neb;
-  double *c = lF + el * V * nF;
+  double *c = lF + el * W;
   if (valid) {
 #pragma unroll
     for (int k = 0; k < S::FPT; k++) {
@@ -883,25 +926,29 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
     }
   }
   __syncthreads();
-  if (!valid) return;
-  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-  double acc[V];
+  double unew[V];
+  if (valid) {
+    const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
+    double acc[V];
 #pragma unroll
-  for (int v = 0; v < V; v++) acc[v] = divF[(e * V + v) * nS + t];
+    for (int v = 0; v < V; v++) acc[v] = divF[(e * V + v) * nS + t];
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-    const double *Df = tab_Dsf<N>(lops, id);
-    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+      const double *Df = tab_Dsf<N>(lops, id);
+      const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      acc[v] += (-Df[0]) * c[v * nF + f0];
-      acc[v] += Df[1] * c[v * nF + f1];
+      for (int v = 0; v < V; v++) {
+        acc[v] += (-Df[0]) * c[v * nF + f0];
+        acc[v] += Df[1] * c[v * nF + f1];
+      }
     }
-  }
 #pragma unroll
-  for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
-  rk_epilogue<D>(a, e, t, nS, acc);
+    for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+    rk_epilogue<D>(a, e, t, nS, acc, unew);
+  }
+  // the new state's partner-owned trace for the next stage
+  if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, c, e, t, valid, unew, owner);
 }
 
 template <int D, int N, int BM>
@@ -1176,7 +1223,7 @@ size_t lds_per_element(Op op, bool viscous) {
     case Op::OneSided:
       per_ele = (size_t)S::V * S::nS + (S::V * S::nF > (int)kG ? S::V * S::nF : kG) + 1;
       break;
-    case Op::FaceCorr: per_ele = (size_t)S::V * S::nF; break;
+    case Op::FaceCorr: per_ele = (size_t)S::V * (S::nF > S::nS ? S::nF : S::nS); break;
     case Op::AdjPrep: per_ele = (size_t)2 * S::V * S::nS; break;
     case Op::AdjMain:
       per_ele = viscous ? (size_t)AdjLds<D, N, true>::per_ele : (size_t)AdjLds<D, N, false>::per_ele;
@@ -1412,6 +1459,10 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
   rk.nStages = r->nStages;
   rk.stage = r->stage;
   rk.mode = r->mode;
+  if (r->trace_out && r->mode != 1) return zfr::fail("RK fusion: trace_out is for stage updates (mode 1)");
+  if (r->trace_out && op == zk::Op::Div && r->trace_out == d->Utr)
+    return zfr::fail("RK fusion: trace_out must not be the trace the divergence kernel reads (d->Utr)");
+  rk.trace = r->trace_out;
   zk::AdjPtrs j;
   j.rk = &rk;
   return zk::dispatch(op, d, U, divF, st, j);

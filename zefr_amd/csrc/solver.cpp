@@ -346,6 +346,7 @@ void FRSolver::setup_device() {
   el.divF_d.alloc((size_t)input->nStages * st * sizeof(double));
   hip_or_throw(hipMemset(el.divF_d.p, 0, el.divF_d.bytes), "hipMemset");
   el.Utr_d.alloc((size_t)E * V * nF * sizeof(double));
+  if (!onesided() && input->nStages > 1) el.Utr2_d.alloc((size_t)E * V * nF * sizeof(double));
   if (onesided()) el.Ftr_d.alloc((size_t)E * V * nF * sizeof(double));
   else if (input->viscous) el.Gtr_d.alloc((size_t)E * D * V * nF * sizeof(double));
   norm_work_d.alloc(1024 * 8 * sizeof(double));
@@ -381,7 +382,8 @@ void FRSolver::compute_residual_start(unsigned, int color) {
 void FRSolver::compute_residual_mid(unsigned stage, int) {
   if (!input->viscous) return;
   if (stage >= input->nStages) raise("stage out of range");
-  const auto d = eles->desc(*input, 1);
+  auto d = eles->desc(*input, 1);
+  if (trace_in) d.Utr = trace_in;
   if (onesided()) {
     double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
     status_or_throw(zfr_gradient_flux(&d, eles->U_spts_d.as<double>(), divF, stream));
@@ -392,7 +394,8 @@ void FRSolver::compute_residual_mid(unsigned stage, int) {
 
 void FRSolver::compute_residual_finish(unsigned stage, int) {
   if (stage >= input->nStages) raise("stage out of range");
-  const auto d = eles->desc(*input, 2);
+  auto d = eles->desc(*input, 2);
+  if (trace_in) d.Utr = trace_in;
   double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
   if (onesided()) status_or_throw(zfr_face_correction(&d, divF, stream));
   else status_or_throw(zfr_divergence(&d, eles->U_spts_d.as<double>(), divF, stream));
@@ -462,9 +465,11 @@ void FRSolver::rk_final_update() {
                                       beta, ns, input->dt, stream));
 }
 
-// compute_residual_finish with the RK update of this stage done by the same kernel.
-void FRSolver::finish_with_rk(int stage) {
-  const auto d = eles->desc(*input, 2);
+// compute_residual_finish with the RK update of this stage done by the same kernel; with
+// trace_out, that kernel also writes the new state's trace there (stage updates only).
+void FRSolver::finish_with_rk(int stage, double *trace_out) {
+  auto d = eles->desc(*input, 2);
+  if (trace_in) d.Utr = trace_in;
   const int ns = (int)input->nStages;
   const size_t st = eles->state_size();
   zfr_rk_args_t r{};
@@ -478,6 +483,7 @@ void FRSolver::finish_with_rk(int stage) {
   if (stage < ns - 1) {
     r.mode = 1;
     r.alpha = kRkAlpha[stage];
+    r.trace_out = trace_out;
   } else {
     r.mode = 2;
     for (int s = 0; s < ns; s++) r.beta[s] = (ns > 1) ? kRkBeta[s] : 1.0;
@@ -487,16 +493,30 @@ void FRSolver::finish_with_rk(int stage) {
   else status_or_throw(zfr_divergence_rk(&d, eles->U_spts_d.as<double>(), divF, &r, stream));
 }
 
+// With fuse_rk, each stage update also writes the next stage's trace, so only stage 0
+// runs the extrapolation kernel.  On the one-sided path the writing kernel
+// (face_correction) reads no trace and writes Utr_d in place; on the divergence path the
+// writing kernel reads the partners' traces, so the stages alternate Utr_d and Utr2_d.
 void FRSolver::step_RK_eager() {
   const int ns = (int)input->nStages;
   if (ns > 1)
     hip_or_throw(hipMemcpyAsync(eles->U_ini_d.p, eles->U_spts_d.p, eles->U_spts_d.bytes, hipMemcpyDeviceToDevice,
                                 stream), "hipMemcpyAsync");
+  double *const tr[2] = {eles->Utr_d.as<double>(),
+                         onesided() ? eles->Utr_d.as<double>() : eles->Utr2_d.as<double>()};
+  const bool chain = fuse_rk && ns > 1 && tr[1] != nullptr;
   for (int stage = 0; stage < ns; stage++) {
     if (fuse_rk) {
-      compute_residual_start(stage);
-      compute_residual_mid(stage);
-      finish_with_rk(stage);
+      trace_in = (chain && stage > 0) ? tr[stage % 2] : nullptr;
+      try {
+        if (!chain || stage == 0) compute_residual_start(stage);
+        compute_residual_mid(stage);
+        finish_with_rk(stage, (chain && stage < ns - 1) ? tr[(stage + 1) % 2] : nullptr);
+      } catch (...) {
+        trace_in = nullptr;
+        throw;
+      }
+      trace_in = nullptr;
     } else {
       compute_residual(stage);
       if (stage < ns - 1) rk_stage_update(stage);
