@@ -220,3 +220,24 @@ def test_setup_rejects_unknown_boundary_group():
     deck = write_deck(tmp("unk.deck"), mesh, 2, 2, False, bounds=(("fluid", "fluid"),))
     with pytest.raises(RuntimeError):
         product(deck, device=False)
+
+
+def test_library_surface_outside_the_residual_path():
+    """The reference module's remaining names exist: adapt_dt returns the supplied dt, the
+    averaging calls are no-ops without tavg, write_wall_time prints the elapsed time, and
+    the force, error and overset (TIOGA) calls raise with the reason."""
+    from zefr_amd import zefr
+    mesh = make_mesh("surface.msh", (2, 2), (0.0, 0.0), (1.0, 1.0))
+    path = _deck_lines(tmp("surface.deck"), [l.format(mesh=mesh) for l in BASE])
+    z = zefr.Zefr()
+    z.read_input(path)
+    assert z.adapt_dt() == z.get_input().dt == 1e-3
+    assert z.update_averages() is None and z.write_averages() is None
+    z.write_wall_time()
+    with pytest.raises(NotImplementedError, match="wall boundaries"):
+        z.write_forces()
+    with pytest.raises(NotImplementedError, match="analytic"):
+        z.write_error()
+    for name in ("get_basic_geo_data", "get_extra_geo_data", "get_gpu_geo_data", "get_callback_funcs"):
+        with pytest.raises(NotImplementedError, match="overset"):
+            getattr(zefr, name)()

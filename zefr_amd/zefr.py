@@ -20,6 +20,7 @@ absent from this build (single grid, fixed mesh).
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -105,6 +106,7 @@ class Zefr:
         if not self._h:
             raise RuntimeError(last_error())
         self._owned = owned and handle is None
+        self._t_start = time.perf_counter()
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -152,6 +154,34 @@ class Zefr:
         r = np.zeros(self.dims()["nVars"])
         check(lib().zefr_write_residual(self._h, r.ctypes.data_as(C.POINTER(C.c_double))))
         return r
+
+    # -- Zefr methods outside the residual path (src/zefr.cpp:432-484)
+    def adapt_dt(self):
+        """Zefr::adapt_dt (src/zefr.cpp:432-438): without adapt_dt (the only mode here;
+        adaptive steps are rejected at input) it returns the supplied dt."""
+        return self.get_input().dt
+
+    def update_averages(self):
+        """Zefr::update_averages (src/zefr.cpp:455-459): a no-op without tavg (time averaging
+        is not supported)."""
+
+    def write_averages(self):
+        """Zefr::write_averages (src/zefr.cpp:461-465): a no-op without tavg."""
+
+    def write_wall_time(self):
+        """Zefr::write_wall_time (src/zefr.cpp:477-484): the wall time since the object was
+        created."""
+        print(f"Elapsed time: {time.perf_counter() - self._t_start} s")
+
+    def write_forces(self):
+        """Zefr::write_forces (src/zefr.cpp:467-470): wall forces need wall boundaries; only
+        periodic meshes are supported."""
+        raise NotImplementedError("write_forces: wall boundaries are not supported (periodic meshes only)")
+
+    def write_error(self):
+        """Zefr::write_error (src/zefr.cpp:472-475): error norms against analytic solutions
+        are not computed."""
+        raise NotImplementedError("write_error: error norms against analytic solutions are not supported")
 
     # -- FRSolver-level calls
     def write_solution(self):
@@ -436,3 +466,18 @@ def get_dq_spts_d(etype=0):
 
 def get_q_spt(ele, spt, var):
     return lib().zefr_get_q_spt(ele, spt, var)
+
+
+# TIOGA/overset data exchange (include/zefr_interface.hpp:98-112): no overset grids here.
+def _no_overset(name):
+    def fn(*args, **kwargs):
+        raise NotImplementedError(f"{name}: overset (TIOGA) grids are not supported")
+    fn.__name__ = name
+    fn.__doc__ = f"zefr::{name} (include/zefr_interface.hpp:98-112): overset grids are not supported."
+    return fn
+
+
+get_basic_geo_data = _no_overset("get_basic_geo_data")
+get_extra_geo_data = _no_overset("get_extra_geo_data")
+get_gpu_geo_data = _no_overset("get_gpu_geo_data")
+get_callback_funcs = _no_overset("get_callback_funcs")
