@@ -92,9 +92,9 @@ struct AdjLds {
 };
 
 template <int D, int N, bool VISC, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
-void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
-                double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
+__device__ __forceinline__ void adj_main_block(const double *__restrict__ U, const double *__restrict__ w,
+                                               const double *__restrict__ Wtr, double *__restrict__ Xtr,
+                                               double *__restrict__ vout, const KArgs &a, long bid) {
   using S = Shape<D, N>;
   using L = AdjLds<D, N, VISC>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
@@ -107,7 +107,7 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
   double *lGB = lR + EB * L::R;
   double *lTo = lGB + EB * L::GB;
   unsigned *lmask = reinterpret_cast<unsigned *>(lTo + EB * V * nF);  // owned faces per element
-  const long e0 = (long)blockIdx.x * EB;
+  const long e0 = bid * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
   if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
@@ -340,6 +340,22 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
   }
 #pragma unroll
   for (int v = 0; v < V; v++) vout[(e * V + v) * nS + t] = Ub[v];
+}
+
+template <int D, int N, bool VISC, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
+                double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
+  adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, blockIdx.x);
+}
+
+// The same kernel for workgroups of one wave (the viscous adjoint runs one element per
+// workgroup; one wave for hex p=3): its barriers need no cross-wave synchronisation.
+template <int D, int N, bool VISC, int BM, int FC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_adj_main_w64(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
+                    double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
+  adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, blockIdx.x);
 }
 
 // v += E^T (the owner's cotangent of this element's trace) at the points the partner owns.

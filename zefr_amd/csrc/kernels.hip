@@ -315,6 +315,12 @@ __global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__res
   extrapolate_block<D, N>(U, a, blockIdx.x);
 }
 
+// The same kernel for workgroups of one wave (see k_visc_onesided_w64).
+template <int D, int N>
+__global__ __launch_bounds__(64) void k_extrapolate_w64(const double *__restrict__ U, KArgs a) {
+  extrapolate_block<D, N>(U, a, blockIdx.x);
+}
+
 template <int D, int N>
 __global__ __launch_bounds__(kMaxThreads) void k_extrapolate_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
@@ -883,6 +889,15 @@ void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KA
   visc_onesided_block<D, N, BM, FC>(U, divF, a, blockIdx.x);
 }
 
+// The same kernel for workgroups of one wave (64 threads: one hex p=3 element per
+// workgroup, the tile the autotuner picks there): the compiler knows the workgroup is a
+// single wave, so its barriers need no cross-wave synchronisation.
+template <int D, int N, int BM, int FC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
+void k_visc_onesided_w64(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
+  visc_onesided_block<D, N, BM, FC>(U, divF, a, blockIdx.x);
+}
+
 template <int D, int N, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
 void k_visc_onesided_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
@@ -1202,7 +1217,12 @@ void launch_adj_main(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds,
                      const AdjPtrs &j, double *out, const KArgs &a) {
   if (!d->viscous)
     hipLaunchKernelGGL((k_adj_main<D, N, false, 0, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-  else if (d->ldg_b == 0.5)
+  else if (block.x <= 64) {
+    if (d->ldg_b == 0.5)
+      hipLaunchKernelGGL((k_adj_main_w64<D, N, true, 1, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
+    else
+      hipLaunchKernelGGL((k_adj_main_w64<D, N, true, 2, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
+  } else if (d->ldg_b == 0.5)
     hipLaunchKernelGGL((k_adj_main<D, N, true, 1, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
   else
     hipLaunchKernelGGL((k_adj_main<D, N, true, 2, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
@@ -1260,11 +1280,10 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   const size_t lds = sizeof(double) * (S::NOPS_PAD + EB * per_ele);
   switch (op) {
     case Op::Extrap:
-      hipLaunchKernelGGL((k_extrapolate<D, N>), grid, block, lds, st, U, a);
-      break;
     case Op::ExtrapPartner:
-      a.trace_sel = 1;
-      hipLaunchKernelGGL((k_extrapolate<D, N>), grid, block, lds, st, U, a);
+      a.trace_sel = (op == Op::ExtrapPartner) ? 1 : 0;
+      if (block.x <= 64) hipLaunchKernelGGL((k_extrapolate_w64<D, N>), grid, block, lds, st, U, a);
+      else hipLaunchKernelGGL((k_extrapolate<D, N>), grid, block, lds, st, U, a);
       break;
     case Op::Grad:
       hipLaunchKernelGGL((k_gradient<D, N>), grid, block, lds, st, U, out, a);
@@ -1274,13 +1293,15 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       else launch_div<D, N, 0>(d, grid, block, lds, st, U, out, a);
       break;
     case Op::OneSided:
-      if (d->fconv_type == 2) {
-        if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_visc_onesided<D, N, 1, 1>), grid, block, lds, st, U, out, a);
-        else hipLaunchKernelGGL((k_visc_onesided<D, N, 2, 1>), grid, block, lds, st, U, out, a);
+#define ZK_OS(K_, B_, F_) hipLaunchKernelGGL((K_<D, N, B_, F_>), grid, block, lds, st, U, out, a)
+      if (block.x <= 64) {
+        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided_w64, 1, 1); else ZK_OS(k_visc_onesided_w64, 2, 1); }
+        else { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided_w64, 1, 0); else ZK_OS(k_visc_onesided_w64, 2, 0); }
       } else {
-        if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_visc_onesided<D, N, 1, 0>), grid, block, lds, st, U, out, a);
-        else hipLaunchKernelGGL((k_visc_onesided<D, N, 2, 0>), grid, block, lds, st, U, out, a);
+        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided, 1, 1); else ZK_OS(k_visc_onesided, 2, 1); }
+        else { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided, 1, 0); else ZK_OS(k_visc_onesided, 2, 0); }
       }
+#undef ZK_OS
       break;
     case Op::FaceCorr:
       if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_face_correction<D, N, 1>), grid, block, lds, st, out, a);
