@@ -217,7 +217,7 @@ int zefr_restart_solution(zefr_t *z);
  * cost.  zefr_batch_residual launches the graph on the batch's stream and returns;
  * zefr_batch_synchronize waits for it; zefr_batch_launches returns the kernel launches in
  * the graph.  The member solvers must have no work pending when the batch is launched.
- * Results land in each solver's divF(0), bit-identical to its own compute_residual(0). */
+ * Results land in each solver's divF(0), equal to its own compute_residual(0) to rounding. */
 typedef struct zefr_batch_s zefr_batch_t;
 zefr_batch_t *zefr_batch_create(zefr_t **solvers, int n);
 int zefr_batch_residual(zefr_batch_t *b);
@@ -285,8 +285,15 @@ int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void 
  * flux: corrected gradient + compute_F + compute_divF_spts (volume part of divF) and the
  * complete common normal flux (Rusanov + LDG, src/faces.cpp:1244-1614) at the flux
  * points whose gradient this element owns, written unscaled to Ftr.  Replaces
- * zfr_gradient + the volume half of zfr_divergence on this path (src/solver.cpp:1552-1610). */
+ * zfr_gradient + the volume half of zfr_divergence on this path (src/solver.cpp:1552-1610),
+ * and extrapolate_U as well: the partner traces it needs (the other side's state at its
+ * owned points) are extrapolated here from the partner's U, so no trace launch precedes
+ * it.  d->Utr is scratch: the kernel parks those traces at its own owned points. */
 int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream);
+/* The same, reading the partner traces from d->Utr at the partner's points instead (as
+ * written by zfr_extrapolate_partner, or by the previous RK stage's fused update with
+ * trace_out = d->Utr); used for RK stages 1.. whose trace is already on the device. */
+int zfr_gradient_flux_traced(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream);
 /* Elements::compute_divF_fpts (src/elements.cpp:1476-1501) on that path:
  * divF += oppDiv_fpts Fcomm with Fcomm = owner's Ftr * signed dA of this side. */
 int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream);
@@ -315,8 +322,8 @@ int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type,
  * with the other stages read from divF0 ([nStages][E][V][nS], stage 0 first).
  * trace_out (mode 1 only; NULL for none): the same kernel also writes the new U's trace
  * there ([E][V][nF]), as the next stage's extrapolation would (zfr_extrapolate_partner for
- * zfr_face_correction_rk, zfr_extrapolate for zfr_divergence_rk), so that launch can be
- * skipped.  zfr_face_correction_rk reads no trace and may write d->Utr itself;
+ * zfr_face_correction_rk, read by zfr_gradient_flux_traced; zfr_extrapolate for
+ * zfr_divergence_rk).  zfr_face_correction_rk reads no trace and may write d->Utr itself;
  * zfr_divergence_rk reads the partners' d->Utr while it writes, so its trace_out must be
  * another buffer (the next stage then runs with Utr = trace_out). */
 typedef struct zfr_rk_args {
@@ -334,8 +341,8 @@ int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, co
                       void *stream);
 /* Vector-Jacobian product of one residual (new; no reference counterpart): for the state
  * U [E][V][nS] and a cotangent w of divF (same layout) writes v = (d divF / dU)^T w.
- * Euler, or Navier-Stokes with LDG beta = +-1/2.  Overwrites d->Utr with the trace of U;
- * Wtr and Xtr are [E][V][nF] scratch buffers. */
+ * Euler, or Navier-Stokes with LDG beta = +-1/2.  d->Utr, Wtr and Xtr are [E][V][nF]
+ * scratch buffers (the partner traces are extrapolated from U inside). */
 int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, double *v, double *Wtr,
                      double *Xtr, void *stream);
 /* Multi-case residual plan (new; no reference counterpart): the residuals of n independent
@@ -344,9 +351,9 @@ int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, 
  * per case and kernel.  Cases are grouped by (kernel, nDims, order, LDG branch, convective
  * flux); each group's workgroups find their case by a binary search over the cases' first
  * workgroups.  Every case takes the kernels its own descriptor selects (Euler: extrapolate
- * -> divergence; LDG beta = +-1/2: partner trace -> gradient_flux -> face_correction;
- * other beta: extrapolate -> gradient -> divergence), with the same arithmetic as the
- * single-case entry points, so the results are bit-identical to them.  The descriptors'
+ * -> divergence; LDG beta = +-1/2: gradient_flux -> face_correction; other beta:
+ * extrapolate -> gradient -> divergence), with the same arithmetic as the
+ * single-case entry points (separately compiled, so they agree to rounding).  The descriptors'
  * tiles are ignored: each group takes one tile for its total element count.
  * zfr_multi_create copies the case tables to the device (hipMalloc, blocking copies);
  * zfr_multi_residual only launches, so it may be captured into a HIP graph;

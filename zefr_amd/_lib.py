@@ -141,6 +141,7 @@ SIGNATURES = {
     "zfr_gradient": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_divergence": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_gradient_flux": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
+    "zfr_gradient_flux_traced": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_face_correction": (_I, [C.POINTER(EleDesc), _P, _P]),
     "zfr_rk_stage": (_I, [C.POINTER(EleDesc), _P, _P, _P, C.c_double, C.c_double, _P]),
     "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),

@@ -11,7 +11,7 @@
 // The owner side of each face (slot 0 for beta = +1/2 and Euler, slot 1 for -1/2) does all
 // the face work, as in the forward path: it combines both sides' weights of its common
 // flux, and hands the partner's share of the trace cotangent back through Xtr.
-//   k_adj_prep   : Utr = E U and Wtr = dA_side * (signed Dsf^T w) at every flux point;
+//   k_adj_prep   : Wtr = dA_side * (signed Dsf^T w) at every flux point;
 //   k_adj_main   : forward recompute, volume and face adjoints, gradient adjoint; writes
 //                  v (all of this element's own contributions) and Xtr at owned points;
 //   k_adj_gather : v += E^T (partner Xtr) at the points the partner owns.
@@ -44,37 +44,51 @@ __device__ __forceinline__ double line_Dsf(const double *lops, int side, const d
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restrict__ U, const double *__restrict__ w,
-                                                          double *__restrict__ Wtr, KArgs a) {
+__global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restrict__ w, double *__restrict__ Wtr,
+                                                          KArgs a) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
   extern __shared__ double lds[];
   double *lops = lds;
-  double *lU = lops + S::NOPS_PAD;
-  double *lW = lU + a.EB * V * nS;
+  double *lW = lops + S::NOPS_PAD;
   const long e0 = (long)blockIdx.x * a.EB;
   const long neb = min((long)a.EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  for (int i = threadIdx.x; i < S::NOPS; i += blockDim.x) lops[i] = a.ops[i];
   stage_block<D, N>(w, lW, e0, neb);
   __syncthreads();
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   if (el >= neb) return;
-  const double *u = lU + el * V * nS, *wl = lW + el * V * nS;
+  const double *wl = lW + el * V * nS;
 #pragma unroll
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
     if (f >= nF) continue;
     int dim, side, base, str;
     fpt_info<D, N>(f, dim, side, base, str);
-    const double *lE = tab_lE<N>(lops, side);
     const double dAs = a.fgeo[(e * (D + 1) + D) * nF + f];
     const double sg = side ? dAs : -dAs;
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      a.Utr[(e * V + v) * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
-      Wtr[(e * V + v) * nF + f] = sg * line_Dsf<N>(lops, side, wl + v * nS, base, str);
-    }
+    for (int v = 0; v < V; v++) Wtr[(e * V + v) * nF + f] = sg * line_Dsf<N>(lops, side, wl + v * nS, base, str);
+  }
+}
+
+// The partner's trace at its flux point fp (all V variables), extrapolated from the
+// partner's U in global memory with the 1D table (global copy of ops1d).
+template <int D, int N>
+__device__ __forceinline__ void partner_trace(const double *__restrict__ U, const double *ops, long ep, int fp,
+                                              double *out) {
+  using S = Shape<D, N>;
+  int dim, side, base, str;
+  fpt_info<D, N>(fp, dim, side, base, str);
+  const double *lE = tab_lE<N>(ops, side);
+  const double *Up = U + ep * S::V * S::nS + base;
+#pragma unroll
+  for (int v = 0; v < S::V; v++) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < N; m++) s += lE[m] * Up[v * S::nS + m * str];
+    out[v] = s;
   }
 }
 
@@ -131,8 +145,14 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
       if (sl == owner && f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
       if (VISC) {
         if (sl == owner) {
+          // the partner's trace, from its U; parked in Utr at this element's own point
+          double pt[V];
+          partner_trace<D, N>(U, a.ops, ep, fp, pt);
 #pragma unroll
-          for (int v = 0; v < V; v++) c[v * nF + f] = a.Utr[(ep * V + v) * nF + fp];
+          for (int v = 0; v < V; v++) {
+            c[v * nF + f] = pt[v];
+            a.Utr[(e * V + v) * nF + f] = pt[v];
+          }
         } else {
           int dim, side, base, str;
           fpt_info<D, N>(f, dim, side, base, str);
@@ -229,10 +249,15 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
         long ep; int sl, fp;
         unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
         double Up[V], Fb[V], Upb[V];
+        if (VISC) {  // parked in phase 1
+#pragma unroll
+          for (int v = 0; v < V; v++) Up[v] = a.Utr[(e * V + v) * nF + f];
+        } else {
+          partner_trace<D, N>(U, a.ops, ep, fp, Up);
+        }
 #pragma unroll
         for (int v = 0; v < V; v++) {
           Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
-          Up[v] = a.Utr[(ep * V + v) * nF + fp];
           Fb[v] = Wtr[(e * V + v) * nF + f] + Wtr[(ep * V + v) * nF + fp];
           Uob[v] = Upb[v] = Gb[v] = 0.0;
         }

@@ -687,7 +687,12 @@ struct OneSidedWaves {
   static constexpr int value = (D == 3 && N == 4) ? (FC == 1 ? 3 : 4) : 1;
 };
 
-template <int D, int N, int BM, int FC>
+// SELF: extrapolate the partner traces this element needs from the partner's U (no trace
+// launch before this kernel; they are parked in Utr at this element's own owned points
+// for the face phase); otherwise read them from Utr at the partner's points (written by a
+// trace launch or by the previous RK stage's update).  A template parameter: as a
+// run-time branch the two prologues together spill the hex p=3 kernel.
+template <int D, int N, int BM, int FC, bool SELF>
 __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U, double *__restrict__ divF,
                                                     const KArgs &a, long bid) {
   using S = Shape<D, N>;
@@ -708,9 +713,12 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
   const long e = e0 + el;
   const bool valid = el < neb;
   // the slots of this thread's flux points and, where this element owns the face, the
-  // partner's trace: fetched before the staging barrier so the loads overlap it
+  // partner's trace: with SELF extrapolated here from the partner's solution line
+  // (no trace launch runs before this kernel), else read from the partner's Utr.  Issued
+  // before the staging barrier so the loads overlap it.
   int slot[S::FPT];
   double cp[S::FPT][V];
+  constexpr bool self = SELF;
 #pragma unroll
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
@@ -720,8 +728,22 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
       slot[k] = sl;
       if (sl == owner) {
+        if (self) {
+          int pdim, pside, pbase, pstr;
+          fpt_info<D, N>(fp, pdim, pside, pbase, pstr);
+          const double *lEp = tab_lE<N>(a.ops, pside);
+          const double *Up = U + ep * V * nS + pbase;
 #pragma unroll
-        for (int v = 0; v < V; v++) cp[k][v] = a.Utr[(ep * V + v) * nF + fp];
+          for (int v = 0; v < V; v++) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < N; m++) s += lEp[m] * Up[v * nS + m * pstr];
+            cp[k][v] = s;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; v++) cp[k][v] = a.Utr[(ep * V + v) * nF + fp];
+        }
       }
     }
   }
@@ -740,7 +762,10 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       if (f >= nF) continue;
       if (slot[k] == owner) {
 #pragma unroll
-        for (int v = 0; v < V; v++) c[v * nF + f] = cp[k][v];
+        for (int v = 0; v < V; v++) {
+          c[v * nF + f] = cp[k][v];
+          if (self) a.Utr[(e * V + v) * nF + f] = cp[k][v];  // own slot: read back in the face phase
+        }
         if (f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
       } else {
         int dim, side, base, str;
@@ -788,10 +813,13 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       if constexpr (FC == 1) {
         // Roe: the convective flux first, so that its many temporaries are dead before the
         // gradient's lines are gathered (formed in the Rusanov order it needs ~350 registers)
-        long ep; int sl, fp;
-        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+        {
+          long ep; int sl, fp;
+          unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+          const long ip = self ? (e * V * nF + f) : (ep * V * nF + fp);
 #pragma unroll
-        for (int v = 0; v < V; v++) Up[v] = a.Utr[(ep * V + v) * nF + fp];
+          for (int v = 0; v < V; v++) Up[v] = a.Utr[ip + v * nF];
+        }
         double Fo[V], G[V];
         if (owner == 0) conv_common_flux<D, FC>(Uo, Up, n, a, Fo);
         else conv_common_flux<D, FC>(Up, Uo, n, a, Fo);
@@ -823,8 +851,9 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       {
         long ep; int sl, fp;
         unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+        const long ip = self ? (e * V * nF + f) : (ep * V * nF + fp);  // cached since phase 1
 #pragma unroll
-        for (int v = 0; v < V; v++) Up[v] = a.Utr[(ep * V + v) * nF + fp];  // cached since phase 1
+        for (int v = 0; v < V; v++) Up[v] = a.Utr[ip + v * nF];
       }
       double Fo[V];
       if (owner == 0) common_flux<D, true, BM, FC>(Uo, Up, G, G, n, 1.0, a, Fo);
@@ -883,27 +912,28 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
   for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
 }
 
-template <int D, int N, int BM, int FC>
+template <int D, int N, int BM, int FC, bool SELF>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
 void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
-  visc_onesided_block<D, N, BM, FC>(U, divF, a, blockIdx.x);
+  visc_onesided_block<D, N, BM, FC, SELF>(U, divF, a, blockIdx.x);
 }
 
 // The same kernel for workgroups of one wave (64 threads: one hex p=3 element per
 // workgroup, the tile the autotuner picks there): the compiler knows the workgroup is a
 // single wave, so its barriers need no cross-wave synchronisation.
-template <int D, int N, int BM, int FC>
+template <int D, int N, int BM, int FC, bool SELF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
 void k_visc_onesided_w64(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
-  visc_onesided_block<D, N, BM, FC>(U, divF, a, blockIdx.x);
+  visc_onesided_block<D, N, BM, FC, SELF>(U, divF, a, blockIdx.x);
 }
 
 template <int D, int N, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
 void k_visc_onesided_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
-  visc_onesided_block<D, N, BM, FC>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
+  visc_onesided_block<D, N, BM, FC, true>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
 }
+
 
 // divF += oppDiv_fpts Fcomm (src/elements.cpp:1476-1501) with Fcomm = Ftr(owner) * signed
 // dA of this side; the owner's Ftr is this element's own or its partner's.
@@ -1191,7 +1221,7 @@ inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
   return a;
 }
 
-enum class Op { Extrap, ExtrapPartner, Grad, Div, OneSided, FaceCorr, AdjPrep, AdjMain, AdjGather };
+enum class Op { Extrap, ExtrapPartner, Grad, Div, OneSided, OneSidedFromTrace, FaceCorr, AdjPrep, AdjMain, AdjGather };
 
 // k_divergence for one convective flux (FC: 0 Rusanov, 1 Roe) and the LDG beta branch.
 template <int D, int N, int FC>
@@ -1241,10 +1271,11 @@ size_t lds_per_element(Op op, bool viscous) {
     case Op::Grad: per_ele = kUC > kG ? kUC : kG; break;
     case Op::Div: per_ele = kUC + kG; break;
     case Op::OneSided:
+    case Op::OneSidedFromTrace:
       per_ele = (size_t)S::V * S::nS + (S::V * S::nF > (int)kG ? S::V * S::nF : kG) + 1;
       break;
     case Op::FaceCorr: per_ele = (size_t)S::V * (S::nF > S::nS ? S::nF : S::nS); break;
-    case Op::AdjPrep: per_ele = (size_t)2 * S::V * S::nS; break;
+    case Op::AdjPrep: per_ele = (size_t)S::V * S::nS; break;
     case Op::AdjMain:
       per_ele = viscous ? (size_t)AdjLds<D, N, true>::per_ele : (size_t)AdjLds<D, N, false>::per_ele;
       break;
@@ -1264,7 +1295,8 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   if (adj && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5)
     return zfr::fail("the residual VJP supports LDG beta = +-0.5 (and Euler)");
   if (d->fconv_type != 0 && d->fconv_type != 2) return zfr::fail("fconv_type must be 0 (Rusanov) or 2 (Roe)");
-  if ((op == Op::OneSided || op == Op::FaceCorr) && d->ldg_b != 0.5 && d->ldg_b != -0.5)
+  if ((op == Op::OneSided || op == Op::OneSidedFromTrace || op == Op::FaceCorr) && d->ldg_b != 0.5 &&
+      d->ldg_b != -0.5)
     return zfr::fail("one-sided LDG kernels need ldg_b = +-0.5");
   // the viscous adjoint runs one element per workgroup: its LDS then allows ~7
   // workgroups per CU, more than any larger tile
@@ -1293,22 +1325,26 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       else launch_div<D, N, 0>(d, grid, block, lds, st, U, out, a);
       break;
     case Op::OneSided:
-#define ZK_OS(K_, B_, F_) hipLaunchKernelGGL((K_<D, N, B_, F_>), grid, block, lds, st, U, out, a)
+    case Op::OneSidedFromTrace: {
+#define ZK_OS(K_, B_, F_, S_) hipLaunchKernelGGL((K_<D, N, B_, F_, S_>), grid, block, lds, st, U, out, a)
+#define ZK_OS2(K_, B_, F_) do { if (op == Op::OneSided) ZK_OS(K_, B_, F_, true); else ZK_OS(K_, B_, F_, false); } while (0)
       if (block.x <= 64) {
-        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided_w64, 1, 1); else ZK_OS(k_visc_onesided_w64, 2, 1); }
-        else { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided_w64, 1, 0); else ZK_OS(k_visc_onesided_w64, 2, 0); }
+        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided_w64, 1, 1); else ZK_OS2(k_visc_onesided_w64, 2, 1); }
+        else { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided_w64, 1, 0); else ZK_OS2(k_visc_onesided_w64, 2, 0); }
       } else {
-        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided, 1, 1); else ZK_OS(k_visc_onesided, 2, 1); }
-        else { if (d->ldg_b == 0.5) ZK_OS(k_visc_onesided, 1, 0); else ZK_OS(k_visc_onesided, 2, 0); }
+        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided, 1, 1); else ZK_OS2(k_visc_onesided, 2, 1); }
+        else { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided, 1, 0); else ZK_OS2(k_visc_onesided, 2, 0); }
       }
+#undef ZK_OS2
 #undef ZK_OS
       break;
+    }
     case Op::FaceCorr:
       if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_face_correction<D, N, 1>), grid, block, lds, st, out, a);
       else hipLaunchKernelGGL((k_face_correction<D, N, 2>), grid, block, lds, st, out, a);
       break;
     case Op::AdjPrep:
-      hipLaunchKernelGGL((k_adj_prep<D, N>), grid, block, lds, st, U, j.w, j.Wtr, a);
+      hipLaunchKernelGGL((k_adj_prep<D, N>), grid, block, lds, st, j.w, j.Wtr, a);
       break;
     case Op::AdjMain:
       if (d->fconv_type == 2) launch_adj_main<D, N, 1>(d, grid, block, lds, st, U, j, out, a);
@@ -1379,13 +1415,10 @@ void launch_multi(const MultiGroup &g, hipStream_t st) {
 #undef ZK_DIV
       break;
     case Op::OneSided:
-      if (g.bm == 1) {
-        if (g.fc) hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 1, 1>), grid, block, g.lds, st, c, s0, g.nc);
-        else hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 1, 0>), grid, block, g.lds, st, c, s0, g.nc);
-      } else {
-        if (g.fc) hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 2, 1>), grid, block, g.lds, st, c, s0, g.nc);
-        else hipLaunchKernelGGL((k_visc_onesided_multi<D, N, 2, 0>), grid, block, g.lds, st, c, s0, g.nc);
-      }
+#define ZK_OSM(B_, F_) hipLaunchKernelGGL((k_visc_onesided_multi<D, N, B_, F_>), grid, block, g.lds, st, c, s0, g.nc)
+      if (g.bm == 1) { if (g.fc) ZK_OSM(1, 1); else ZK_OSM(1, 0); }
+      else { if (g.fc) ZK_OSM(2, 1); else ZK_OSM(2, 0); }
+#undef ZK_OSM
       break;
     case Op::FaceCorr:
       if (g.bm == 1) hipLaunchKernelGGL((k_face_correction_multi<D, N, 1>), grid, block, g.lds, st, c, s0, g.nc);
@@ -1453,6 +1486,11 @@ int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void 
 int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
   if (!d->viscous) return zfr::fail("zfr_gradient_flux: descriptor is inviscid");
   return zk::dispatch(zk::Op::OneSided, d, U, divF, (hipStream_t)stream);
+}
+
+int zfr_gradient_flux_traced(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
+  if (!d->viscous) return zfr::fail("zfr_gradient_flux_traced: descriptor is inviscid");
+  return zk::dispatch(zk::Op::OneSidedFromTrace, d, U, divF, (hipStream_t)stream);
 }
 
 int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream) {
@@ -1648,8 +1686,7 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
     if (d->nEles <= 0) continue;
     if (visc && bm != 0) {
       if (!d->Ftr) return zfr::fail("zfr_multi_create: the one-sided path needs Ftr");
-      phase[0][{Op::ExtrapPartner, D, N, 0, bm, 0}].push_back(i);
-      phase[1][{Op::OneSided, D, N, 1, bm, fc}].push_back(i);
+      phase[1][{Op::OneSided, D, N, 1, bm, fc}].push_back(i);  // extrapolates the partner traces itself
       phase[2][{Op::FaceCorr, D, N, 1, bm, 0}].push_back(i);
     } else {
       if (visc && !d->Gtr) return zfr::fail("zfr_multi_create: the general LDG path needs Gtr");

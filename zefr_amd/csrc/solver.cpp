@@ -375,8 +375,8 @@ void FRSolver::compute_residual_start(unsigned, int color) {
   need_device(*this);
   if (color > -1) raise("multi-colour (MCGS) residuals are not supported");
   const auto d = eles->desc(*input, 0);
-  if (onesided()) status_or_throw(zfr_extrapolate_partner(&d, eles->U_spts_d.as<double>(), stream));
-  else status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
+  // the one-sided kernel extrapolates the partner traces it needs itself
+  if (!onesided()) status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
 }
 
 void FRSolver::compute_residual_mid(unsigned stage, int) {
@@ -386,7 +386,9 @@ void FRSolver::compute_residual_mid(unsigned stage, int) {
   if (trace_in) d.Utr = trace_in;
   if (onesided()) {
     double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
-    status_or_throw(zfr_gradient_flux(&d, eles->U_spts_d.as<double>(), divF, stream));
+    // with trace_in (RK stages 1..), the previous stage's update has written the traces
+    if (trace_in) status_or_throw(zfr_gradient_flux_traced(&d, eles->U_spts_d.as<double>(), divF, stream));
+    else status_or_throw(zfr_gradient_flux(&d, eles->U_spts_d.as<double>(), divF, stream));
   } else {
     status_or_throw(zfr_gradient(&d, eles->U_spts_d.as<double>(), nullptr, stream));
   }
@@ -412,7 +414,6 @@ void FRSolver::residual_at(const double *U, double *divF, hipStream_t st) {
   need_device(*this);
   const auto d0 = eles->desc(*input, 0), d1 = eles->desc(*input, 1), d2 = eles->desc(*input, 2);
   if (onesided()) {
-    status_or_throw(zfr_extrapolate_partner(&d0, U, st));
     status_or_throw(zfr_gradient_flux(&d1, U, divF, st));
     status_or_throw(zfr_face_correction(&d2, divF, st));
   } else {
@@ -494,9 +495,11 @@ void FRSolver::finish_with_rk(int stage, double *trace_out) {
 }
 
 // With fuse_rk, each stage update also writes the next stage's trace, so only stage 0
-// runs the extrapolation kernel.  On the one-sided path the writing kernel
-// (face_correction) reads no trace and writes Utr_d in place; on the divergence path the
-// writing kernel reads the partners' traces, so the stages alternate Utr_d and Utr2_d.
+// forms it from U (the extrapolation kernel, or on the one-sided path the gradient_flux
+// kernel itself).  On the one-sided path the writing kernel (face_correction) reads no
+// trace and writes Utr_d in place, and stages 1.. read it (gradient_flux_traced); on the
+// divergence path the writing kernel reads the partners' traces, so the stages alternate
+// Utr_d and Utr2_d.
 void FRSolver::step_RK_eager() {
   const int ns = (int)input->nStages;
   if (ns > 1)
@@ -686,8 +689,7 @@ void FRSolver::time_kernels(int reps, double *ms_out) {
   double *U = el.U_spts_d.as<double>(), *divF = el.divF_d.as<double>();
   for (int r = 0; r < reps; r++) {
     hip_or_throw(hipEventRecord(ev[3 * r], stream), "hipEventRecord");
-    if (onesided()) status_or_throw(zfr_extrapolate_partner(&d0, U, stream));
-    else status_or_throw(zfr_extrapolate(&d0, U, stream));
+    if (!onesided()) status_or_throw(zfr_extrapolate(&d0, U, stream));  // one-sided: no trace launch
     hip_or_throw(hipEventRecord(ev[3 * r + 1], stream), "hipEventRecord");
     if (onesided()) status_or_throw(zfr_gradient_flux(&d1, U, divF, stream));
     else if (input->viscous) status_or_throw(zfr_gradient(&d1, U, nullptr, stream));
