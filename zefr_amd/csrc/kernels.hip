@@ -19,9 +19,10 @@
 //    re-reading the gradient.
 //  * With LDG beta = +-1/2 (the usual setting) only one side's gradient enters each
 //    common flux and the common solution is the other side's trace, so the "owner" side
-//    can finish the whole common flux right after its own gradient: extrapolate ->
-//    gradient_flux (gradient, volume divergence, owned common fluxes) -> face_correction.
-//    No gradient trace is stored at all.
+//    can finish the whole common flux right after its own gradient, and the only other-
+//    side data it needs (the partner's trace at its owned points) it extrapolates from the
+//    partner's U itself: gradient_flux (gradient, volume divergence, owned common fluxes)
+//    -> face_correction, two launches.  No gradient trace is stored at all.
 //  * Euler: extrapolate -> divergence.
 //
 // All arithmetic is FP64.  Reference call sites are cited at each kernel.
@@ -673,9 +674,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence_multi(const MultiCas
 // and the common solution is the other side's state.  The element on that "owner" side
 // therefore has everything for the complete common flux (Rusanov + LDG) right after its
 // own corrected gradient: it writes the unscaled common normal flux to Ftr at its owned
-// flux points, and no gradient trace is needed.  A residual is then
-//   k_extrapolate -> k_visc_onesided (gradient, volume flux divergence, owned common
-//   fluxes) -> k_face_correction (oppDiv_fpts Fcomm, both sides).
+// flux points, and no gradient trace is needed.  The partner's solution trace at an owned
+// point is extrapolated from the partner's U inside the kernel.  A residual is then
+//   k_visc_onesided (gradient, volume flux divergence, owned common fluxes)
+//   -> k_face_correction (oppDiv_fpts Fcomm, both sides).
 
 // Register budget: for hex p=3 four waves per SIMD (<= 128 VGPRs) is what the kernel's
 // LDS allows per CU and the Rusanov face phase fits it without spilling, so the allocator
