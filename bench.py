@@ -359,7 +359,7 @@ def batch_timing(workdir, count=64, reps=20, seed=2024):
     seeded sizes n x n (n in 8..64) and orders 1..5, Euler, seeded perturbed freestream
     (5% Gaussian point noise).  One residual of every case per batch: eager launches on
     the solvers' own streams against replays of the batch graph (the members' residuals as a
-    multi-case plan: one launch per kernel and order).  Each leg runs `reps` batches back to
+    multi-case plan: one launch per order; an Euler residual is one kernel).  Each leg runs `reps` batches back to
     back and synchronises once."""
     import numpy as np
     from zefr_amd import zefr
@@ -407,7 +407,7 @@ def batch_timing(workdir, count=64, reps=20, seed=2024):
     launches = b.launches
     del b, zs
     return {"workload": f"{count} independent periodic quad cases (n in 8..64, p in 1..5, seed {seed}), Euler, "
-                        "one residual each", "dof": dof, "eager_ms": out["eager_ms"], "eager_launches": 2 * count,
+                        "one residual each", "dof": dof, "eager_ms": out["eager_ms"], "eager_launches": count,
             "graph_ms": out["graph_ms"], "graph_launches": launches,
             "graph_Gdof_res_per_s": round(dof / (out["graph_ms"] * 1e-3) / 1e9, 3)}
 

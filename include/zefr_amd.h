@@ -279,7 +279,10 @@ int zfr_extrapolate_partner(const zfr_ele_desc_t *d, const double *U, void *stre
 int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void *stream);
 /* Elements::compute_F + compute_divF_spts + Faces::compute_common_F + compute_divF_fpts
  * (src/elements.cpp:1449-1501, 1588-1699; src/faces.cpp:1244-1364, 1476-1676):
- * divF[E][V][nS] from U and the traces. */
+ * divF[E][V][nS] from U and the traces.  Euler: from U alone (the partner traces are
+ * extrapolated from the partner's U inside, replacing extrapolate_U; d->Utr is not read);
+ * viscous: reads the solution trace Utr and the gradient trace Gtr (zfr_extrapolate,
+ * zfr_gradient).  zfr_divergence_rk reads Utr in both cases (its update overwrites U). */
 int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream);
 /* Navier-Stokes with LDG ldg_b = +-1/2, where one side's gradient enters each common
  * flux: corrected gradient + compute_F + compute_divF_spts (volume part of divF) and the
@@ -350,9 +353,9 @@ int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, 
  * [E_i][V][nS_i]) in one launch per kernel, shape and flux branch instead of one launch
  * per case and kernel.  Cases are grouped by (kernel, nDims, order, LDG branch, convective
  * flux); each group's workgroups find their case by a binary search over the cases' first
- * workgroups.  Every case takes the kernels its own descriptor selects (Euler: extrapolate
- * -> divergence; LDG beta = +-1/2: gradient_flux -> face_correction; other beta:
- * extrapolate -> gradient -> divergence), with the same arithmetic as the
+ * workgroups.  Every case takes the kernels its own descriptor selects (Euler: divergence;
+ * LDG beta = +-1/2: gradient_flux -> face_correction; other beta: extrapolate -> gradient
+ * -> divergence), with the same arithmetic as the
  * single-case entry points (separately compiled, so they agree to rounding).  The descriptors'
  * tiles are ignored: each group takes one tile for its total element count.
  * zfr_multi_create copies the case tables to the device (hipMalloc, blocking copies);

@@ -73,25 +73,6 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restri
   }
 }
 
-// The partner's trace at its flux point fp (all V variables), extrapolated from the
-// partner's U in global memory with the 1D table (global copy of ops1d).
-template <int D, int N>
-__device__ __forceinline__ void partner_trace(const double *__restrict__ U, const double *ops, long ep, int fp,
-                                              double *out) {
-  using S = Shape<D, N>;
-  int dim, side, base, str;
-  fpt_info<D, N>(fp, dim, side, base, str);
-  const double *lE = tab_lE<N>(ops, side);
-  const double *Up = U + ep * S::V * S::nS + base;
-#pragma unroll
-  for (int v = 0; v < S::V; v++) {
-    double s = 0.0;
-#pragma unroll
-    for (int m = 0; m < N; m++) s += lE[m] * Up[v * S::nS + m * str];
-    out[v] = s;
-  }
-}
-
 // LDS per element: U [V][nS]; one region that holds the common solution, then the forward
 // gradient [D][V][nS]; the gradient cotangent [D][V][nS] (viscous); the own trace
 // cotangent [V][nF]; the owned-face mask.  w is read from global memory (each value by

@@ -191,6 +191,26 @@ __device__ __forceinline__ double line_extrap(const double *lE, const double *bl
   return s;
 }
 
+// The partner's trace at its flux point fp (all V variables), extrapolated from the
+// partner's solution line in global memory with the 1D table (global copy of ops1d): what
+// k_extrapolate writes there, for kernels that run without a trace launch before them.
+template <int D, int N>
+__device__ __forceinline__ void partner_trace(const double *__restrict__ U, const double *ops, long ep, int fp,
+                                              double *out) {
+  using S = Shape<D, N>;
+  int dim, side, base, str;
+  fpt_info<D, N>(fp, dim, side, base, str);
+  const double *lE = tab_lE<N>(ops, side);
+  const double *Up = U + ep * S::V * S::nS + base;
+#pragma unroll
+  for (int v = 0; v < S::V; v++) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < N; m++) s += lE[m] * Up[v * S::nS + m * str];
+    out[v] = s;
+  }
+}
+
 // RK update of one spt straight after its residual, in the last kernel of the residual
 // (the stage update of src/solver.cpp:2897-2941 and the final combination of
 // :2997-3062, with the same arithmetic as k_rk_stage / k_rk_final).  acc holds this
@@ -489,7 +509,9 @@ __global__ __launch_bounds__(kMaxThreads) void k_gradient_multi(const MultiCase 
 // compute_F + compute_divF_spts + compute_common_F + compute_divF_fpts.
 // Phases (viscous): U -> LDS | Ucomm -> lC | gradient -> lG | common flux -> lC |
 // transformed flux -> lG | divergence.  Inviscid: U -> LDS | flux + common flux | divergence.
-template <int D, int N, bool VISC, int BM, int FC>
+// SELF (Euler only): the partner traces are extrapolated here from the partner's U, so an
+// Euler residual is this one launch; otherwise they are read from Utr.
+template <int D, int N, bool VISC, int BM, int FC, bool SELF>
 __device__ __forceinline__ void divergence_block(const double *__restrict__ U, double *__restrict__ divF,
                                                  const KArgs &a, long bid) {
   using S = Shape<D, N>;
@@ -547,8 +569,12 @@ __device__ __forceinline__ void divergence_block(const double *__restrict__ U, d
         for (int v = 0; v < V; v++) Us[v] = line_extrap<N>(lE, u + v * nS, base, str);
       };
       auto par_state = [&](double *Us) {
+        if constexpr (SELF) {
+          partner_trace<D, N>(U, a.ops, ep, fp, Us);
+        } else {
 #pragma unroll
-        for (int v = 0; v < V; v++) Us[v] = a.Utr[(ep * V + v) * nF + fp];
+          for (int v = 0; v < V; v++) Us[v] = a.Utr[(ep * V + v) * nF + fp];
+        }
       };
       // viscous normal fluxes first (one side's state + gradient live at a time):
       // G1 = left (BM 0, 1) or right (BM 2) side, G2 = right side (BM 0)
@@ -656,16 +682,17 @@ __device__ __forceinline__ void divergence_block(const double *__restrict__ U, d
   if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, lU + el * V * nS, e, t, valid, unew, -1);
 }
 
-template <int D, int N, bool VISC, int BM, int FC>
+template <int D, int N, bool VISC, int BM, int FC, bool SELF = !VISC>
 __global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__restrict__ U, double *__restrict__ divF,
                                                             KArgs a) {
-  divergence_block<D, N, VISC, BM, FC>(U, divF, a, blockIdx.x);
+  static_assert(!(VISC && SELF), "the viscous divergence reads the gradient traces, hence the traces too");
+  divergence_block<D, N, VISC, BM, FC, SELF>(U, divF, a, blockIdx.x);
 }
 
 template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) void k_divergence_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
-  divergence_block<D, N, VISC, BM, FC>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
+  divergence_block<D, N, VISC, BM, FC, !VISC>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
 }
 
 // ---------------------------------------------------------------- one-sided LDG path
@@ -731,17 +758,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       slot[k] = sl;
       if (sl == owner) {
         if (self) {
-          int pdim, pside, pbase, pstr;
-          fpt_info<D, N>(fp, pdim, pside, pbase, pstr);
-          const double *lEp = tab_lE<N>(a.ops, pside);
-          const double *Up = U + ep * V * nS + pbase;
-#pragma unroll
-          for (int v = 0; v < V; v++) {
-            double s = 0.0;
-#pragma unroll
-            for (int m = 0; m < N; m++) s += lEp[m] * Up[v * nS + m * pstr];
-            cp[k][v] = s;
-          }
+          partner_trace<D, N>(U, a.ops, ep, fp, cp[k]);
         } else {
 #pragma unroll
           for (int v = 0; v < V; v++) cp[k][v] = a.Utr[(ep * V + v) * nF + fp];
@@ -1226,10 +1243,15 @@ inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
 enum class Op { Extrap, ExtrapPartner, Grad, Div, OneSided, OneSidedFromTrace, FaceCorr, AdjPrep, AdjMain, AdjGather };
 
 // k_divergence for one convective flux (FC: 0 Rusanov, 1 Roe) and the LDG beta branch.
+// Euler: the standalone residual extrapolates the partner traces itself (SELF); with a
+// fused RK update it must read them from Utr, since its epilogue overwrites U in place
+// while other workgroups may still read it.
 template <int D, int N, int FC>
 void launch_div(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds, hipStream_t st, const double *U,
                 double *out, const KArgs &a) {
-  if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC>), grid, block, lds, st, U, out, a);
+  if (!d->viscous && a.rk.mode != 0)
+    hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, false>), grid, block, lds, st, U, out, a);
+  else if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, true>), grid, block, lds, st, U, out, a);
   else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 1, FC>), grid, block, lds, st, U, out, a);
   else if (d->ldg_b == -0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 2, FC>), grid, block, lds, st, U, out, a);
   else hipLaunchKernelGGL((k_divergence<D, N, true, 0, FC>), grid, block, lds, st, U, out, a);
@@ -1692,7 +1714,7 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
       phase[2][{Op::FaceCorr, D, N, 1, bm, 0}].push_back(i);
     } else {
       if (visc && !d->Gtr) return zfr::fail("zfr_multi_create: the general LDG path needs Gtr");
-      phase[0][{Op::Extrap, D, N, 0, 0, 0}].push_back(i);
+      if (visc) phase[0][{Op::Extrap, D, N, 0, 0, 0}].push_back(i);  // Euler: extrapolated in the divergence
       if (visc) phase[1][{Op::Grad, D, N, 1, 0, 0}].push_back(i);
       phase[2][{Op::Div, D, N, visc, bm, fc}].push_back(i);
     }

@@ -375,8 +375,8 @@ void FRSolver::compute_residual_start(unsigned, int color) {
   need_device(*this);
   if (color > -1) raise("multi-colour (MCGS) residuals are not supported");
   const auto d = eles->desc(*input, 0);
-  // the one-sided kernel extrapolates the partner traces it needs itself
-  if (!onesided()) status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
+  // the one-sided and the Euler kernels extrapolate the partner traces they need themselves
+  if (input->viscous && !onesided()) status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
 }
 
 void FRSolver::compute_residual_mid(unsigned stage, int) {
@@ -417,9 +417,11 @@ void FRSolver::residual_at(const double *U, double *divF, hipStream_t st) {
     status_or_throw(zfr_gradient_flux(&d1, U, divF, st));
     status_or_throw(zfr_face_correction(&d2, divF, st));
   } else {
-    status_or_throw(zfr_extrapolate(&d0, U, st));
-    if (input->viscous) status_or_throw(zfr_gradient(&d1, U, nullptr, st));
-    status_or_throw(zfr_divergence(&d2, U, divF, st));
+    if (input->viscous) {
+      status_or_throw(zfr_extrapolate(&d0, U, st));
+      status_or_throw(zfr_gradient(&d1, U, nullptr, st));
+    }
+    status_or_throw(zfr_divergence(&d2, U, divF, st));  // Euler: extrapolates the partner traces itself
   }
 }
 
@@ -497,9 +499,11 @@ void FRSolver::finish_with_rk(int stage, double *trace_out) {
 // With fuse_rk, each stage update also writes the next stage's trace, so only stage 0
 // forms it from U (the extrapolation kernel, or on the one-sided path the gradient_flux
 // kernel itself).  On the one-sided path the writing kernel (face_correction) reads no
-// trace and writes Utr_d in place, and stages 1.. read it (gradient_flux_traced); on the
-// divergence path the writing kernel reads the partners' traces, so the stages alternate
-// Utr_d and Utr2_d.
+// trace and writes Utr_d in place, and stages 1.. read it (gradient_flux_traced).  On
+// the Euler and general-LDG paths the writing kernel (divergence) reads the partners'
+// traces, so the stages alternate Utr_d and Utr2_d; there the fused divergence reads the
+// trace even for Euler, whose standalone residual extrapolates it itself, because its
+// update overwrites U while other workgroups may still read it.
 void FRSolver::step_RK_eager() {
   const int ns = (int)input->nStages;
   if (ns > 1)
@@ -512,7 +516,11 @@ void FRSolver::step_RK_eager() {
     if (fuse_rk) {
       trace_in = (chain && stage > 0) ? tr[stage % 2] : nullptr;
       try {
-        if (!chain || stage == 0) compute_residual_start(stage);
+        // the fused divergence (Euler, general LDG) reads a trace: form it at stage 0
+        if ((!chain || stage == 0) && !onesided()) {
+          const auto d0 = eles->desc(*input, 0);
+          status_or_throw(zfr_extrapolate(&d0, eles->U_spts_d.as<double>(), stream));
+        }
         compute_residual_mid(stage);
         finish_with_rk(stage, (chain && stage < ns - 1) ? tr[(stage + 1) % 2] : nullptr);
       } catch (...) {
@@ -689,7 +697,7 @@ void FRSolver::time_kernels(int reps, double *ms_out) {
   double *U = el.U_spts_d.as<double>(), *divF = el.divF_d.as<double>();
   for (int r = 0; r < reps; r++) {
     hip_or_throw(hipEventRecord(ev[3 * r], stream), "hipEventRecord");
-    if (!onesided()) status_or_throw(zfr_extrapolate(&d0, U, stream));  // one-sided: no trace launch
+    if (input->viscous && !onesided()) status_or_throw(zfr_extrapolate(&d0, U, stream));  // else no trace launch
     hip_or_throw(hipEventRecord(ev[3 * r + 1], stream), "hipEventRecord");
     if (onesided()) status_or_throw(zfr_gradient_flux(&d1, U, divF, stream));
     else if (input->viscous) status_or_throw(zfr_gradient(&d1, U, nullptr, stream));
