@@ -23,7 +23,8 @@
 //    side data it needs (the partner's trace at its owned points) it extrapolates from the
 //    partner's U itself: gradient_flux (gradient, volume divergence, owned common fluxes)
 //    -> face_correction, two launches.  No gradient trace is stored at all.
-//  * Euler: extrapolate -> divergence.
+//  * Euler: divergence alone, extrapolating the partner traces from the partner's U (with
+//    a fused RK update: extrapolate -> divergence, since the update overwrites U).
 //
 // All arithmetic is FP64.  Reference call sites are cited at each kernel.
 
