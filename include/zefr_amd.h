@@ -16,7 +16,9 @@
  *     include/solver_kernels.h:18-22), which take C++ containers in the reference.
  *
  * Every function returning int returns 0 on success and a nonzero status otherwise;
- * `zefr_last_error()` then holds a thread-local message.  No entry point aborts.
+ * `zefr_last_error()` then holds a thread-local message.  No entry point aborts.  The
+ * kernel-level entry points check that the descriptor's arrays a launch touches are not
+ * NULL before launching.
  *
  * Device data layout (element-major, "element-blocked"): per element e the solution
  * block is U[e][var][spt], so the reference's stride-based accessors still describe it
@@ -216,7 +218,8 @@ int zefr_restart_solution(zefr_t *z);
  * many small cases whose residuals are too small to fill the GPU or to outlast the launch
  * cost.  zefr_batch_residual launches the graph on the batch's stream and returns;
  * zefr_batch_synchronize waits for it; zefr_batch_launches returns the kernel launches in
- * the graph.  The member solvers must have no work pending when the batch is launched.
+ * the graph.  The member solvers must have no work pending when the batch is launched,
+ * and must outlive the batch (the graph holds their device arrays).
  * Results land in each solver's divF(0), equal to its own compute_residual(0) to rounding. */
 typedef struct zefr_batch_s zefr_batch_t;
 zefr_batch_t *zefr_batch_create(zefr_t **solvers, int n);

@@ -87,3 +87,28 @@ def test_errors_without_a_device_are_reported_not_raised_through_c():
         assert _lib.last_error()
     finally:
         L.zefr_destroy(z)
+
+
+def test_kernel_entries_reject_missing_buffers_before_any_launch():
+    """A descriptor without the arrays a launch would touch is an error with a message,
+    returned before any HIP call (so this runs without a GPU), not a device fault."""
+    L = _lib.lib()
+    d = _lib.EleDesc(nDims=3, order=3, nVars=5, nEles=8, nSpts=64, nFpts=96, viscous=1, ldg_b=0.5,
+                     gamma=1.4, prandtl=0.72, mu=1e-3, fix_vis=1, ldg_tau=0.1)
+    buf = C.c_void_p(0x1000)  # never dereferenced: every call below fails its checks first
+    cases = [
+        ("zfr_gradient_flux", (C.byref(d), buf, buf, None), "ops1d"),
+        ("zfr_face_correction", (C.byref(d), buf, None), "ops1d"),
+        ("zfr_divergence", (C.byref(d), buf, buf, None), "ops1d"),
+    ]
+    for name, args, msg in cases:
+        assert getattr(L, name)(*args) != 0
+        assert msg in _lib.last_error()
+    d.ops1d = d.metric = d.fgeo = d.nbr = buf  # geometry present, traces missing
+    assert L.zfr_gradient_flux(C.byref(d), buf, buf, None) != 0 and "Utr" in _lib.last_error()
+    assert L.zfr_gradient_flux_traced(C.byref(d), buf, buf, None) != 0 and "Utr" in _lib.last_error()
+    d.Utr = buf
+    assert L.zfr_gradient_flux(C.byref(d), buf, buf, None) != 0 and "Ftr" in _lib.last_error()
+    assert L.zfr_face_correction(C.byref(d), buf, None) != 0 and "Ftr" in _lib.last_error()
+    d.ldg_b = 0.2
+    assert L.zfr_divergence(C.byref(d), buf, buf, None) != 0 and "Gtr" in _lib.last_error()

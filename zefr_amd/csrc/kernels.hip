@@ -1489,36 +1489,77 @@ inline unsigned grid_for(long total) {
 
 extern "C" {
 
+namespace {
+// Buffers a launch reads or writes through the descriptor: a null one is an error here,
+// not a fault on the device.
+int need(const void *p, const char *what) {
+  return p ? 0 : zfr::fail(std::string("kernel launch: the descriptor's ") + what + " is NULL");
+}
+int need_geometry(const zfr_ele_desc_t *d) {
+  if (!d) return zfr::fail("kernel launch: NULL descriptor");
+  int rc = need(d->ops1d, "ops1d");
+  if (!rc) rc = need(d->metric, "metric");
+  if (!rc) rc = need(d->fgeo, "fgeo");
+  if (!rc) rc = need(d->nbr, "nbr");
+  return rc;
+}
+}  // namespace
+
 int zfr_extrapolate(const zfr_ele_desc_t *d, const double *U, void *stream) {
+  int rc = need_geometry(d);
+  if (!rc) rc = need(d->Utr, "Utr");
+  if (rc) return rc;
   return zk::dispatch(zk::Op::Extrap, d, U, nullptr, (hipStream_t)stream);
 }
 
 int zfr_extrapolate_partner(const zfr_ele_desc_t *d, const double *U, void *stream) {
+  int rc = need_geometry(d);
+  if (!rc) rc = need(d->Utr, "Utr");
+  if (rc) return rc;
   if (!d->viscous || (d->ldg_b != 0.5 && d->ldg_b != -0.5))
     return zfr::fail("zfr_extrapolate_partner: the descriptor is not on the one-sided LDG path (beta = +-1/2)");
   return zk::dispatch(zk::Op::ExtrapPartner, d, U, nullptr, (hipStream_t)stream);
 }
 
 int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void *stream) {
+  int rc = need_geometry(d);
+  if (!rc) rc = need(d->Utr, "Utr");
+  if (!rc) rc = need(d->Gtr, "Gtr");
+  if (rc) return rc;
   if (!d->viscous) return zfr::fail("zfr_gradient: descriptor is inviscid");
   return zk::dispatch(zk::Op::Grad, d, U, dU_out, (hipStream_t)stream);
 }
 
 int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
+  int rc = need_geometry(d);
+  if (!rc && d->viscous) rc = need(d->Utr, "Utr");
+  if (!rc && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5) rc = need(d->Gtr, "Gtr");
+  if (rc) return rc;
   return zk::dispatch(zk::Op::Div, d, U, divF, (hipStream_t)stream);
 }
 
 int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
+  int rc = need_geometry(d);
+  if (!rc) rc = need(d->Utr, "Utr");
+  if (!rc) rc = need(d->Ftr, "Ftr");
+  if (rc) return rc;
   if (!d->viscous) return zfr::fail("zfr_gradient_flux: descriptor is inviscid");
   return zk::dispatch(zk::Op::OneSided, d, U, divF, (hipStream_t)stream);
 }
 
 int zfr_gradient_flux_traced(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
+  int rc = need_geometry(d);
+  if (!rc) rc = need(d->Utr, "Utr");
+  if (!rc) rc = need(d->Ftr, "Ftr");
+  if (rc) return rc;
   if (!d->viscous) return zfr::fail("zfr_gradient_flux_traced: descriptor is inviscid");
   return zk::dispatch(zk::Op::OneSidedFromTrace, d, U, divF, (hipStream_t)stream);
 }
 
 int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream) {
+  int rc = need_geometry(d);
+  if (!rc) rc = need(d->Ftr, "Ftr");
+  if (rc) return rc;
   if (!d->viscous) return zfr::fail("zfr_face_correction: descriptor is inviscid");
   return zk::dispatch(zk::Op::FaceCorr, d, nullptr, divF, (hipStream_t)stream);
 }
@@ -1528,6 +1569,11 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
              hipStream_t st) {
   if (!r || !r->U || !r->U_ini || (r->mode == 2 && r->nStages > 1 && !r->divF0))
     return zfr::fail("RK fusion: missing buffer");
+  int rc = need_geometry(d);
+  if (!rc && op == zk::Op::Div) rc = need(d->Utr, "Utr");  // the fused divergence reads the trace
+  if (!rc && op == zk::Op::Div && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5) rc = need(d->Gtr, "Gtr");
+  if (!rc && op == zk::Op::FaceCorr) rc = need(d->Ftr, "Ftr");
+  if (rc) return rc;
   if (r->mode != 1 && r->mode != 2) return zfr::fail("RK fusion: mode must be 1 (stage) or 2 (final)");
   if (r->nStages < 1 || r->nStages > 4 || r->stage < 0 || r->stage >= r->nStages)
     return zfr::fail("RK fusion: stage out of range");
@@ -1658,6 +1704,7 @@ int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type,
 
 int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, double *v, double *Wtr,
                      double *Xtr, void *stream) {
+  if (int rc = need_geometry(d)) return rc;
   if (!d->Utr || !Wtr || !Xtr || !w || !v) return zfr::fail("zfr_residual_vjp: missing buffer");
   zk::AdjPtrs j;
   j.w = w;
