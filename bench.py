@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--no-pmg", action="store_true", help="skip the p-multigrid timing")
     p.add_argument("--no-roe", action="store_true", help="skip the Roe+LDG variant timing")
     p.add_argument("--no-batch", action="store_true", help="skip the batched small-case timing")
+    p.add_argument("--no-c2", action="store_true", help="skip the 2D Navier-Stokes 256^2 p=4 timing")
     p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
     p.add_argument("--cpu-steps", type=int, default=150)
     p.add_argument("--workdir", default=None)
@@ -226,6 +227,57 @@ output_prefix vortex
     res["workload"] = (f"2D Euler isentropic vortex, 32^2 quads on [-5,5]^2, p=3 "
                        f"({d['nEles'] * d['nSpts'] * d['nVars']} DoF), RK44 steps")
     return res
+
+
+def c2_timing(workdir, steps, n=256, order=4):
+    """The 2D Navier-Stokes configuration: 256^2 periodic quads on [0, 2 pi]^2, p = 4, the
+    2D Taylor-Green field u = sin x cos y, v = -cos x sin y (Ma 0.1, Re 1600, Pr 0.72),
+    Rusanov + LDG (beta 0.5, tau 0.1).  Residual throughput after autotuning, wall time of
+    `steps` back-to-back residuals, and the two kernels' average durations (HIP events)."""
+    import numpy as np
+    from zefr_amd import zefr
+    from zefr_amd._lib import lib
+    from zefr_amd.mesh import write_box
+    mesh = write_box(os.path.join(workdir, f"c2_quad{n}.msh"), (n, n), (0.0, 0.0), (2 * math.pi, 2 * math.pi))
+    deck = os.path.join(workdir, "c2.deck")
+    with open(deck, "w") as f:
+        f.write(f"nDims 2\nmeshfile {mesh}\nmesh_bound periodic periodic\nmesh_bound fluid fluid\n"
+                f"order {order}\nequation EulerNS\nviscous 1\nn_steps 1\ndt_scheme RK44\ndt 1e-3\n"
+                f"dt_type 0\nic_type 0\nfconv_type Rusanov\nfvisc_type LDG\nrus_k 0.0\nldg_b 0.5\n"
+                f"ldg_tau 0.1\ngamma 1.4\nprandtl 0.72\ndisable_nondim 1\nfix_vis 1\nmach_fs 0.1\n"
+                f"Re_fs 1600\nP_fs 1.0\nrho_fs 1.0\nL_fs 1.0\nR 286.9\noutput_prefix c2\n")
+    z = zefr.Zefr()
+    z.read_input(deck)
+    z.setup_solver()
+    d = z.dims()
+    X = z.host_array("coord_spts").reshape(d["nSpts"], 2, d["nEles"])  # reference layout
+    x, y = X[:, 0, :], X[:, 1, :]
+    g, P0, rho0 = 1.4, 1.0, 1.0
+    U0 = 0.1 * math.sqrt(g * P0 / rho0)
+    u = U0 * np.sin(x) * np.cos(y)
+    v = -U0 * np.cos(x) * np.sin(y)
+    p = P0 + rho0 * U0 * U0 / 4.0 * (np.cos(2 * x) + np.cos(2 * y))
+    rho = rho0 * p / P0
+    z.set_state(np.stack([rho, rho * u, rho * v, p / (g - 1.0) + 0.5 * rho * (u * u + v * v)], axis=1))
+    z.autotune(3)
+    dof = d["nEles"] * d["nSpts"] * d["nVars"]
+    for _ in range(3):
+        z.compute_residual(0)
+    z.synchronize()
+    ms3 = (C.c_double * 3)()
+    t = time.perf_counter()
+    rc = lib().zefr_time_residual_kernels(z._h, steps, ms3)
+    z.synchronize()
+    wall = time.perf_counter() - t
+    if rc != 0:
+        raise RuntimeError(lib().zefr_last_error().decode())
+    tiles = z.tiles()
+    del z
+    return {"workload": f"2D Navier-Stokes Taylor-Green residual, {n}^2 quads on [0,2pi]^2, p={order}, "
+                        "Rusanov+LDG(beta=0.5,tau=0.1)", "dof": dof,
+            "value": round(dof / (wall / steps) / 1e9, 4), "unit": "Gdof-residuals/s",
+            "ms_per_step": round(wall / steps * 1e3, 4), "tiles_eles_per_block": tiles,
+            "kernels_ms": {"gradient_flux": round(ms3[1], 4), "face_correction": round(ms3[2], 4)}}
 
 
 def pmg_timing(workdir, cycles=10):
@@ -571,6 +623,7 @@ def main():
     rk = None
     if not args.no_steps:
         rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
+    c2 = None if args.no_c2 else c2_timing(workdir, args.steps)
     pmg = None if args.no_pmg else pmg_timing(workdir)
     roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps, not args.no_vjp)
     batch = None if args.no_batch else batch_timing(workdir)
@@ -604,6 +657,7 @@ def main():
         "kernels": per_kernel,
         "vjp": vjp,
         "rk4_step": rk,
+        "c2_quad256_p4": c2,
         "pmg": pmg,
         "roe": roe,
         "batch": batch,
