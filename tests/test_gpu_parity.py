@@ -89,6 +89,30 @@ def test_rk4_steps_multi_element_tiles(order, viscous, tiles):
     assert err <= RK_TOL, f"{case['name']}: relative L-inf after {nsteps} steps {err:.3e}"
 
 
+@pytest.mark.parametrize("case", [CASES[2], CASES[0], CASES[3]], ids=["onesided_hex_p3", "euler_quad_p3",
+                                                                      "general_beta_hex_p2"])
+def test_autotuned_tiles_keep_parity(case):
+    """The autotuner picks one workgroup tile per launch from timings; the residual and two
+    RK4 steps at the tiles it picks still match the oracle.  Paths without a trace launch
+    (one-sided LDG, Euler) leave that launch's tile unset, so it reports the solver-wide
+    tile (the one-sided / divergence launch's)."""
+    z, o, _ = paired(case)
+    z.autotune(2)
+    tiles = z.tiles()  # effective tiles: an unset one falls back to the solver-wide tile
+    has_trace_launch = case["viscous"] and case.get("extra", {}).get("ldg_b", 0.5) not in (0.5, -0.5)
+    assert min(tiles) > 0, tiles
+    if not has_trace_launch:
+        assert tiles[0] == tiles[1], tiles
+    z.compute_residual(0)
+    o.compute_residual(0)
+    err, kappa, ok = residual_parity(z.state("divF", 0), o)
+    assert ok, f"{case['name']} at tiles {tiles}: relative L-inf {err:.3e} (kappa {kappa:.1e})"
+    for _ in range(2):
+        z.do_step()
+    o.step_rk(2)
+    assert rel_err(z.state("U"), o.U()) <= RK_TOL
+
+
 def test_vortex_100_rk4_steps():
     """The CPU-runnable benchmark case: 2D Euler vortex, 32x32 quads on [-5,5]^2, p=3,
     Rusanov, RK4 dt=1e-3, 100 steps."""
