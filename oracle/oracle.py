@@ -18,7 +18,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libzefr_oracle.so")
+# arithmetic type of the oracle build -> its library (oracle/Makefile builds both)
+_LIB_PATHS = {"double": os.path.join(_HERE, "libzefr_oracle.so"),
+              "long double": os.path.join(_HERE, "libzefr_oracle_ld.so")}
+_LIB_PATH = _LIB_PATHS["double"]
 
 
 class ZoPhys(C.Structure):
@@ -34,15 +37,18 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(precision="double"):
+    """The oracle library: "double" (the FP64 restatement, the checker) or "long double"
+    (the same code on x87 extended precision, the exact answer FP64 results are measured
+    against)."""
+    if precision not in _libs:
+        path = _LIB_PATHS[precision]
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         L.zo_last_error.restype = C.c_char_p
         L.zo_create.restype = C.c_void_p
         L.zo_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(ZoPhys)]
@@ -73,17 +79,17 @@ def lib():
         L.zo_pmg_array.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_long, C.c_int]
         L.zo_pmg_operator.restype = C.c_long
         L.zo_pmg_operator.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_long]
-        _lib = L
-    return _lib
+        _libs[precision] = L
+    return _libs[precision]
 
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def _check(rc):
+def _check(rc, L=None):
     if rc != 0:
-        raise RuntimeError(lib().zo_last_error().decode())
+        raise RuntimeError((L or lib()).zo_last_error().decode())
 
 
 def make_phys(viscous=False, gamma=1.4, prandtl=0.72, mu=0.0, rt=0.0, c_sth=0.0, fix_vis=True,
@@ -96,33 +102,35 @@ def make_phys(viscous=False, gamma=1.4, prandtl=0.72, mu=0.0, rt=0.0, c_sth=0.0,
 class Oracle:
     """One oracle solver instance on a Gmsh file (single element type, periodic)."""
 
-    def __init__(self, meshfile, nDims, order, phys: ZoPhys):
-        self._h = lib().zo_create(meshfile.encode(), nDims, order, C.byref(phys))
+    def __init__(self, meshfile, nDims, order, phys: ZoPhys, precision="double"):
+        self.precision = precision
+        self._L = lib(precision)
+        self._h = self._L.zo_create(meshfile.encode(), nDims, order, C.byref(phys))
         if not self._h:
-            raise RuntimeError(lib().zo_last_error().decode())
+            raise RuntimeError(self._L.zo_last_error().decode())
         d = (C.c_int * 10)()
-        lib().zo_dims(self._h, d)
+        self._L.zo_dims(self._h, d)
         (self.nEles, self.nSpts, self.nFpts, self.nVars, self.nDims, self.nGfpts,
          self.nGfpts_int, self.nStages, self.nFaces, self.nIntFaces) = list(d)
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().zo_destroy(h)
+            self._L.zo_destroy(h)
             self._h = None
 
     def maps(self):
         g = np.zeros((self.nFpts, self.nEles), np.int32)
         s = np.zeros((self.nFpts, self.nEles), np.int8)
-        lib().zo_get_maps(self._h, g.ctypes.data_as(C.POINTER(C.c_int)),
+        self._L.zo_get_maps(self._h, g.ctypes.data_as(C.POINTER(C.c_int)),
                           s.ctypes.data_as(C.POINTER(C.c_byte)))
         return g, s
 
     def array(self, name, shape=None):
         buf = np.zeros(self._size_hint(name), np.float64)
-        cnt = lib().zo_get_array(self._h, name.encode(), _dp(buf), buf.size)
+        cnt = self._L.zo_get_array(self._h, name.encode(), _dp(buf), buf.size)
         if cnt < 0:
-            raise RuntimeError(lib().zo_last_error().decode())
+            raise RuntimeError(self._L.zo_last_error().decode())
         out = buf[:cnt].copy()
         return out.reshape(shape) if shape is not None else out
 
@@ -147,24 +155,24 @@ class Oracle:
     def set_U(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
         assert U.shape == (self.nSpts, self.nVars, self.nEles)
-        lib().zo_set_U(self._h, _dp(U))
+        self._L.zo_set_U(self._h, _dp(U))
 
     def U(self):
         return self.array("U_spts", (self.nSpts, self.nVars, self.nEles))
 
     def compute_residual(self, stage=0):
-        _check(lib().zo_compute_residual(self._h, stage))
+        _check(self._L.zo_compute_residual(self._h, stage), self._L)
 
     def divF(self, stage=0):
         d = self.array("divF", (self.nStages, self.nSpts, self.nVars, self.nEles))
         return d[stage]
 
     def step_rk(self, nsteps=1):
-        _check(lib().zo_step_rk(self._h, nsteps))
+        _check(self._L.zo_step_rk(self._h, nsteps), self._L)
 
     def report_residuals(self):
         r = np.zeros(self.nVars)
-        _check(lib().zo_report_residuals(self._h, _dp(r)))
+        _check(self._L.zo_report_residuals(self._h, _dp(r)), self._L)
         return r
 
 
@@ -176,22 +184,23 @@ class OraclePMG:
 
     def __init__(self, fine: Oracle, meshfile, levels, steps, rel_fac=1.0, cycle="V", fmg_vcycles=1):
         self.fine = fine  # keeps the fine solver alive as long as the cycle object
+        self._L = fine._L
         self.levels = list(levels)
         n = len(self.levels)
         lv = (C.c_int * n)(*self.levels)
         st = (C.c_int * n)(*steps)
-        self._h = lib().zo_pmg_create(fine._h, meshfile.encode(), n, lv, st, rel_fac, cycle.encode(), fmg_vcycles)
+        self._h = self._L.zo_pmg_create(fine._h, meshfile.encode(), n, lv, st, rel_fac, cycle.encode(), fmg_vcycles)
         if not self._h:
-            raise RuntimeError(lib().zo_last_error().decode())
+            raise RuntimeError(self._L.zo_last_error().decode())
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().zo_pmg_destroy(h)
+            self._L.zo_pmg_destroy(h)
             self._h = None
 
     def cycle(self, n=1):
-        _check(lib().zo_pmg_cycle(self._h, n))
+        _check(self._L.zo_pmg_cycle(self._h, n), self._L)
 
     def _shape(self, level):
         f = self.fine
@@ -199,25 +208,25 @@ class OraclePMG:
 
     def array(self, level, which="U"):
         a = np.zeros(self._shape(level))
-        if lib().zo_pmg_array(self._h, level, self._WHICH[which], _dp(a), a.size, 0) < 0:
-            raise RuntimeError(lib().zo_last_error().decode())
+        if self._L.zo_pmg_array(self._h, level, self._WHICH[which], _dp(a), a.size, 0) < 0:
+            raise RuntimeError(self._L.zo_last_error().decode())
         return a
 
     def set_U(self, level, U):
         a = np.ascontiguousarray(U, np.float64)
         assert a.shape == self._shape(level)
-        if lib().zo_pmg_array(self._h, level, 0, _dp(a), a.size, 1) < 0:
-            raise RuntimeError(lib().zo_last_error().decode())
+        if self._L.zo_pmg_array(self._h, level, 0, _dp(a), a.size, 1) < 0:
+            raise RuntimeError(self._L.zo_last_error().decode())
 
     def operator(self, level, kind):
         """Dense operator of a level: kind 'res' (to level+1) or 'pro' (to level-1),
         [nSpts_to][nSpts_from]."""
         k = {"res": 0, "pro": 1}[kind]
-        cnt = lib().zo_pmg_operator(self._h, level, k, None, 0)
+        cnt = self._L.zo_pmg_operator(self._h, level, k, None, 0)
         if cnt < 0:
-            raise RuntimeError(lib().zo_last_error().decode())
+            raise RuntimeError(self._L.zo_last_error().decode())
         a = np.zeros(cnt)
-        lib().zo_pmg_operator(self._h, level, k, _dp(a), cnt)
+        self._L.zo_pmg_operator(self._h, level, k, _dp(a), cnt)
         D = self.fine.nDims
         to = self.levels[level + 1] if k == 0 else self.levels[level - 1]
         return a.reshape((to + 1) ** D, (self.levels[level] + 1) ** D)

@@ -31,13 +31,15 @@
 // GiMMiK hex p=3/p=4 constants and polynomial exactness; Gauss-Legendre rules against
 // numpy; metrics against affine closed forms and cell volumes; connectivity by brute
 // force (each global flux point has one slot-0 and one slot-1 point at the same
-// physical location); fluxes against textbook Euler/NS from analytic gradients;
+// physical location); the slot of every element flux point against the first-incidence
+// rule derived from the mesh file by an independent reader (periodic faces included; the
+// swapped rule is rejected); fluxes against textbook Euler/NS from analytic gradients;
 // residuals by freestream preservation and discrete conservation; RK44 and the norms
-// by their textbook definitions.
-// Parity unpinned: the exact global flux-point index permutation and which side of a
-// face is slot 0 (setup_global_fpts + orient_fpts).  No reference-held output records
-// them; they are restated from the cited code, and the brute-force checks above only
-// prove that the numbering is a valid pairing.
+// by their textbook definitions.  Rounding: the same source built with
+// -DORACLE_REAL="long double" is the exact answer the FP64 results are measured against.
+// Parity unpinned: the exact global flux-point index permutation (setup_global_fpts +
+// orient_fpts).  No reference-held output records it; it is restated from the cited
+// code and checked only for being a valid pairing.  Residuals do not depend on it.
 //
 // gimmik_mm_cpu: its generated source (src/gimmik_cpu.c) is absent from the reference.
 // GiMMiK bakes the nonzero entries of the constant operator into an unrolled dot product,
@@ -57,6 +59,15 @@
 #include <string>
 #include <vector>
 
+// Arithmetic type of the oracle: double for the FP64 restatement; the extended-precision
+// build (libzefr_oracle_ld.so, -DORACLE_REAL="long double") evaluates the same algorithm
+// on the same double inputs with a 64-bit significand, as the exact answer that FP64
+// evaluations are measured against.  The C interface below stays in double either way.
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+using real = ORACLE_REAL;
+
 namespace {
 
 enum { LINE_T = 0, QUAD_T = 2, HEX_T = 4 };
@@ -64,9 +75,9 @@ enum { BC_PERIODIC = 1 };
 
 struct Phys {
   int viscous;
-  double gamma, prandtl, mu, rt, c_sth;
+  real gamma, prandtl, mu, rt, c_sth;
   int fix_vis;
-  double rus_k, ldg_b, ldg_tau, dt;
+  real rus_k, ldg_b, ldg_tau, dt;
   int nStages;
   int res_type;
   int fconv;  // 0 Rusanov (the reference's), 2 Roe (textbook, below)
@@ -76,7 +87,7 @@ struct Phys {
 
 // ---------------------------------------------------------------- points / polynomials
 // src/points.cpp:9-98 (closed forms up to 5 points, 15-digit tables beyond).
-std::vector<double> gl_points(unsigned n) {
+std::vector<double> gl_points_d(unsigned n) {
   switch (n) {
     case 1: return {0.0};
     case 2: return {-std::sqrt(1. / 3.), std::sqrt(1. / 3.)};
@@ -99,7 +110,7 @@ std::vector<double> gl_points(unsigned n) {
 }
 
 // src/points.cpp:123-209
-std::vector<double> gl_weights(unsigned n) {
+std::vector<double> gl_weights_d(unsigned n) {
   switch (n) {
     case 1: return {2.0};
     case 2: return {1.0, 1.0};
@@ -119,23 +130,35 @@ std::vector<double> gl_weights(unsigned n) {
   }
 }
 
+// The same rules as the arithmetic type of this build: the double values widened, so
+// that an extended-precision build evaluates the same discretisation as the FP64 one.
+std::vector<real> gl_points(unsigned n) {
+  const std::vector<double> g = gl_points_d(n);
+  return std::vector<real>(g.begin(), g.end());
+}
+
+std::vector<real> gl_weights(unsigned n) {
+  const std::vector<double> g = gl_weights_d(n);
+  return std::vector<real>(g.begin(), g.end());
+}
+
 // include/polynomials.hpp:15-28 -- product form, factors taken in increasing grid order.
-double lagrange(const std::vector<double> &g, unsigned mode, double x) {
-  double v = 1.0;
+real lagrange(const std::vector<real> &g, unsigned mode, real x) {
+  real v = 1.0;
   for (unsigned i = 0; i < g.size(); i++)
     if (i != mode) v *= (x - g[i]) / (g[mode] - g[i]);
   return v;
 }
 
 // include/polynomials.hpp:47-76 -- sum over j != mode of prod_{i != mode, j} (x - g_i), / denominator.
-double lagrange_d1(const std::vector<double> &g, unsigned mode, double x) {
-  double den = 1.0;
+real lagrange_d1(const std::vector<real> &g, unsigned mode, real x) {
+  real den = 1.0;
   for (unsigned i = 0; i < g.size(); i++)
     if (i != mode) den *= (g[mode] - g[i]);
-  double val = 0.0;
+  real val = 0.0;
   for (unsigned j = 0; j < g.size(); j++) {
     if (j == mode) continue;
-    double term = 1.0;
+    real term = 1.0;
     for (unsigned i = 0; i < g.size(); i++)
       if (i != mode && i != j) term *= (x - g[i]);
     val += term;
@@ -147,8 +170,8 @@ double lagrange_d1(const std::vector<double> &g, unsigned mode, double x) {
 // include/funcs.hpp:254-276: sort indices by coordinate 'dim'; runs whose values stay
 // within tol of the run's first entry are re-sorted by the next coordinate.
 // mat is [nDims][ncols] row-major (mat(dim, col)).
-void fuzzysort_ind(const std::vector<double> &mat, unsigned ncols, unsigned *inds, unsigned n,
-                   unsigned nDims, unsigned dim = 0, double tol = 1e-6) {
+void fuzzysort_ind(const std::vector<real> &mat, unsigned ncols, unsigned *inds, unsigned n,
+                   unsigned nDims, unsigned dim = 0, real tol = 1e-6) {
   auto val = [&](unsigned c) { return mat[(size_t)dim * ncols + c]; };
   std::sort(inds, inds + n, [&](unsigned a, unsigned b) { return val(a) < val(b); });
   unsigned i = 0, j;
@@ -166,19 +189,19 @@ void fuzzysort_ind(const std::vector<double> &mat, unsigned ncols, unsigned *ind
 
 // ---------------------------------------------------------------- operator product
 // C[M][N] = alpha * A[M][K] * B[K][N] + beta * C, row-major; nonzeros of A only (see header).
-void op_mm(int M, int N, int K, double alpha, const double *A, const double *B, double beta, double *C) {
+void op_mm(int M, int N, int K, real alpha, const real *A, const real *B, real beta, real *C) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < M; i++) {
-    double *c = C + (size_t)i * N;
+    real *c = C + (size_t)i * N;
     if (beta == 0.0)
       for (int j = 0; j < N; j++) c[j] = 0.0;
     else if (beta != 1.0)
       for (int j = 0; j < N; j++) c[j] *= beta;
     for (int k = 0; k < K; k++) {
-      double a = A[(size_t)i * K + k];
+      real a = A[(size_t)i * K + k];
       if (a == 0.0) continue;
       a *= alpha;
-      const double *b = B + (size_t)k * N;
+      const real *b = B + (size_t)k * N;
       for (int j = 0; j < N; j++) c[j] += a * b[j];
     }
   }
@@ -187,12 +210,12 @@ void op_mm(int M, int N, int K, double alpha, const double *A, const double *B, 
 // ---------------------------------------------------------------- pointwise fluxes
 // include/flux.hpp:11-63 with zero grid velocity.
 template <int V, int D>
-void fconv(const double *U, double F[V][D], double &P, double gamma) {
+void fconv(const real *U, real F[V][D], real &P, real gamma) {
   if (D == 2) {
-    double invrho = 1. / U[0];
-    double momF = (U[1] * U[1] + U[2] * U[2]) * invrho;
+    real invrho = 1. / U[0];
+    real momF = (U[1] * U[1] + U[2] * U[2]) * invrho;
     P = (gamma - 1.0) * (U[3] - 0.5 * momF);
-    double H = (U[3] + P) * invrho;
+    real H = (U[3] + P) * invrho;
     F[0][0] = U[1];
     F[1][0] = U[1] * U[1] * invrho + P;
     F[2][0] = U[1] * U[2] * invrho;
@@ -202,10 +225,10 @@ void fconv(const double *U, double F[V][D], double &P, double gamma) {
     F[2][1] = U[2] * U[2] * invrho + P;
     F[3][1] = U[2] * H;
   } else {
-    double invrho = 1. / U[0];
-    double momF = (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * invrho;
+    real invrho = 1. / U[0];
+    real momF = (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * invrho;
     P = (gamma - 1.0) * (U[4] - 0.5 * momF);
-    double H = (U[4] + P) * invrho;
+    real H = (U[4] + P) * invrho;
     for (int d = 0; d < 3; d++) {
       F[0][d] = U[d + 1];
       for (int m = 0; m < 3; m++) F[m + 1][d] = U[d + 1] * U[m + 1] * invrho;
@@ -220,34 +243,34 @@ void fconv(const double *U, double F[V][D], double &P, double gamma) {
 // reference code; tests/test_roe.py pins it against numpy's eigen-decomposition of the
 // flux Jacobian at the Roe-averaged state.  Fn = F.n for the unit normal n.
 template <int V, int D>
-void roe_flux(const double *UL, const double *UR, const double *n, double gamma, double *Fn) {
-  double FL[V][D], FR[V][D], PL, PR;
+void roe_flux(const real *UL, const real *UR, const real *n, real gamma, real *Fn) {
+  real FL[V][D], FR[V][D], PL, PR;
   fconv<V, D>(UL, FL, PL, gamma);
   fconv<V, D>(UR, FR, PR, gamma);
-  double uL[D], uR[D];
+  real uL[D], uR[D];
   for (int d = 0; d < D; d++) { uL[d] = UL[d + 1] / UL[0]; uR[d] = UR[d + 1] / UR[0]; }
-  const double HL = (UL[V - 1] + PL) / UL[0], HR = (UR[V - 1] + PR) / UR[0];
-  const double rl = std::sqrt(UL[0]), rr = std::sqrt(UR[0]);
-  double ut[D], unt = 0, q2 = 0, unL = 0, unR = 0;
+  const real HL = (UL[V - 1] + PL) / UL[0], HR = (UR[V - 1] + PR) / UR[0];
+  const real rl = std::sqrt(UL[0]), rr = std::sqrt(UR[0]);
+  real ut[D], unt = 0, q2 = 0, unL = 0, unR = 0;
   for (int d = 0; d < D; d++) {
     ut[d] = (rl * uL[d] + rr * uR[d]) / (rl + rr);
     unt += ut[d] * n[d]; q2 += ut[d] * ut[d];
     unL += uL[d] * n[d]; unR += uR[d] * n[d];
   }
-  const double Ht = (rl * HL + rr * HR) / (rl + rr);
-  const double rhot = rl * rr;
-  const double at = std::sqrt((gamma - 1.0) * (Ht - 0.5 * q2));
-  const double delta = 0.1 * at;
-  auto fix = [&](double lam) {
-    double l = std::abs(lam);
+  const real Ht = (rl * HL + rr * HR) / (rl + rr);
+  const real rhot = rl * rr;
+  const real at = std::sqrt((gamma - 1.0) * (Ht - 0.5 * q2));
+  const real delta = 0.1 * at;
+  auto fix = [&](real lam) {
+    real l = std::abs(lam);
     return (l < delta) ? (lam * lam + delta * delta) / (2.0 * delta) : l;
   };
-  const double lam[3] = {fix(unt - at), std::abs(unt), fix(unt + at)};
-  const double dP = PR - PL, dUn = unR - unL, dRho = UR[0] - UL[0];
+  const real lam[3] = {fix(unt - at), std::abs(unt), fix(unt + at)};
+  const real dP = PR - PL, dUn = unR - unL, dRho = UR[0] - UL[0];
   // wave strengths and right eigenvectors: acoustic (-), entropy, acoustic (+), shear
-  const double alpha[3] = {(dP - rhot * at * dUn) / (2.0 * at * at), dRho - dP / (at * at),
+  const real alpha[3] = {(dP - rhot * at * dUn) / (2.0 * at * at), dRho - dP / (at * at),
                            (dP + rhot * at * dUn) / (2.0 * at * at)};
-  double r[3][V];
+  real r[3][V];
   r[0][0] = 1.0; r[1][0] = 1.0; r[2][0] = 1.0;
   for (int d = 0; d < D; d++) {
     r[0][d + 1] = ut[d] - at * n[d];
@@ -255,15 +278,15 @@ void roe_flux(const double *UL, const double *UR, const double *n, double gamma,
     r[2][d + 1] = ut[d] + at * n[d];
   }
   r[0][V - 1] = Ht - at * unt; r[1][V - 1] = 0.5 * q2; r[2][V - 1] = Ht + at * unt;
-  double diss[V] = {0};
+  real diss[V] = {0};
   for (int k = 0; k < 3; k++)
     for (int m = 0; m < V; m++) diss[m] += lam[k] * alpha[k] * r[k][m];
-  double sh[D], ush = 0;
+  real sh[D], ush = 0;
   for (int d = 0; d < D; d++) { sh[d] = (uR[d] - uL[d]) - dUn * n[d]; ush += ut[d] * sh[d]; }
   for (int d = 0; d < D; d++) diss[d + 1] += lam[1] * rhot * sh[d];
   diss[V - 1] += lam[1] * rhot * ush;
   for (int m = 0; m < V; m++) {
-    double fl = 0, fr = 0;
+    real fl = 0, fr = 0;
     for (int d = 0; d < D; d++) { fl += FL[m][d] * n[d]; fr += FR[m][d] * n[d]; }
     Fn[m] = 0.5 * (fl + fr) - 0.5 * diss[m];
   }
@@ -271,38 +294,38 @@ void roe_flux(const double *UL, const double *UR, const double *n, double gamma,
 
 // include/flux.hpp:75-202: subtract the viscous flux.
 template <int V, int D>
-void fvisc_add(const double *U, const double dU[V][D], double F[V][D], const Phys &ph) {
-  double invrho = 1.0 / U[0];
-  double vel[3] = {0, 0, 0};
-  double ke2 = 0.0;
+void fvisc_add(const real *U, const real dU[V][D], real F[V][D], const Phys &ph) {
+  real invrho = 1.0 / U[0];
+  real vel[3] = {0, 0, 0};
+  real ke2 = 0.0;
   for (int d = 0; d < D; d++) { vel[d] = U[d + 1] * invrho; ke2 += vel[d] * vel[d]; }
-  double e_int = U[D + 1] * invrho - 0.5 * ke2;
-  double mu;
+  real e_int = U[D + 1] * invrho - 0.5 * ke2;
+  real mu;
   if (ph.fix_vis) mu = ph.mu;
   else {
-    double rt_ratio = (ph.gamma - 1.0) * e_int / ph.rt;
+    real rt_ratio = (ph.gamma - 1.0) * e_int / ph.rt;
     mu = ph.mu * std::pow(rt_ratio, 1.5) * (1. + ph.c_sth) / (rt_ratio + ph.c_sth);
   }
   // dvel[i][j] = d(vel_i)/dx_j
-  double dvel[3][3] = {{0}};
+  real dvel[3][3] = {{0}};
   for (int i = 0; i < D; i++)
     for (int j = 0; j < D; j++) dvel[i][j] = (dU[i + 1][j] - dU[0][j] * vel[i]) * invrho;
-  double de[3];
+  real de[3];
   for (int j = 0; j < D; j++) {
-    double s = 0.0;
+    real s = 0.0;
     for (int i = 0; i < D; i++) s += vel[i] * dvel[i][j];
-    double dke = 0.5 * ke2 * dU[0][j] + U[0] * s;
+    real dke = 0.5 * ke2 * dU[0][j] + U[0] * s;
     de[j] = (dU[D + 1][j] - dke - dU[0][j] * e_int) * invrho;
   }
-  double div = 0.0;
+  real div = 0.0;
   for (int i = 0; i < D; i++) div += dvel[i][i];
-  double diag = div / 3.0;
-  double tau[3][3];
+  real diag = div / 3.0;
+  real tau[3][3];
   for (int i = 0; i < D; i++)
     for (int j = 0; j < D; j++)
       tau[i][j] = (i == j) ? 2.0 * mu * (dvel[i][i] - diag) : mu * (dvel[i][j] + dvel[j][i]);
   for (int j = 0; j < D; j++) {
-    double w = 0.0;
+    real w = 0.0;
     for (int i = 0; i < D; i++) { F[i + 1][j] -= tau[i][j]; w += vel[i] * tau[i][j]; }
     F[D + 1][j] -= (w + (mu / ph.prandtl) * ph.gamma * de[j]);
   }
@@ -314,7 +337,7 @@ struct Oracle {
   int nDims = 0, order = 0, nVars = 0;
   // mesh
   int nNodes = 0;
-  std::vector<double> coord_nodes;  // [nNodes][nDims]
+  std::vector<real> coord_nodes;  // [nNodes][nDims]
   int etype = 0, nEles = 0, nNodesPerEle = 0;
   std::vector<int> ele2nodes;  // [nEles][nNodesPerEle]
   std::map<std::vector<unsigned>, int> bnd_faces;
@@ -328,21 +351,21 @@ struct Oracle {
   std::vector<int> gf_fpt, gf_ele;    // [2][nGfpts] face views
   // element local data
   unsigned nSpts1D = 0, nSpts = 0, nFpts = 0, nFaceEle = 0;
-  std::vector<double> loc1d, locDFR, wts1d;
-  std::vector<double> loc_spts, loc_fpts, tnorm;  // [n][nDims]
+  std::vector<real> loc1d, locDFR, wts1d;
+  std::vector<real> loc_spts, loc_fpts, tnorm;  // [n][nDims]
   std::vector<int> idx_spts, idx_fpts;            // [n][nDims]
-  std::vector<double> weights_spts;
-  std::vector<double> shape_spts, shape_fpts, dshape_spts, dshape_fpts;
-  std::vector<double> nodes;  // [nNodesPerEle][nDims][nEles]
-  std::vector<double> coord_spts, coord_fpts;  // [n][nDims][nEles]
-  std::vector<double> jaco_spts, jaco_fpts, inv_jaco_spts, inv_jaco_fpts;  // [D][n][D][E]
-  std::vector<double> jaco_det_spts;  // [nSpts][nEles]
-  std::vector<double> norm, dA;       // [D][nGfpts], [2][nGfpts]
-  std::vector<double> oppE, oppD, oppD_fpts, oppDiv, oppDiv_fpts;
+  std::vector<real> weights_spts;
+  std::vector<real> shape_spts, shape_fpts, dshape_spts, dshape_fpts;
+  std::vector<real> nodes;  // [nNodesPerEle][nDims][nEles]
+  std::vector<real> coord_spts, coord_fpts;  // [n][nDims][nEles]
+  std::vector<real> jaco_spts, jaco_fpts, inv_jaco_spts, inv_jaco_fpts;  // [D][n][D][E]
+  std::vector<real> jaco_det_spts;  // [nSpts][nEles]
+  std::vector<real> norm, dA;       // [D][nGfpts], [2][nGfpts]
+  std::vector<real> oppE, oppD, oppD_fpts, oppDiv, oppDiv_fpts;
   // solution data (reference layouts, nElesPad == nEles on the CPU path)
-  std::vector<double> U_spts, U_fpts, Ucomm, F_spts, Fcomm, dU_spts, dU_fpts, divF, U_ini;
-  double rk_alpha[3] = {0.5, 0.5, 1.0};
-  double rk_beta[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+  std::vector<real> U_spts, U_fpts, Ucomm, F_spts, Fcomm, dU_spts, dU_fpts, divF, U_ini;
+  real rk_alpha[3] = {0.5, 0.5, 1.0};
+  real rk_beta[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
 
   // ------------------------------------------------------------ Gmsh reader
   void read_mesh(const std::string &file) {
@@ -377,10 +400,10 @@ struct Oracle {
     f >> nNodes;
     coord_nodes.assign((size_t)nNodes * nDims, 0.0);
     for (int n = 0; n < nNodes; n++) {
-      unsigned id; double z;
-      f >> id;
-      if (nDims == 2) f >> coord_nodes[n * 2] >> coord_nodes[n * 2 + 1] >> z;
-      else f >> coord_nodes[n * 3] >> coord_nodes[n * 3 + 1] >> coord_nodes[n * 3 + 2];
+      unsigned id;
+      double x[3];  // parsed as double: every build sees the same node coordinates
+      f >> id >> x[0] >> x[1] >> x[2];
+      for (int d = 0; d < nDims; d++) coord_nodes[(size_t)n * nDims + d] = x[d];
     }
     // geometry.cpp:431-840 (element pass) and 842-950 (boundary pass)
     seek("$Elements");
@@ -419,7 +442,7 @@ struct Oracle {
       if (bf.second != BC_PERIODIC) continue;
       const auto &face1 = bf.first;
       unsigned nn = face1.size();
-      std::vector<double> c1(nDims, 0.0);
+      std::vector<real> c1(nDims, 0.0);
       for (int d = 0; d < nDims; d++) {
         for (unsigned k = 0; k < nn; k++) c1[d] += coord_nodes[face1[k] * nDims + d];
         c1[d] /= nn;
@@ -428,7 +451,7 @@ struct Oracle {
       for (auto &bf2 : bnd_faces) {
         const auto &face2 = bf2.first;
         if (face2 == face1 || face2.size() != nn || bf2.second != BC_PERIODIC) continue;
-        std::vector<double> c2(nDims, 0.0);
+        std::vector<real> c2(nDims, 0.0);
         for (int d = 0; d < nDims; d++) {
           for (unsigned k = 0; k < nn; k++) c2[d] += coord_nodes[face2[k] * nDims + d];
           c2[d] /= nn;
@@ -437,7 +460,7 @@ struct Oracle {
         for (int d = 0; d < nDims; d++) {
           if (std::abs(c1[d] - c2[d]) < 1.e-6) {
             bool onPlane = true;
-            double c0 = coord_nodes[face1[0] * nDims + d];
+            real c0 = coord_nodes[face1[0] * nDims + d];
             for (unsigned k = 1; k < nn; k++)
               if (std::abs(c0 - coord_nodes[face1[k] * nDims + d]) > 1.e-6) onPlane = false;
             if (!onPlane) count++;
@@ -553,7 +576,7 @@ struct Oracle {
       unsigned fp = 0;
       for (unsigned face = 0; face < 4; face++)  // quads.cpp:86-123
         for (unsigned j = 0; j < N; j++, fp++) {
-          double *l = &loc_fpts[fp * 2]; int *x = &idx_fpts[fp * 2];
+          real *l = &loc_fpts[fp * 2]; int *x = &idx_fpts[fp * 2];
           switch (face) {
             case 0: l[0] = loc1d[j]; l[1] = -1.0; x[0] = j; x[1] = -1; break;
             case 1: l[0] = 1.0; l[1] = loc1d[j]; x[0] = N; x[1] = j; break;
@@ -587,7 +610,7 @@ struct Oracle {
       for (unsigned face = 0; face < 6; face++)  // hexas.cpp:133-194
         for (unsigned j = 0; j < N; j++)
           for (unsigned k = 0; k < N; k++, fp++) {
-            double *l = &loc_fpts[fp * 3]; int *x = &idx_fpts[fp * 3];
+            real *l = &loc_fpts[fp * 3]; int *x = &idx_fpts[fp * 3];
             const int r = N - k - 1;
             switch (face) {
               case 0: l[0] = loc1d[k]; l[1] = loc1d[j]; l[2] = -1.0; x[0] = k; x[1] = j; x[2] = -1; break;
@@ -599,7 +622,7 @@ struct Oracle {
             }
           }
       tnorm.assign(nFpts * 3, 0.0);  // hexas.cpp:252-292
-      const double tn[6][3] = {{0, 0, -1}, {0, 0, 1}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}};
+      const real tn[6][3] = {{0, 0, -1}, {0, 0, 1}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}};
       for (unsigned f = 0; f < nFpts; f++)
         for (int d = 0; d < 3; d++) tnorm[f * 3 + d] = tn[f / (N * N)][d];
     }
@@ -607,10 +630,10 @@ struct Oracle {
 
   // Linear Lagrange shape functions on the 2-point grid {-1, 1}, node order from
   // structured_to_gmsh_{quad,hex} (funcs.cpp:1007-1091): ijk -> gmsh = {0,1,3,2[,4,5,7,6]}.
-  void shape_at(const double *loc, double *val, double *dval /* [nNodes][nDims] */) const {
-    const std::vector<double> xl = {-1.0, 1.0};
+  void shape_at(const real *loc, real *val, real *dval /* [nNodes][nDims] */) const {
+    const std::vector<real> xl = {-1.0, 1.0};
     const int ijk2gmsh[8] = {0, 1, 3, 2, 4, 5, 7, 6};
-    double lag[3][2], dlag[3][2];
+    real lag[3][2], dlag[3][2];
     for (int d = 0; d < nDims; d++)
       for (int i = 0; i < 2; i++) { lag[d][i] = lagrange(xl, i, loc[d]); dlag[d][i] = lagrange_d1(xl, i, loc[d]); }
     int pt = 0;
@@ -640,7 +663,7 @@ struct Oracle {
     const int nN = nNodesPerEle, D = nDims;
     shape_spts.assign((size_t)nSpts * nN, 0.0); dshape_spts.assign((size_t)D * nSpts * nN, 0.0);
     shape_fpts.assign((size_t)nFpts * nN, 0.0); dshape_fpts.assign((size_t)D * nFpts * nN, 0.0);
-    std::vector<double> v(nN), dv(nN * D);
+    std::vector<real> v(nN), dv(nN * D);
     for (unsigned s = 0; s < nSpts; s++) {
       shape_at(&loc_spts[s * D], v.data(), dv.data());
       for (int n = 0; n < nN; n++) {
@@ -668,10 +691,10 @@ struct Oracle {
   }
 
   // cblas_dgemm(RowMajor, NoTrans, NoTrans, alpha=1, beta=0) as used for geometry
-  static void dgemm_dense(int M, int N, int K, const double *A, const double *B, double *C) {
+  static void dgemm_dense(int M, int N, int K, const real *A, const real *B, real *C) {
     for (int i = 0; i < M; i++)
       for (int j = 0; j < N; j++) {
-        double s = 0.0;
+        real s = 0.0;
         for (int k = 0; k < K; k++) s += A[(size_t)i * K + k] * B[(size_t)k * N + j];
         C[(size_t)i * N + j] = s;
       }
@@ -680,14 +703,14 @@ struct Oracle {
   // ------------------------------------------------------------ orient_fpts (solver.cpp:312-437)
   void orient_fpts() {
     const int D = nDims;
-    std::vector<double> cL((size_t)D * nGfpts, 0.0), cR((size_t)D * nGfpts, 0.0);
+    std::vector<real> cL((size_t)D * nGfpts, 0.0), cR((size_t)D * nGfpts, 0.0);
     std::vector<size_t> ptrL(nGfpts, 0);
     for (int e = 0; e < nEles; e++)
       for (unsigned f = 0; f < nFpts; f++) {
         int g = fpt2gfpt[(size_t)f * nEles + e];
         int s = fpt2slot[(size_t)f * nEles + e];
         for (int d = 0; d < D; d++) {
-          double c = coord_fpts[((size_t)f * D + d) * nEles + e];
+          real c = coord_fpts[((size_t)f * D + d) * nEles + e];
           if (s == 0) cL[(size_t)d * nGfpts + g] = c; else cR[(size_t)d * nGfpts + g] = c;
         }
         // address of coord_fpts(fpt, 0, ele) in units of doubles
@@ -726,22 +749,22 @@ struct Oracle {
   }
 
   // ------------------------------------------------------------ metrics (elements.cpp:845-996)
-  void inverse_transforms(const std::vector<double> &jac, std::vector<double> &inv, std::vector<double> *det,
+  void inverse_transforms(const std::vector<real> &jac, std::vector<real> &inv, std::vector<real> *det,
                           unsigned nPts) {
     const int D = nDims;
     auto J = [&](int a, unsigned p, int b, int e) { return jac[(((size_t)a * nPts + p) * D + b) * nEles + e]; };
-    auto I = [&](int a, unsigned p, int b, int e) -> double & { return inv[(((size_t)a * nPts + p) * D + b) * nEles + e]; };
+    auto I = [&](int a, unsigned p, int b, int e) -> real & { return inv[(((size_t)a * nPts + p) * D + b) * nEles + e]; };
     for (int e = 0; e < nEles; e++)
       for (unsigned p = 0; p < nPts; p++) {
-        double dt;
+        real dt;
         if (D == 2) {
           dt = J(0, p, 0, e) * J(1, p, 1, e) - J(0, p, 1, e) * J(1, p, 0, e);
           I(0, p, 0, e) = J(1, p, 1, e);  I(0, p, 1, e) = -J(1, p, 0, e);
           I(1, p, 0, e) = -J(0, p, 1, e); I(1, p, 1, e) = J(0, p, 0, e);
         } else {
-          double xr = J(0, p, 0, e), xs = J(1, p, 0, e), xt = J(2, p, 0, e);
-          double yr = J(0, p, 1, e), ys = J(1, p, 1, e), yt = J(2, p, 1, e);
-          double zr = J(0, p, 2, e), zs = J(1, p, 2, e), zt = J(2, p, 2, e);
+          real xr = J(0, p, 0, e), xs = J(1, p, 0, e), xt = J(2, p, 0, e);
+          real yr = J(0, p, 1, e), ys = J(1, p, 1, e), yt = J(2, p, 1, e);
+          real zr = J(0, p, 2, e), zs = J(1, p, 2, e), zt = J(2, p, 2, e);
           dt = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
           I(0, p, 0, e) = ys * zt - yt * zs; I(0, p, 1, e) = xt * zs - xs * zt; I(0, p, 2, e) = xs * yt - xt * ys;
           I(1, p, 0, e) = yt * zr - yr * zt; I(1, p, 1, e) = xr * zt - xt * zr; I(1, p, 2, e) = xt * yr - xr * yt;
@@ -771,13 +794,13 @@ struct Oracle {
       for (unsigned f = 0; f < nFpts; f++) {
         int g = fpt2gfpt[(size_t)f * nEles + e];
         int s = fpt2slot[(size_t)f * nEles + e];
-        double n[3] = {0, 0, 0};
+        real n[3] = {0, 0, 0};
         for (int d1 = 0; d1 < D; d1++)
           for (int d2 = 0; d2 < D; d2++)
             n[d1] += inv_jaco_fpts[(((size_t)d2 * nFpts + f) * D + d1) * nEles + e] * tnorm[f * D + d2];
         if (s == 0)
           for (int d = 0; d < D; d++) norm[(size_t)d * nGfpts + g] = n[d];
-        double a = 0;
+        real a = 0;
         for (int d = 0; d < D; d++) a += n[d] * n[d];
         dA[(size_t)s * nGfpts + g] = std::sqrt(a);
         if (s == 0)
@@ -786,17 +809,17 @@ struct Oracle {
   }
 
   // ------------------------------------------------------------ FR operators (elements.cpp:555-654)
-  double nodal_basis(unsigned spt, const double *loc) const {  // hexas.cpp:337-348, quads.cpp:239-248
-    double v = 1.0;
+  real nodal_basis(unsigned spt, const real *loc) const {  // hexas.cpp:337-348, quads.cpp:239-248
+    real v = 1.0;
     for (int d = 0; d < nDims; d++) v *= lagrange(loc1d, idx_spts[spt * nDims + d], loc[d]);
     return v;
   }
   // DFR derivative basis on [-1, GL, 1] (hexas.cpp:384-435 / 464-490, quads.cpp:280-361)
-  double d_nodal_basis(const int *idx, const double *loc, int dim) const {
-    double v = 1.0;
+  real d_nodal_basis(const int *idx, const real *loc, int dim) const {
+    real v = 1.0;
     for (int d = 0; d < nDims; d++) {
       unsigned m = idx[d] + 1;
-      double t = (d == dim) ? lagrange_d1(locDFR, m, loc[d]) : lagrange(locDFR, m, loc[d]);
+      real t = (d == dim) ? lagrange_d1(locDFR, m, loc[d]) : lagrange(locDFR, m, loc[d]);
       v = (d == 0) ? t : v * t;
     }
     return v;
@@ -812,7 +835,7 @@ struct Oracle {
     for (int dim = 0; dim < D; dim++)
       for (unsigned j = 0; j < nSpts; j++)
         for (unsigned i = 0; i < nSpts; i++) {
-          double v = d_nodal_basis(&idx_spts[j * D], &loc_spts[i * D], dim);
+          real v = d_nodal_basis(&idx_spts[j * D], &loc_spts[i * D], dim);
           oppD[((size_t)dim * nSpts + i) * nSpts + j] = v;
           oppDiv[((size_t)i * D + dim) * nSpts + j] = v;
         }
@@ -867,7 +890,7 @@ struct Oracle {
   }
 
   // face views (solver.cpp:2402-2430): element-local address of (slot, var, gfpt)
-  size_t fview(const std::vector<double> &, int slot, int var, unsigned g) const {
+  size_t fview(const std::vector<real> &, int slot, int var, unsigned g) const {
     return ((size_t)gf_fpt[(size_t)slot * nGfpts + g] * nVars + var) * nEles + gf_ele[(size_t)slot * nGfpts + g];
   }
 
@@ -878,19 +901,19 @@ struct Oracle {
 #pragma omp parallel for schedule(static)
     for (int spt = 0; spt < (int)nSpts; spt++)
       for (size_t e = 0; e < E; e++) {
-        double U[V], tdU[V][D], F[V][D], dU[V][D];
+        real U[V], tdU[V][D], F[V][D], dU[V][D];
         for (int v = 0; v < V; v++) {
           U[v] = U_spts[((size_t)spt * V + v) * E + e];
           if (ph.viscous)
             for (int d = 0; d < D; d++) tdU[v][d] = dU_spts[(((size_t)d * nSpts + spt) * V + v) * E + e];
         }
-        double ij[D][D];
+        real ij[D][D];
         for (int a = 0; a < D; a++)
           for (int b = 0; b < D; b++) ij[a][b] = inv_jaco_spts[(((size_t)a * nSpts + spt) * D + b) * E + e];
         for (int v = 0; v < V; v++)
           for (int d = 0; d < D; d++) dU[v][d] = 0.0;
         if (ph.viscous) {
-          double inv_det = 1.0 / jaco_det_spts[(size_t)spt * E + e];
+          real inv_det = 1.0 / jaco_det_spts[(size_t)spt * E + e];
           for (int v = 0; v < V; v++)
             for (int d1 = 0; d1 < D; d1++) {
               for (int d2 = 0; d2 < D; d2++) dU[v][d1] += tdU[v][d2] * ij[d2][d1];
@@ -898,12 +921,12 @@ struct Oracle {
               dU_spts[(((size_t)d1 * nSpts + spt) * V + v) * E + e] = dU[v][d1];
             }
         }
-        double P;
+        real P;
         fconv<V, D>(U, F, P, ph.gamma);
         if (ph.viscous) fvisc_add<V, D>(U, dU, F, ph);
         for (int v = 0; v < V; v++)
           for (int d1 = 0; d1 < D; d1++) {
-            double t = 0.0;
+            real t = 0.0;
             for (int d2 = 0; d2 < D; d2++) t += F[v][d2] * ij[d1][d2];
             F_spts[(((size_t)d1 * nSpts + spt) * V + v) * E + e] = t;
           }
@@ -912,10 +935,10 @@ struct Oracle {
 
   template <int V>
   void common_U() {  // faces.cpp:1366-1474, LDG branch, interior flux points
-    const double beta = 1.0 * ph.ldg_b;  // flip_beta == 1 on one rank (geometry.cpp:3364-3392)
+    const real beta = 1.0 * ph.ldg_b;  // flip_beta == 1 on one rank (geometry.cpp:3364-3392)
 #pragma omp parallel for schedule(static)
     for (int g = 0; g < (int)nGfpts; g++) {
-      double UL[V] = {}, UR[V] = {};
+      real UL[V] = {}, UR[V] = {};
       if (beta == 0.5)
         for (int n = 0; n < V; n++) UR[n] = U_fpts[fview(U_fpts, 1, n, g)];
       else if (beta == -0.5)
@@ -923,7 +946,7 @@ struct Oracle {
       else
         for (int n = 0; n < V; n++) { UL[n] = U_fpts[fview(U_fpts, 0, n, g)]; UR[n] = U_fpts[fview(U_fpts, 1, n, g)]; }
       for (int n = 0; n < V; n++) {
-        double UC = 0.5 * (UL[n] + UR[n]) - beta * (UL[n] - UR[n]);
+        real UC = 0.5 * (UL[n] + UR[n]) - beta * (UL[n] - UR[n]);
         Ucomm[fview(Ucomm, 0, n, g)] = UC;
         Ucomm[fview(Ucomm, 1, n, g)] = UC;
       }
@@ -934,10 +957,10 @@ struct Oracle {
   void rusanov() {  // faces.cpp:1244-1364, rus_bias == 0 (interior)
 #pragma omp parallel for schedule(static)
     for (int g = 0; g < (int)nGfpts; g++) {
-      double UL[V], UR[V], FL[V][D], FR[V][D], PL, PR;
+      real UL[V], UR[V], FL[V][D], FR[V][D], PL, PR;
       for (int n = 0; n < V; n++) { UL[n] = U_fpts[fview(U_fpts, 0, n, g)]; UR[n] = U_fpts[fview(U_fpts, 1, n, g)]; }
       if (ph.fconv == 2) {
-        double nv[D], F[V];
+        real nv[D], F[V];
         for (int d = 0; d < D; d++) nv[d] = norm[(size_t)d * nGfpts + g];
         roe_flux<V, D>(UL, UR, nv, ph.gamma, F);
         for (int n = 0; n < V; n++) {
@@ -948,22 +971,22 @@ struct Oracle {
       }
       fconv<V, D>(UL, FL, PL, ph.gamma);
       fconv<V, D>(UR, FR, PR, ph.gamma);
-      double aL = std::sqrt(ph.gamma * PL / UL[0]);
-      double aR = std::sqrt(ph.gamma * PR / UR[0]);
-      double VnL = 0.0, VnR = 0.0;
+      real aL = std::sqrt(ph.gamma * PL / UL[0]);
+      real aR = std::sqrt(ph.gamma * PR / UR[0]);
+      real VnL = 0.0, VnR = 0.0;
       for (int d = 0; d < D; d++) {
         VnL += UL[d + 1] / UL[0] * norm[(size_t)d * nGfpts + g];
         VnR += UR[d + 1] / UR[0] * norm[(size_t)d * nGfpts + g];
       }
-      double eig = std::max(std::abs(VnL) + aL, std::abs(VnR) + aR);
-      double FnL[V] = {0}, FnR[V] = {0};
+      real eig = std::max(std::abs(VnL) + aL, std::abs(VnR) + aR);
+      real FnL[V] = {0}, FnR[V] = {0};
       for (int n = 0; n < V; n++)
         for (int d = 0; d < D; d++) {
           FnL[n] += FL[n][d] * norm[(size_t)d * nGfpts + g];
           FnR[n] += FR[n][d] * norm[(size_t)d * nGfpts + g];
         }
       for (int n = 0; n < V; n++) {
-        double F = (0.5 * (FnR[n] + FnL[n]) - 0.5 * eig * (1.0 - ph.rus_k) * (UR[n] - UL[n]));
+        real F = (0.5 * (FnR[n] + FnL[n]) - 0.5 * eig * (1.0 - ph.rus_k) * (UR[n] - UL[n]));
         Fcomm[fview(Fcomm, 0, n, g)] = F * dA[g];
         Fcomm[fview(Fcomm, 1, n, g)] = -F * dA[nGfpts + g];
       }
@@ -972,11 +995,11 @@ struct Oracle {
 
   template <int V, int D>
   void ldg_flux() {  // faces.cpp:1476-1614, LDG_bias == 0 (interior)
-    const double tau = ph.ldg_tau;
-    const double beta = 1.0 * ph.ldg_b;
+    const real tau = ph.ldg_tau;
+    const real beta = 1.0 * ph.ldg_b;
 #pragma omp parallel for schedule(static)
     for (int g = 0; g < (int)nGfpts; g++) {
-      double UL[V], UR[V], dUL[V][D] = {}, dUR[V][D] = {};
+      real UL[V], UR[V], dUL[V][D] = {}, dUR[V][D] = {};
       for (int n = 0; n < V; n++) { UL[n] = U_fpts[fview(U_fpts, 0, n, g)]; UR[n] = U_fpts[fview(U_fpts, 1, n, g)]; }
       auto dUv = [&](int slot, int d, int n) {
         size_t f = gf_fpt[(size_t)slot * nGfpts + g], e = gf_ele[(size_t)slot * nGfpts + g];
@@ -988,17 +1011,17 @@ struct Oracle {
           else if (beta == -0.5) dUR[n][d] = dUv(1, d, n);
           else { dUL[n][d] = dUv(0, d, n); dUR[n][d] = dUv(1, d, n); }
         }
-      double FL[V][D] = {}, FR[V][D] = {};
+      real FL[V][D] = {}, FR[V][D] = {};
       fvisc_add<V, D>(UL, dUL, FL, ph);
       fvisc_add<V, D>(UR, dUR, FR, ph);
-      double FnL[V] = {0}, FnR[V] = {0};
+      real FnL[V] = {0}, FnR[V] = {0};
       for (int n = 0; n < V; n++)
         for (int d = 0; d < D; d++) {
           FnL[n] += FL[n][d] * norm[(size_t)d * nGfpts + g];
           FnR[n] += FR[n][d] * norm[(size_t)d * nGfpts + g];
         }
       for (int n = 0; n < V; n++) {
-        double Fc = 0.5 * (FnL[n] + FnR[n]) + tau * (UL[n] - UR[n]) + beta * (FnL[n] - FnR[n]);
+        real Fc = 0.5 * (FnL[n] + FnR[n]) + tau * (UL[n] - UR[n]) + beta * (FnL[n] - FnR[n]);
         Fcomm[fview(Fcomm, 0, n, g)] += Fc * dA[g];
         Fcomm[fview(Fcomm, 1, n, g)] -= Fc * dA[nGfpts + g];
       }
@@ -1019,7 +1042,7 @@ struct Oracle {
       for (int d = 0; d < D; d++)  // extrapolate_dU
         op_mm(nFpts, n, nSpts, 1.0, oppE.data(), dU_spts.data() + (size_t)d * nSpts * n, 0.0,
               dU_fpts.data() + (size_t)d * nFpts * n);
-    double *dv = divF.data() + (size_t)stage * nSpts * n;
+    real *dv = divF.data() + (size_t)stage * nSpts * n;
     op_mm(nSpts, n, nSpts * D, 1.0, oppDiv.data(), F_spts.data(), 0.0, dv);  // compute_divF_spts
     rusanov<V, D>();
     if (ph.viscous) ldg_flux<V, D>();
@@ -1033,13 +1056,13 @@ struct Oracle {
 
   // step_RK (solver.cpp:2865-3097) with dt_type 0; src (reference layout [nS][V][E]) is
   // the p-multigrid source term of the sourceBT branches (solver.cpp:2946-2962, 3061-3073)
-  void step_rk(const double *src = nullptr) {
+  void step_rk(const real *src = nullptr) {
     const size_t E = nEles, V = nVars, n = V * E;
-    const double dt = ph.dt;
+    const real dt = ph.dt;
     if (ph.nStages > 1) U_ini = U_spts;
     for (int stage = 0; stage < ph.nStages - 1; stage++) {
       compute_residual(stage);
-      const double *dv = divF.data() + (size_t)stage * nSpts * n;
+      const real *dv = divF.data() + (size_t)stage * nSpts * n;
 #pragma omp parallel for schedule(static)
       for (int s = 0; s < (int)nSpts; s++)
         for (size_t v = 0; v < V; v++)
@@ -1053,10 +1076,10 @@ struct Oracle {
     }
     compute_residual(ph.nStages - 1);
     if (ph.nStages > 1) U_spts = U_ini;
-    const double *beta = (ph.nStages > 1) ? rk_beta : nullptr;
+    const real *beta = (ph.nStages > 1) ? rk_beta : nullptr;
     for (int stage = 0; stage < ph.nStages; stage++) {
-      const double b = beta ? beta[stage] : 1.0;
-      const double *dv = divF.data() + (size_t)stage * nSpts * n;
+      const real b = beta ? beta[stage] : 1.0;
+      const real *dv = divF.data() + (size_t)stage * nSpts * n;
 #pragma omp parallel for schedule(static)
       for (int s = 0; s < (int)nSpts; s++)
         for (size_t v = 0; v < V; v++)
@@ -1071,21 +1094,21 @@ struct Oracle {
   }
 
   // report_residuals (solver.cpp:6990-7137): per-variable norm of divF(0)/detJ, / nDoF
-  void report_residuals(double *res) const {
+  void report_residuals(real *res) const {
     const size_t E = nEles, V = nVars;
-    std::vector<double> r(V, 0.0);
+    std::vector<real> r(V, 0.0);
     for (size_t s = 0; s < nSpts; s++)
       for (size_t v = 0; v < V; v++)
         for (size_t e = 0; e < E; e++) {
-          double x = divF[(s * V + v) * E + e], j = jaco_det_spts[s * E + e];
+          real x = divF[(s * V + v) * E + e], j = jaco_det_spts[s * E + e];
           if (ph.res_type == 0) r[v] = std::max(r[v], std::abs(x / j));
           else if (ph.res_type == 1) r[v] += std::abs(x / j);
           else r[v] += x * x / (j * j);
           if (std::isnan(r[v])) fail("oracle: NaN in residual");
         }
-    const double nDoF = (double)(nSpts * E);
+    const real nDoF = (real)(nSpts * E);
     for (size_t v = 0; v < V; v++) {
-      double x = (ph.res_type == 2) ? std::sqrt(r[v]) : r[v];
+      real x = (ph.res_type == 2) ? std::sqrt(r[v]) : r[v];
       res[v] = x / nDoF;
     }
   }
@@ -1101,37 +1124,37 @@ struct OraclePMG {
   Oracle *fine = nullptr;
   std::vector<std::unique_ptr<Oracle>> grids;  // grids[0] empty: level 0 is `fine`
   std::vector<int> levels, steps;
-  double rel_fac = 1.0;
+  real rel_fac = 1.0;
   std::string cycle_type = "V";
   int fmg_vcycles = 1;
-  std::vector<std::vector<double>> oppRes, oppPro;           // per level
-  std::vector<std::vector<double>> sources, solutions, corrections;  // [nS][V][E] per level
+  std::vector<std::vector<real>> oppRes, oppPro;           // per level
+  std::vector<std::vector<real>> sources, solutions, corrections;  // [nS][V][E] per level
 
   int nLevels() const { return (int)levels.size(); }
   Oracle &level(int n) { return n == 0 ? *fine : *grids[n]; }
 
   // dense single-order step between orders: rows = points of order `to`, columns = order `from`
-  static std::vector<double> step_op(int D, int from, int to) {
+  static std::vector<real> step_op(int D, int from, int to) {
     const unsigned nf = from + 1, nt = to + 1;
     const auto xf = gl_points(nf), xt = gl_points(nt);
     const unsigned Nf = D == 2 ? nf * nf : nf * nf * nf, Nt = D == 2 ? nt * nt : nt * nt * nt;
-    std::vector<double> A((size_t)Nt * Nf);
+    std::vector<real> A((size_t)Nt * Nf);
     for (unsigned spt = 0; spt < Nf; spt++) {
       const unsigned i1 = spt % nf, j1 = (spt / nf) % nf, k1 = spt / (nf * nf);
       for (unsigned r = 0; r < Nt; r++) {
         const unsigned i2 = r % nt, j2 = (r / nt) % nt, k2 = r / (nt * nt);
-        double v = lagrange(xf, i1, xt[i2]) * lagrange(xf, j1, xt[j2]);
+        real v = lagrange(xf, i1, xt[i2]) * lagrange(xf, j1, xt[j2]);
         if (D == 3) v *= lagrange(xf, k1, xt[k2]);
         A[(size_t)r * Nf + spt] = v;
       }
     }
     return A;
   }
-  static std::vector<double> dense_mm(const std::vector<double> &A, const std::vector<double> &B, int m, int k, int n) {
-    std::vector<double> C((size_t)m * n, 0.0);
+  static std::vector<real> dense_mm(const std::vector<real> &A, const std::vector<real> &B, int m, int k, int n) {
+    std::vector<real> C((size_t)m * n, 0.0);
     for (int i = 0; i < m; i++)
       for (int j = 0; j < n; j++) {
-        double s = 0.0;
+        real s = 0.0;
         for (int l = 0; l < k; l++) s += A[(size_t)i * k + l] * B[(size_t)l * n + j];
         C[(size_t)i * n + j] = s;
       }
@@ -1139,14 +1162,14 @@ struct OraclePMG {
   }
   static int npts(int D, int p) { return D == 2 ? (p + 1) * (p + 1) : (p + 1) * (p + 1) * (p + 1); }
   // hexas.cpp:646-697: ((opps[res] opps[res+1]) opps[res+2]) ... opps[order-1]
-  static std::vector<double> restriction(int D, int order, int res) {
-    std::vector<double> R = step_op(D, res + 1, res);
+  static std::vector<real> restriction(int D, int order, int res) {
+    std::vector<real> R = step_op(D, res + 1, res);
     for (int P = res + 1; P < order; P++) R = dense_mm(R, step_op(D, P + 1, P), npts(D, res), npts(D, P), npts(D, P + 1));
     return R;
   }
   // hexas.cpp:594-644: ((opps[pro] opps[pro-1]) ...) opps[order+1]
-  static std::vector<double> prolongation(int D, int order, int pro) {
-    std::vector<double> Pm = step_op(D, pro - 1, pro);
+  static std::vector<real> prolongation(int D, int order, int pro) {
+    std::vector<real> Pm = step_op(D, pro - 1, pro);
     for (int P = pro - 1; P > order; P--) Pm = dense_mm(Pm, step_op(D, P - 1, P), npts(D, pro), npts(D, P), npts(D, P - 1));
     return Pm;
   }
@@ -1181,12 +1204,12 @@ struct OraclePMG {
   }
 
   // C[nSc][V*E] = alpha A[nSc][nSf] B[nSf][V*E] + beta C
-  static void gemm(const std::vector<double> &A, int m, int k, const double *B, int n, double alpha, double beta,
-                   double *C) {
+  static void gemm(const std::vector<real> &A, int m, int k, const real *B, int n, real alpha, real beta,
+                   real *C) {
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < m; i++) {
-        double s = 0.0;
+        real s = 0.0;
         for (int l = 0; l < k; l++) s += A[(size_t)i * k + l] * B[(size_t)l * n + j];
         C[(size_t)i * n + j] = (beta == 0.0) ? alpha * s : alpha * s + beta * C[(size_t)i * n + j];
       }
@@ -1200,7 +1223,7 @@ struct OraclePMG {
     gemm(oppRes[nf], c.nSpts, f.nSpts, f.divF.data(), n, 1.0, 0.0, c.divF.data());
   }
   // multigrid.cpp:394-413
-  void prolong_err(int nc, const std::vector<double> &corr) {
+  void prolong_err(int nc, const std::vector<real> &corr) {
     Oracle &c = level(nc), &f = level(nc - 1);
     gemm(oppPro[nc], f.nSpts, c.nSpts, corr.data(), c.nVars * c.nEles, rel_fac, 1.0, f.U_spts.data());
   }
@@ -1290,6 +1313,20 @@ struct zo_phys {
 
 const char *zo_last_error(void) { return g_err.c_str(); }
 
+}  // extern "C"
+
+namespace {
+Phys phys_from(const zo_phys *p) {
+  Phys ph;
+  ph.viscous = p->viscous; ph.gamma = p->gamma; ph.prandtl = p->prandtl; ph.mu = p->mu; ph.rt = p->rt;
+  ph.c_sth = p->c_sth; ph.fix_vis = p->fix_vis; ph.rus_k = p->rus_k; ph.ldg_b = p->ldg_b;
+  ph.ldg_tau = p->ldg_tau; ph.dt = p->dt; ph.nStages = p->nStages; ph.res_type = p->res_type; ph.fconv = p->fconv;
+  return ph;
+}
+}  // namespace
+
+extern "C" {
+
 void *zo_create(const char *meshfile, int nDims, int order, const zo_phys *p) {
   Oracle *o = new Oracle();
   int rc = guarded([&] {
@@ -1298,7 +1335,7 @@ void *zo_create(const char *meshfile, int nDims, int order, const zo_phys *p) {
     if (p->nStages != 1 && p->nStages != 4) fail("oracle: nStages must be 1 or 4");
     if (p->fconv != 0 && p->fconv != 2) fail("oracle: fconv must be 0 (Rusanov) or 2 (Roe)");
     o->nDims = nDims; o->order = order;
-    std::memcpy(&o->ph, p, sizeof(zo_phys));
+    o->ph = phys_from(p);
     o->setup(meshfile);
   });
   if (rc) { delete o; return nullptr; }
@@ -1326,7 +1363,7 @@ int zo_get_maps(void *h, int *fpt2gfpt, signed char *slot) {
 // name: one of the double arrays below; n: capacity in doubles; returns count or -1
 long zo_get_array(void *h, const char *name, double *out, long n) {
   auto *o = static_cast<Oracle *>(h);
-  const std::map<std::string, const std::vector<double> *> arrs = {
+  const std::map<std::string, const std::vector<real> *> arrs = {
       {"oppE", &o->oppE}, {"oppD", &o->oppD}, {"oppD_fpts", &o->oppD_fpts}, {"oppDiv", &o->oppDiv},
       {"oppDiv_fpts", &o->oppDiv_fpts}, {"coord_spts", &o->coord_spts}, {"coord_fpts", &o->coord_fpts},
       {"jaco_det_spts", &o->jaco_det_spts}, {"inv_jaco_spts", &o->inv_jaco_spts},
@@ -1339,13 +1376,13 @@ long zo_get_array(void *h, const char *name, double *out, long n) {
   if (it == arrs.end()) { g_err = std::string("oracle: unknown array ") + name; return -1; }
   const auto &a = *it->second;
   if ((long)a.size() > n) { g_err = "oracle: output buffer too small"; return -1; }
-  std::memcpy(out, a.data(), a.size() * sizeof(double));
+  std::copy(a.begin(), a.end(), out);
   return (long)a.size();
 }
 
 int zo_set_U(void *h, const double *U) {
   auto *o = static_cast<Oracle *>(h);
-  std::memcpy(o->U_spts.data(), U, o->U_spts.size() * sizeof(double));
+  std::copy(U, U + o->U_spts.size(), o->U_spts.begin());
   return 0;
 }
 
@@ -1360,7 +1397,12 @@ int zo_step_rk(void *h, int nsteps) {
 }
 
 int zo_report_residuals(void *h, double *res) {
-  return guarded([&] { static_cast<Oracle *>(h)->report_residuals(res); });
+  return guarded([&] {
+    auto *o = static_cast<Oracle *>(h);
+    std::vector<real> r(o->nVars);
+    o->report_residuals(r.data());
+    std::copy(r.begin(), r.end(), res);
+  });
 }
 
 // Stand-alone pieces for closed-form checks of the restatement.
@@ -1404,7 +1446,7 @@ long zo_pmg_array(void *h, int level, int which, double *out, long n, int set) {
     auto *p = static_cast<OraclePMG *>(h);
     if (level < 1 || level >= p->nLevels()) fail("oracle: pmg level out of range");
     Oracle &g = *p->grids[level];
-    std::vector<double> *v = nullptr;
+    std::vector<real> *v = nullptr;
     switch (which) {
       case 0: v = &g.U_spts; break;
       case 2: v = &g.divF; break;
@@ -1440,37 +1482,45 @@ long zo_pmg_operator(void *h, int level, int kind, double *out, long n) {
 }
 
 double zo_lagrange(const double *grid, int n, int mode, double x) {
-  return lagrange(std::vector<double>(grid, grid + n), mode, x);
+  return (double)lagrange(std::vector<real>(grid, grid + n), mode, x);
 }
 
 double zo_lagrange_d1(const double *grid, int n, int mode, double x) {
-  return lagrange_d1(std::vector<double>(grid, grid + n), mode, x);
+  return (double)lagrange_d1(std::vector<real>(grid, grid + n), mode, x);
 }
 
 // Pointwise fluxes on one state (nDims 2 or 3): F[V][D] convective (+ viscous if dU given).
 // Convective common normal flux for the unit normal n (no area): Rusanov or Roe.
 int zo_common_flux(int nDims, const zo_phys *p, const double *UL, const double *UR, const double *n, double *F) {
   return guarded([&] {
-    Phys ph; std::memcpy(&ph, p, sizeof(zo_phys));
+    const Phys ph = phys_from(p);
     if (ph.fconv != 2) fail("oracle: zo_common_flux is the Roe flux (fconv 2)");
-    if (nDims == 2) roe_flux<4, 2>(UL, UR, n, ph.gamma, F);
-    else roe_flux<5, 3>(UL, UR, n, ph.gamma, F);
+    const int V = nDims + 2;
+    real ul[5], ur[5], nn[3], f[5];
+    std::copy(UL, UL + V, ul);
+    std::copy(UR, UR + V, ur);
+    std::copy(n, n + nDims, nn);
+    if (nDims == 2) roe_flux<4, 2>(ul, ur, nn, ph.gamma, f);
+    else roe_flux<5, 3>(ul, ur, nn, ph.gamma, f);
+    std::copy(f, f + V, F);
   });
 }
 
 int zo_point_flux(int nDims, const zo_phys *p, const double *U, const double *dU, double *F) {
   return guarded([&] {
-    Phys ph; std::memcpy(&ph, p, sizeof(zo_phys));
+    const Phys ph = phys_from(p);
+    real u[5];
+    std::copy(U, U + nDims + 2, u);
     if (nDims == 2) {
-      double f[4][2], P, du[4][2];
-      fconv<4, 2>(U, f, P, ph.gamma);
-      if (dU) { std::memcpy(du, dU, sizeof(du)); fvisc_add<4, 2>(U, du, f, ph); }
-      std::memcpy(F, f, sizeof(f));
+      real f[4][2], P, du[4][2];
+      fconv<4, 2>(u, f, P, ph.gamma);
+      if (dU) { std::copy(dU, dU + 8, &du[0][0]); fvisc_add<4, 2>(u, du, f, ph); }
+      std::copy(&f[0][0], &f[0][0] + 8, F);
     } else {
-      double f[5][3], P, du[5][3];
-      fconv<5, 3>(U, f, P, ph.gamma);
-      if (dU) { std::memcpy(du, dU, sizeof(du)); fvisc_add<5, 3>(U, du, f, ph); }
-      std::memcpy(F, f, sizeof(f));
+      real f[5][3], P, du[5][3];
+      fconv<5, 3>(u, f, P, ph.gamma);
+      if (dU) { std::copy(dU, dU + 15, &du[0][0]); fvisc_add<5, 3>(u, du, f, ph); }
+      std::copy(&f[0][0], &f[0][0] + 15, F);
     }
   });
 }

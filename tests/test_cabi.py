@@ -112,3 +112,34 @@ def test_kernel_entries_reject_missing_buffers_before_any_launch():
     assert L.zfr_face_correction(C.byref(d), buf, None) != 0 and "Ftr" in _lib.last_error()
     d.ldg_b = 0.2
     assert L.zfr_divergence(C.byref(d), buf, buf, None) != 0 and "Gtr" in _lib.last_error()
+
+
+def test_kernel_entries_reject_null_arguments():
+    """A NULL descriptor or a NULL state / residual argument is an error with a message,
+    returned before any HIP call."""
+    L = _lib.lib()
+    buf = C.c_void_p(0x1000)  # never dereferenced
+    d = _lib.EleDesc(nDims=3, order=3, nVars=5, nEles=8, nSpts=64, nFpts=96, viscous=1, ldg_b=0.5,
+                     gamma=1.4, prandtl=0.72, mu=1e-3, fix_vis=1, ldg_tau=0.1,
+                     ops1d=buf, metric=buf, fgeo=buf, nbr=buf, Utr=buf, Ftr=buf, Gtr=buf)
+    rk = C.c_void_p(0x2000)
+    null_desc = [("zfr_extrapolate", (None, buf, None)), ("zfr_gradient", (None, buf, buf, None)),
+                 ("zfr_divergence", (None, buf, buf, None)), ("zfr_gradient_flux", (None, buf, buf, None)),
+                 ("zfr_gradient_flux_traced", (None, buf, buf, None)), ("zfr_face_correction", (None, buf, None)),
+                 ("zfr_face_correction_rk", (None, buf, rk, None)), ("zfr_divergence_rk", (None, buf, buf, rk, None))]
+    for name, args in null_desc:
+        assert getattr(L, name)(*args) != 0, name
+        assert "NULL descriptor" in _lib.last_error(), (name, _lib.last_error())
+    null_arg = [("zfr_extrapolate", (C.byref(d), None, None), "U"),
+                ("zfr_gradient", (C.byref(d), None, buf, None), "U"),
+                ("zfr_divergence", (C.byref(d), None, buf, None), "U"),
+                ("zfr_divergence", (C.byref(d), buf, None, None), "divF"),
+                ("zfr_gradient_flux", (C.byref(d), None, buf, None), "U"),
+                ("zfr_gradient_flux", (C.byref(d), buf, None, None), "divF"),
+                ("zfr_gradient_flux_traced", (C.byref(d), buf, None, None), "divF"),
+                ("zfr_face_correction", (C.byref(d), None, None), "divF"),
+                ("zfr_face_correction_rk", (C.byref(d), None, rk, None), "divF"),
+                ("zfr_divergence_rk", (C.byref(d), None, buf, rk, None), "U")]
+    for name, args, what in null_arg:
+        assert getattr(L, name)(*args) != 0, name
+        assert f"{what} is NULL" in _lib.last_error(), (name, _lib.last_error())

@@ -89,6 +89,52 @@ def test_c3_hex_p3_64_properties():
     assert np.abs(z.state("divF", 0, layout="device")).max() < 1e-11
 
 
+def test_c3_hex_p3_64_200_steps_conservation_and_decay():
+    """The 3D Taylor-Green configuration as specified: 64^3 hexes, p=3, Re 1600, Ma 0.1,
+    Pr 0.71, Roe + LDG, RK4 dt = 2e-4, 200 steps.  No oracle at this size; checks:
+      * mass, x-momentum and energy integrals (sum of w detJ U) stay constant to rounding
+        (DFR on Gauss-Legendre points is discretely conservative on a periodic mesh);
+      * the kinetic energy decreases monotonically at the viscous rate of the initial field:
+        for u = U0 (sin x cos y cos z, -cos x sin y cos z, 0) on the 2 pi box,
+        dK/dt / K = -mu int |grad u|^2 / (rho0 int |u|^2 / 2) = -6 mu / rho0 (incompressible
+        limit; Ma 0.1 adds O(Ma^2) pressure work), to 5%."""
+    import torch
+    from zefr_amd.autograd import device_field
+    print("[full] c3 200-step setup", flush=True)
+    mesh = make_mesh("c3_hex64.msh", (64, 64, 64), (0.0,) * 3, (TWO_PI,) * 3)
+    deck = write_deck(tmp("c3_roe.deck"), mesh, 3, 3, True, dt=2e-4, ic_type=1, test_case=4,
+                      extra={"fconv_type": "Roe"})
+    z = product(deck)
+    inp = z.get_input()
+    mu, rho0 = inp.mu, inp.rho_fs
+    w = torch.from_numpy(z.host_array("weights_spts")).cuda()
+    detJ = device_field(z, "jaco_det_spts")[:, 0, :]  # [E][nS]
+
+    def integrals():
+        U = z.state_tensor()  # [E][V][nS]
+        wd = w[None, :] * detJ
+        tot = (U * wd[:, None, :]).sum(dim=(0, 2))
+        scale = (U.abs() * wd[:, None, :]).sum(dim=(0, 2))
+        ke = (0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 3] ** 2) / U[:, 0] * wd).sum()
+        return tot.cpu().numpy(), scale.cpu().numpy(), float(ke)
+
+    tot0, scale0, ke0 = integrals()
+    kes = [ke0]
+    for step in range(200):
+        z.do_step()
+        if (step + 1) % 20 == 0:
+            tot, scale, ke = integrals()
+            kes.append(ke)
+            assert (np.abs(tot - tot0)[[0, 1, 4]] <= 1e-13 * scale0[[0, 1, 4]]).all(), (step, tot - tot0, scale0)
+    print(f"[full] c3 200 steps: K {ke0:.12e} -> {kes[-1]:.12e}", flush=True)
+    assert all(b < a for a, b in zip(kes, kes[1:])), kes
+    t = 200 * inp.dt
+    rate = (kes[-1] - ke0) / (t * ke0)
+    want = -6.0 * mu / rho0
+    print(f"[full] c3 decay rate {rate:.4e} vs {want:.4e}", flush=True)
+    assert abs(rate - want) <= 0.05 * abs(want), (rate, want)
+
+
 def _tgv_torch(X, Y, Z, Ma, gamma=1.4, P0=1.0, rho0=1.0, R=286.9, drift=(0.02, 0.015, 0.01)):
     import torch
     U0 = Ma * (gamma * P0 / rho0) ** 0.5

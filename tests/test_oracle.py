@@ -246,6 +246,101 @@ def test_connectivity_brute_force(D, n, order, perturb):
         assert np.abs(_wrap(pts[0] - pts[1], L)).max() < 1e-12
 
 
+# ---------------------------------------------------------------- face-slot rule
+# Which side of a face is slot 0, derived here by brute force from the mesh file alone
+# (a reader of the test's own, no oracle or product code): src/geometry.cpp:1430-1582 walks
+# the elements in file order and, within each, its faces in the order of face_nodesBT
+# (src/geometry.cpp:218-265); the first (element, face) to meet a face takes slot 0 and the
+# second slot 1, a periodic face counting as met when its partner was (:1459-1463); and
+# orient_fpts (src/solver.cpp:312-437) renumbers the global points without touching the slots.
+# The slot decides whose gradient the one-sided LDG flux takes, so it changes residuals.
+FACE_NODES = {2: [(0, 1), (1, 2), (2, 3), (3, 0)],
+              3: [(0, 1, 2, 3), (5, 4, 7, 6), (0, 3, 7, 4), (2, 1, 5, 6), (1, 0, 4, 5), (3, 2, 6, 7)]}
+
+
+def _read_gmsh_volume(path, D):
+    """Node coordinates and volume elements (linear quads, type 3, in 2D; hexes, type 5, in
+    3D) of a Gmsh 2.2 ASCII file, in file order."""
+    with open(path) as f:
+        lines = [ln.strip() for ln in f]
+    i = lines.index("$Nodes")
+    nn = int(lines[i + 1])
+    ids, X = {}, []
+    for k in range(nn):
+        t = lines[i + 2 + k].split()
+        ids[int(t[0])] = k
+        X.append([float(v) for v in t[1:1 + D]])
+    i = lines.index("$Elements")
+    ne = int(lines[i + 1])
+    want, nv = (3, 4) if D == 2 else (5, 8)
+    eles = []
+    for k in range(ne):
+        t = [int(v) for v in lines[i + 2 + k].split()]
+        if t[1] == want:
+            ntag = t[2]
+            eles.append([ids[n] for n in t[3 + ntag:3 + ntag + nv]])
+    return np.array(X), eles
+
+
+def brute_force_slots(path, D, first_is_slot0=True):
+    """slot[face q][element] by the first-incidence rule over (element, local face)."""
+    X, eles = _read_gmsh_volume(path, D)
+    keys = [[tuple(sorted(e[n] for n in fn)) for fn in FACE_NODES[D]] for e in eles]
+    inc = {}
+    for e, ks in enumerate(keys):
+        for q, k in enumerate(ks):
+            inc.setdefault(k, []).append((e, q))
+    bnd = [k for k, v in inc.items() if len(v) == 1]
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    cen = {k: X[list(k)].mean(axis=0) for k in bnd}
+    partner = {}
+    for k in bnd:  # the periodic partner: the same centroid across the box along one axis
+        c = cen[k]
+        d = next(a for a in range(D) if np.ptp(X[list(k), a]) < 1e-9 and
+                 (abs(c[a] - lo[a]) < 1e-9 or abs(c[a] - hi[a]) < 1e-9))
+        hits = [k2 for k2 in bnd if k2 != k and abs(abs(cen[k2][d] - c[d]) - (hi[d] - lo[d])) < 1e-9 and
+                all(abs(cen[k2][a] - c[a]) < 1e-9 for a in range(D) if a != d)]
+        assert len(hits) == 1, (k, hits)
+        partner[k] = hits[0]
+    group = {k: (min(k, partner[k]) if k in partner else k) for k in inc}
+    order = [(e, q) for e in range(len(eles)) for q in range(2 * D)]
+    if not first_is_slot0:
+        order = order[::-1]
+    seen, slot = set(), np.zeros((2 * D, len(eles)), np.int8)
+    for e, q in order:
+        g = group[keys[e][q]]
+        slot[q, e] = 1 if g in seen else 0
+        seen.add(g)
+    return slot
+
+
+def _slot_meshes():
+    quadbox = tmp("QuadBox_4x4_slots.msh")
+    with open(os.path.join(HERE, "golden", "QuadBox_4x4.msh")) as f, open(quadbox, "w") as g:
+        g.write(f.read().replace('"FARFIELD"', '"PERIODIC"'))
+    cases = [("QuadBox_4x4", quadbox, 2, 3),
+             ("HexBox", os.path.join(HERE, "golden", "HexBox.msh"), 3, 2),
+             ("box3x3x3", make_mesh("slots333.msh", (3, 3, 3), (0.0,) * 3, (TWO_PI,) * 3), 3, 1),
+             ("box2x3x4_curved", make_mesh("slots234.msh", (2, 3, 4), (0.0, -1.0, 0.5), (1.0, 1.0, 2.0), 0.03), 3, 2),
+             ("quad5x3_curved", make_mesh("slots53.msh", (5, 3), (0.0, 0.0), (1.0, 2.0), 0.04), 2, 4)]
+    return [pytest.param(*c, id=c[0]) for c in cases]
+
+
+@pytest.mark.parametrize("name,path,D,order", _slot_meshes())
+def test_face_slot_rule_brute_force(name, path, D, order):
+    """The oracle's slot of every element flux point equals the first-incidence rule derived
+    from the mesh file, periodic faces included; the swapped rule (last incidence takes slot
+    0) is rejected by the same comparison."""
+    o = Oracle(path, D, order, make_phys(viscous=True))
+    _, s = o.maps()
+    nFF = o.nFpts // (2 * D)
+    want = np.repeat(brute_force_slots(path, D), nFF, axis=0)  # every point of a face shares its slot
+    np.testing.assert_array_equal(s, want)
+    swapped = np.repeat(brute_force_slots(path, D, first_is_slot0=False), nFF, axis=0)
+    assert (swapped != s).any()
+    assert (s == 0).sum() == (s == 1).sum()  # periodic box: every face has two sides
+
+
 # ---------------------------------------------------------------- pointwise fluxes
 def _analytic_point(D, seed):
     """Primitive field at a point with analytic gradients -> (U, dU, dprim)."""
@@ -378,6 +473,32 @@ def test_residual_norms(res_type):
     want = {0: np.abs(r).max(axis=(0, 2)), 1: np.abs(r).sum(axis=(0, 2)),
             2: np.sqrt((r * r).sum(axis=(0, 2)))}[res_type] / n
     np.testing.assert_allclose(o.report_residuals(), want, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- extended precision
+@pytest.mark.parametrize("D", [2, 3])
+def test_long_double_oracle_is_the_same_algorithm_with_more_digits(D):
+    """libzefr_oracle_ld.so is the oracle's source built with a 64-bit significand.  On a
+    curved mesh, freestream preservation holds to its own rounding (about 2^-11 of the FP64
+    build's), and on a well-conditioned state the two builds agree to FP64 rounding: it
+    evaluates the same discretisation on the same double inputs, only more precisely."""
+    n = (4, 3) if D == 2 else (3, 2, 2)
+    mesh = make_mesh(f"ld_free{D}.msh", n, (0.0,) * D, (1.0,) * D, 0.05)
+    out = {}
+    for prec in ("double", "long double"):
+        o = Oracle(mesh, D, 3, make_phys(viscous=True, mu=1e-2), precision=prec)
+        U = np.zeros((o.nSpts, o.nVars, o.nEles))
+        U[:, 0], U[:, 1], U[:, 2], U[:, -1] = 1.0, 0.3, -0.2, 2.6
+        o.set_U(U)
+        o.compute_residual(0)
+        free = np.abs(o.divF(0)).max()
+        o.set_U(smooth_state(o.coord_spts(), D))
+        o.compute_residual(0)
+        out[prec] = (free, o.divF(0).copy())
+    assert out["double"][0] > 1e-17 and out["long double"][0] < 1e-17
+    a, b = out["double"][1], out["long double"][1]
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+    assert np.abs(a - b).max() > 0  # different rounding, so really a second evaluation
 
 
 # ---------------------------------------------------------------- planted mistakes

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restri
   extern __shared__ double lds[];
   double *lops = lds;
   double *lW = lops + S::NOPS_PAD;
-  const long e0 = (long)blockIdx.x * a.EB;
+  const long e0 = xcd_block(blockIdx.x, gridDim.x) * a.EB;
   const long neb = min((long)a.EB, (long)a.E - e0);
   for (int i = threadIdx.x; i < S::NOPS; i += blockDim.x) lops[i] = a.ops[i];
   stage_block<D, N>(w, lW, e0, neb);
@@ -352,7 +352,7 @@ template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
 void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
                 double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
-  adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, blockIdx.x);
+  adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // The same kernel for workgroups of one wave (the viscous adjoint runs one element per
@@ -361,7 +361,7 @@ template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_adj_main_w64(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
                     double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
-  adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, blockIdx.x);
+  adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // v += E^T (the owner's cotangent of this element's trace) at the points the partner owns.
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_gather(const double *__rest
   double *lT = lops + S::NOPS_PAD;
   const int tid = threadIdx.x, NT = blockDim.x;
   for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const long e0 = (long)blockIdx.x * EB;
+  const long e0 = xcd_block(blockIdx.x, gridDim.x) * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   const int el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;

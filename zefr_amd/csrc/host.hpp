@@ -121,7 +121,8 @@ class Elements {
   zfr_ele_desc_t desc(const zefr_input_t &in, int phase = -1) const;
   int tile_phase[3] = {0, 0, 0};  // per-launch tiles picked by FRSolver::autotune (0 = tile)
   int tile_for(int phase) const { return (phase >= 0 && phase < 3 && tile_phase[phase] > 0) ? tile_phase[phase] : tile; }
-  int tile_key() const { return tile + 64 * (tile_phase[0] + 64 * (tile_phase[1] + 64 * tile_phase[2])); }
+  // the four tile settings a captured launch sequence bakes in (compared element-wise)
+  std::array<int, 4> tile_key() const { return {tile, tile_phase[0], tile_phase[1], tile_phase[2]}; }
   size_t state_size() const { return (size_t)nEles * nVars * nSpts; }
 };
 
@@ -210,7 +211,7 @@ class FRSolver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double graph_dt = 0.0;
-  int graph_tile = -1;
+  std::array<int, 4> graph_tile = {-1, -1, -1, -1};
   bool graph_fused = false;
   const double *graph_source = nullptr;
   const double *rk_source = nullptr;  // source term of the step in progress

@@ -1,1035 +1,10 @@
-// kernels.hip -- HIP kernels (gfx950) for the DFR residual hot path of ZEFR on quads and hexes.
-//
-// Design (MI355X-first, not a translation of the reference's kernel set):
-//  * Element-major storage: element e owns the contiguous block U[e][var][spt].  A
-//    workgroup takes EB consecutive elements, so its global loads are one contiguous
-//    chunk, staged into LDS.
-//  * The reference's constant-operator GEMMs (GiMMiK, src/gimmik_gpu.cu) are replaced by
-//    sum-factorised line contractions: every DFR operator on tensor-product elements is
-//    a 1D operator applied along the lines of the element (oppE -> lE, oppD -> Dss + Dsf,
-//    oppDiv -> the same, oppDiv_fpts -> signed Dsf).  The 1D tables are staged in LDS.
-//  * Face coupling is element-centric: each element reads its partner's trace through an
-//    integer map (no pointer tables), and computes the common solution / common flux for
-//    its own side of the face.  Both sides evaluate the same formula on the same L/R
-//    states, so the interface flux stays single-valued.
-//  * Navier-Stokes with a general LDG beta needs two grid-wide dependencies (U traces,
-//    then gradient traces), so a residual is three launches: extrapolate -> gradient ->
-//    divergence.  The divergence kernel recomputes the corrected gradient instead of
-//    storing it: re-reading U and the U trace costs less HBM traffic than writing and
-//    re-reading the gradient.
-//  * With LDG beta = +-1/2 (the usual setting) only one side's gradient enters each
-//    common flux and the common solution is the other side's trace, so the "owner" side
-//    can finish the whole common flux right after its own gradient, and the only other-
-//    side data it needs (the partner's trace at its owned points) it extrapolates from the
-//    partner's U itself: gradient_flux (gradient, volume divergence, owned common fluxes)
-//    -> face_correction, two launches.  No gradient trace is stored at all.
-//  * Euler: divergence alone, extrapolating the partner traces from the partner's U (with
-//    a fused RK update: extrapolate -> divergence, since the update overwrites U).
-//
-// All arithmetic is FP64.  Reference call sites are cited at each kernel.
+// kernels.hip -- shape-independent kernels (RK updates, p-multigrid transfers, residual
+// norm, stream copy) and the kernel-level C entry points of zefr_amd.h.  The residual
+// kernels themselves live in kernels.hpp and are launched from kernels_shape.hip.
 
-#include <hip/hip_runtime.h>
-
-#include <cstdint>
-#include <cstdio>
-#include <map>
-#include <string>
-#include <tuple>
-#include <vector>
-
-#include "zefr_amd.h"
-#include "common.hpp"
-#include "physics.hpp"
+#include "kernels.hpp"
 
 namespace zk {
-
-constexpr int kMaxThreads = 256;
-
-template <int D, int N>
-struct Shape {
-  static constexpr int nS = (D == 2) ? N * N : N * N * N;
-  static constexpr int nFF = (D == 2) ? N : N * N;
-  static constexpr int nF = 2 * D * nFF;
-  static constexpr int V = D + 2;
-  static constexpr int nM = D * D + 1;
-  static constexpr int FPT = (nF + nS - 1) / nS;  // flux points per thread
-  static constexpr int NOPS = 2 * N + N * N + 2 * N;
-  static constexpr int NOPS_PAD = (NOPS + 7) & ~7;
-  static constexpr int defaultEB() { return (kMaxThreads / nS) > 0 ? kMaxThreads / nS : 1; }
-};
-
-// Offsets of the 1D tables inside ops1d.
-template <int N> __device__ __forceinline__ const double *tab_lE(const double *o, int side) { return o + side * N; }
-template <int N> __device__ __forceinline__ const double *tab_Dss(const double *o, int i) { return o + 2 * N + i * N; }
-template <int N> __device__ __forceinline__ const double *tab_Dsf(const double *o, int i) { return o + 2 * N + N * N + 2 * i; }
-
-struct KArgs {
-  int E, EB, viscous;
-  int fix_vis;
-  double gamma, prandtl, mu, rt, c_sth;
-  double rus_k, ldg_b, ldg_tau;
-  const double *ops;
-  const double *metric;
-  const double *fgeo;
-  const unsigned int *nbr;
-  double *Utr;
-  double *Gtr;
-  double *Ftr;
-  // k_extrapolate: 0 = the trace at every flux point; 1 = only where the partner owns the
-  // face (the one-sided LDG path reads nothing else)
-  int trace_sel;
-  // RK update fused into the last kernel of a residual (mode 0: none, 1: stage, 2: final)
-  struct Rk {
-    double *U;
-    const double *Uini, *divF0;  // divF0: stage 0 of the [nStages][E][V][nS] residuals
-    const double *src;           // p-multigrid source term added to every divF_s, or null
-    long total;
-    double alpha, dt, beta[4];
-    int nStages, stage, mode;
-    double *trace;  // mode 1: also write the new state's trace here (the next stage's Utr), or null
-  } rk;
-};
-
-// One case of a multi-case launch: its kernel arguments and arrays.  A multi-case launch
-// runs the same kernel over several independent solvers of one shape in one grid:
-// workgroup b does block b - starts[c] of case c, where starts[c] <= b < starts[c + 1].
-struct MultiCase {
-  KArgs a;
-  const double *U;
-  double *out;
-};
-
-// The case of workgroup b: binary search over the cases' first workgroups (uniform across
-// the workgroup, so the loads are scalar).
-__device__ __forceinline__ int find_case(const long *__restrict__ starts, int nc, long b) {
-  int lo = 0, hi = nc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (starts[mid] <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// Flux point f of the reference ordering (src/quads.cpp:86-123, src/hexas.cpp:133-194):
-// normal direction, side (0: the -1 end, 1: the +1 end), first spt of the line through
-// it along 'dim', and that line's spt stride.
-template <int D, int N>
-__device__ __forceinline__ void fpt_info(int f, int &dim, int &side, int &base, int &str) {
-  if constexpr (D == 2) {
-    const int face = f / N, j = f - face * N;
-    switch (face) {
-      case 0: dim = 1; side = 0; base = j; break;
-      case 1: dim = 0; side = 1; base = N * j; break;
-      case 2: dim = 1; side = 1; base = N - 1 - j; break;
-      default: dim = 0; side = 0; base = N * (N - 1 - j); break;
-    }
-    str = (dim == 0) ? 1 : N;
-  } else {
-    constexpr int NN = N * N;
-    const int face = f / NN, r = f - face * NN, j = r / N, k = r - j * N;
-    switch (face) {
-      case 0: dim = 2; side = 0; base = k + N * j; break;
-      case 1: dim = 2; side = 1; base = (N - 1 - k) + N * j; break;
-      case 2: dim = 0; side = 0; base = N * k + NN * j; break;
-      case 3: dim = 0; side = 1; base = N * (N - 1 - k) + NN * j; break;
-      case 4: dim = 1; side = 0; base = (N - 1 - k) + NN * j; break;
-      default: dim = 1; side = 1; base = k + NN * j; break;
-    }
-    str = (dim == 0) ? 1 : ((dim == 1) ? N : NN);
-  }
-}
-
-// The flux point at the 'side' end of the line through spt (a, b, c) along 'dim'.
-template <int D, int N>
-__device__ __forceinline__ int fpt_end(int a, int b, int c, int dim, int side) {
-  if constexpr (D == 2) {
-    if (dim == 0) return side ? (N + b) : (3 * N + (N - 1 - b));
-    return side ? (2 * N + (N - 1 - a)) : a;
-  } else {
-    constexpr int NN = N * N;
-    if (dim == 0) return side ? (3 * NN + c * N + (N - 1 - b)) : (2 * NN + c * N + b);
-    if (dim == 1) return side ? (5 * NN + c * N + a) : (4 * NN + c * N + (N - 1 - a));
-    return side ? (NN + b * N + (N - 1 - a)) : (b * N + a);
-  }
-}
-
-// ---------------------------------------------------------------- pointwise physics
-// primitives, conv_proj, viscous_dir, visc_proj_sub, visc_normal, common_flux and their
-// reverse-mode counterparts live in physics.hpp (shared with a host build).
-using namespace zphys;
-
-__device__ __forceinline__ void unpack_nbr(unsigned int p, long &ep, int &slot, int &fp) {
-  ep = (long)(p >> 9);
-  slot = (int)((p >> 8) & 1u);
-  fp = (int)(p & 255u);
-}
-
-// Which sides' gradients enter the LDG flux (src/faces.cpp:1520-1545).
-__device__ __forceinline__ bool need_left_grad(double beta) { return beta != -0.5; }
-__device__ __forceinline__ bool need_right_grad(double beta) { return beta != 0.5; }
-
-// Stage EB consecutive elements' U blocks and the 1D tables into LDS.
-template <int D, int N>
-__device__ __forceinline__ void stage_U(const double *__restrict__ U, const KArgs &a, double *lops, double *lU,
-                                        long e0, long neb, int Ustride) {
-  using S = Shape<D, N>;
-  const int tid = threadIdx.x, NT = blockDim.x;
-  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const double *Ub = U + e0 * S::V * S::nS;
-  for (long i = tid; i < neb * S::V * S::nS; i += NT) {
-    const long el = i / (S::V * S::nS), r = i - el * (S::V * S::nS);
-    lU[el * Ustride + r] = Ub[i];
-  }
-}
-
-// Value of var v along the line (base, str) of a [var][spt] block, extrapolated to one end.
-template <int N>
-__device__ __forceinline__ double line_extrap(const double *lE, const double *blk, int base, int str) {
-  double s = 0.0;
-#pragma unroll
-  for (int m = 0; m < N; m++) s += lE[m] * blk[base + m * str];
-  return s;
-}
-
-// The partner's trace at its flux point fp (all V variables), extrapolated from the
-// partner's solution line in global memory with the 1D table (global copy of ops1d): what
-// k_extrapolate writes there, for kernels that run without a trace launch before them.
-template <int D, int N>
-__device__ __forceinline__ void partner_trace(const double *__restrict__ U, const double *ops, long ep, int fp,
-                                              double *out) {
-  using S = Shape<D, N>;
-  int dim, side, base, str;
-  fpt_info<D, N>(fp, dim, side, base, str);
-  const double *lE = tab_lE<N>(ops, side);
-  const double *Up = U + ep * S::V * S::nS + base;
-#pragma unroll
-  for (int v = 0; v < S::V; v++) {
-    double s = 0.0;
-#pragma unroll
-    for (int m = 0; m < N; m++) s += lE[m] * Up[v * S::nS + m * str];
-    out[v] = s;
-  }
-}
-
-// RK update of one spt straight after its residual, in the last kernel of the residual
-// (the stage update of src/solver.cpp:2897-2941 and the final combination of
-// :2997-3062, with the same arithmetic as k_rk_stage / k_rk_final).  acc holds this
-// stage's divF at the spt; the other stages' come from memory.  With a source term
-// (src/solver.cpp:2946-2962, 3061-3073) every divF_s is replaced by divF_s + src.
-template <int D>
-__device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc,
-                                            double *unew = nullptr) {
-  constexpr int V = D + 2, nM = D * D + 1;
-  if (a.rk.mode == 0) return;
-  const double detJ = a.metric[(e * nM + D * D) * nS + t];
-#pragma unroll
-  for (int v = 0; v < V; v++) {
-    const long i = (e * V + v) * nS + t;
-    if (a.rk.mode == 1) {
-      const double r = a.rk.src ? acc[v] + a.rk.src[i] : acc[v];
-      const double u = a.rk.Uini[i] - a.rk.alpha * a.rk.dt / detJ * r;
-      a.rk.U[i] = u;
-      if (unew) unew[v] = u;
-    } else {
-      double u = a.rk.Uini[i];
-      for (int s = 0; s < a.rk.nStages; s++) {
-        double r = (s == a.rk.stage) ? acc[v] : a.rk.divF0[s * a.rk.total + i];
-        if (a.rk.src) r += a.rk.src[i];
-        u -= a.rk.beta[s] * a.rk.dt / detJ * r;
-      }
-      a.rk.U[i] = u;
-    }
-  }
-}
-
-// After a fused RK stage update: the trace of the new state at this element's flux points,
-// written to a.rk.trace ([E][V][nF], the next stage's Utr), so that the next stage's
-// extrapolation launch is skipped.  owner < 0 writes every point (k_extrapolate with
-// trace_sel 0); otherwise only the points whose partner owns the face (trace_sel 1 with
-// that owner slot).  The new state goes through the element's LDS block w (V * nS
-// doubles), which no thread may still read; every thread of the workgroup calls this (it
-// holds barriers).
-template <int D, int N>
-__device__ __forceinline__ void rk_trace(const KArgs &a, const double *lops, double *w, long e, int t, bool valid,
-                                         const double *unew, int owner) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  __syncthreads();
-  if (valid) {
-#pragma unroll
-    for (int v = 0; v < V; v++) w[v * nS + t] = unew[v];
-  }
-  __syncthreads();
-  if (!valid) return;
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    if (f >= nF) continue;
-    if (owner >= 0 && (int)((a.nbr[e * nF + f] >> 8) & 1u) == owner) continue;
-    int dim, side, base, str;
-    fpt_info<D, N>(f, dim, side, base, str);
-    const double *lE = tab_lE<N>(lops, side);
-#pragma unroll
-    for (int v = 0; v < V; v++) a.rk.trace[(e * V + v) * nF + f] = line_extrap<N>(lE, w + v * nS, base, str);
-  }
-}
-
-// The k-th owned flux point, owned faces taken in increasing order (mask bit q = face q).
-template <int D, int N>
-__device__ __forceinline__ int owned_fpt(unsigned mask, int k) {
-  using S = Shape<D, N>;
-  int q = k / S::nFF;
-  const int r = k - q * S::nFF;
-#pragma unroll
-  for (int b = 0; b < 2 * D; b++) {
-    if ((mask >> b) & 1u) {
-      if (q == 0) return b * S::nFF + r;
-      q--;
-    }
-  }
-  return 0;  // not reached for k < popcount(mask) * nFF
-}
-
-// ---------------------------------------------------------------- kernel 1: traces
-// Elements::extrapolate_U (src/elements.cpp:1337-1360): U_fpts = oppE U_spts.  With
-// trace_sel = 1 only the points of faces the partner owns are written: on the one-sided
-// LDG path the owner reads the other side's trace and nothing reads its own.
-template <int D, int N>
-__device__ __forceinline__ void extrapolate_block(const double *__restrict__ U, const KArgs &a, long bid) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  extern __shared__ double lds[];
-  double *lops = lds;
-  double *lU = lds + S::NOPS_PAD;
-  const long e0 = bid * a.EB;
-  const long neb = min((long)a.EB, (long)a.E - e0);
-  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  // this thread's slots, fetched before the staging barrier so the loads overlap it
-  int slot[S::FPT];
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    slot[k] = (a.trace_sel && el < neb && f < nF) ? (int)((a.nbr[e * nF + f] >> 8) & 1u) : -1;
-  }
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  __syncthreads();
-  if (el >= neb) return;
-  const double *u = lU + el * V * nS;
-  const int owner = (a.ldg_b == 0.5) ? 0 : 1;
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    if (f < nF) {
-      if (a.trace_sel && slot[k] == owner) continue;
-      int dim, side, base, str;
-      fpt_info<D, N>(f, dim, side, base, str);
-      const double *lE = tab_lE<N>(lops, side);
-#pragma unroll
-      for (int v = 0; v < V; v++) a.Utr[(e * V + v) * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
-    }
-  }
-}
-
-template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__restrict__ U, KArgs a) {
-  extrapolate_block<D, N>(U, a, blockIdx.x);
-}
-
-// The same kernel for workgroups of one wave (see k_visc_onesided_w64).
-template <int D, int N>
-__global__ __launch_bounds__(64) void k_extrapolate_w64(const double *__restrict__ U, KArgs a) {
-  extrapolate_block<D, N>(U, a, blockIdx.x);
-}
-
-template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_extrapolate_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
-  const int c = find_case(starts, nc, blockIdx.x);
-  extrapolate_block<D, N>(cs[c].U, cs[c].a, (long)blockIdx.x - starts[c]);
-}
-
-// LDG common solution at this thread's flux points (src/faces.cpp:1366-1474):
-// Ucomm = 1/2 (UL + UR) - beta (UL - UR), with the beta = +-1/2 one-sided branches.
-template <int D, int N>
-__device__ __forceinline__ void common_solution(const KArgs &a, const double *lops, const double *u, double *lC,
-                                                long e, int t) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  const double beta = a.ldg_b;
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    if (f >= nF) continue;
-    int dim, side, base, str;
-    fpt_info<D, N>(f, dim, side, base, str);
-    const double *lE = tab_lE<N>(lops, side);
-    long ep; int sl, fp;
-    unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      const double own = line_extrap<N>(lE, u + v * nS, base, str);
-      const double par = a.Utr[(ep * V + v) * nF + fp];
-      const double uL = sl ? par : own, uR = sl ? own : par;
-      const double l = (beta == 0.5) ? 0.0 : uL;
-      const double r = (beta == -0.5) ? 0.0 : uR;
-      lC[v * nF + f] = 0.5 * (l + r) - beta * (l - r);
-    }
-  }
-}
-
-// Corrected reference-space gradient at spt t (oppD U + oppD_fpts Ucomm), mapped to
-// physical space with the adjugate metric (src/elements.cpp:1393-1447, 1630-1650).
-template <int D, int N>
-__device__ __forceinline__ void spt_gradient(const double *lops, const double *u, const double *lC, int t,
-                                             const double ij[][D], double detJ, double g[][D]) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-  double gr[D][V];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-    const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
-    const int base = t - id * str;
-    const double *Dr = tab_Dss<N>(lops, id);
-    const double *Df = tab_Dsf<N>(lops, id);
-    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; m++) s += Dr[m] * u[v * nS + base + m * str];
-      s += Df[0] * lC[v * nF + f0];
-      s += Df[1] * lC[v * nF + f1];
-      gr[d][v] = s;
-    }
-  }
-  const double inv_det = 1.0 / detJ;
-#pragma unroll
-  for (int v = 0; v < V; v++)
-#pragma unroll
-    for (int d1 = 0; d1 < D; d1++) {
-      double s = 0.0;
-#pragma unroll
-      for (int d2 = 0; d2 < D; d2++) s += gr[d2][v] * ij[d2][d1];
-      g[v][d1] = s * inv_det;
-    }
-}
-
-template <int D>
-__device__ __forceinline__ void load_metric(const KArgs &a, long e, int t, int nS, double ij[][D], double &detJ,
-                                            bool want_det) {
-  constexpr int nM = D * D + 1;
-  const double *m = a.metric + e * nM * nS + t;
-#pragma unroll
-  for (int d1 = 0; d1 < D; d1++)
-#pragma unroll
-    for (int d2 = 0; d2 < D; d2++) ij[d1][d2] = m[(d1 * D + d2) * nS];
-  detJ = want_det ? m[D * D * nS] : 1.0;
-}
-
-// ---------------------------------------------------------------- kernel 2: gradient
-// Common solution + corrected gradient + physical transform + gradient extrapolation to
-// the flux points whose gradient the partner element needs.  LDS per element holds U and
-// Ucomm, then (aliased) the physical gradient.
-template <int D, int N>
-__device__ __forceinline__ void gradient_block(const double *__restrict__ U, double *__restrict__ dU_out,
-                                               const KArgs &a, long bid) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  constexpr int W = (V * nS + V * nF > D * V * nS) ? V * nS + V * nF : D * V * nS;
-  extern __shared__ double lds[];
-  double *lops = lds;
-  double *lW = lds + S::NOPS_PAD;
-  const long e0 = bid * a.EB;
-  const long neb = min((long)a.EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lW, e0, neb, W);
-  __syncthreads();
-  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  const bool valid = el < neb;
-  double *u = lW + el * W;
-  double *c = u + V * nS;
-  if (valid) common_solution<D, N>(a, lops, u, c, e, t);
-  __syncthreads();
-  double g[V][D];
-  if (valid) {
-    double ij[D][D], detJ;
-    load_metric<D>(a, e, t, nS, ij, detJ, true);
-    spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
-    if (dU_out) {
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int v = 0; v < V; v++) dU_out[((e * D + d) * V + v) * nS + t] = g[v][d];
-    }
-  }
-  __syncthreads();  // U and Ucomm are dead: the region now holds the gradient
-  if (valid) {
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int v = 0; v < V; v++) u[(d * V + v) * nS + t] = g[v][d];
-  }
-  __syncthreads();
-  if (!valid) return;
-  const double beta = a.ldg_b;
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    if (f >= nF) continue;
-    const int sl = (int)((a.nbr[e * nF + f] >> 8) & 1u);
-    // the partner needs this side's gradient iff this side is L and dUL is used, or R and dUR is used
-    const bool write = sl ? need_right_grad(beta) : need_left_grad(beta);
-    if (!write) continue;
-    int dim, side, base, str;
-    fpt_info<D, N>(f, dim, side, base, str);
-    const double *lE = tab_lE<N>(lops, side);
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int v = 0; v < V; v++)
-        a.Gtr[((e * D + d) * V + v) * nF + f] = line_extrap<N>(lE, u + (d * V + v) * nS, base, str);
-  }
-}
-
-template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restrict__ U, double *__restrict__ dU_out,
-                                                          KArgs a) {
-  gradient_block<D, N>(U, dU_out, a, blockIdx.x);
-}
-
-template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_gradient_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
-  const int c = find_case(starts, nc, blockIdx.x);
-  gradient_block<D, N>(cs[c].U, nullptr, cs[c].a, (long)blockIdx.x - starts[c]);
-}
-
-// ---------------------------------------------------------------- kernel 3: divergence
-// compute_F + compute_divF_spts + compute_common_F + compute_divF_fpts.
-// Phases (viscous): U -> LDS | Ucomm -> lC | gradient -> lG | common flux -> lC |
-// transformed flux -> lG | divergence.  Inviscid: U -> LDS | flux + common flux | divergence.
-// SELF (Euler only): the partner traces are extrapolated here from the partner's U, so an
-// Euler residual is this one launch; otherwise they are read from Utr.
-template <int D, int N, bool VISC, int BM, int FC, bool SELF>
-__device__ __forceinline__ void divergence_block(const double *__restrict__ U, double *__restrict__ divF,
-                                                 const KArgs &a, long bid) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  extern __shared__ double lds[];
-  const int EB = a.EB;
-  double *lops = lds;
-  double *lU = lops + S::NOPS_PAD;
-  double *lC = lU + EB * V * nS;
-  double *lG = lC + EB * V * nF;  // gradient (viscous), then the transformed flux
-  const long e0 = bid * EB;
-  const long neb = min((long)EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  __syncthreads();
-  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  const bool valid = el < neb;
-  const double *u = lU + el * V * nS;
-  double *c = lC + el * V * nF;
-  double *g_l = lG + el * D * V * nS;
-  if (VISC) {
-    if (valid) common_solution<D, N>(a, lops, u, c, e, t);
-    __syncthreads();
-    if (valid) {
-      double ij[D][D], detJ, g[V][D];
-      load_metric<D>(a, e, t, nS, ij, detJ, true);
-      spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
-    }
-    __syncthreads();
-  }
-
-  // common flux at this thread's flux points -> lC (Ucomm has been consumed)
-  if (valid) {
-#pragma unroll 1
-    for (int k = 0; k < S::FPT; k++) {
-      const int f = t + k * nS;
-      if (f >= nF) continue;
-      int dim, side, base, str;
-      fpt_info<D, N>(f, dim, side, base, str);
-      const double *lE = tab_lE<N>(lops, side);
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      const double *fg = a.fgeo + e * (D + 1) * nF + f;
-      double n[D];
-#pragma unroll
-      for (int d = 0; d < D; d++) n[d] = fg[d * nF];
-      const double dAs = fg[D * nF];
-      // own side's state / gradient from LDS, the partner's from its traces
-      auto own_state = [&](double *Us) {
-#pragma unroll
-        for (int v = 0; v < V; v++) Us[v] = line_extrap<N>(lE, u + v * nS, base, str);
-      };
-      auto par_state = [&](double *Us) {
-        if constexpr (SELF) {
-          partner_trace<D, N>(U, a.ops, ep, fp, Us);
-        } else {
-#pragma unroll
-          for (int v = 0; v < V; v++) Us[v] = a.Utr[(ep * V + v) * nF + fp];
-        }
-      };
-      // viscous normal fluxes first (one side's state + gradient live at a time):
-      // G1 = left (BM 0, 1) or right (BM 2) side, G2 = right side (BM 0)
-      double G1[VISC ? V : 1], G2[(VISC && BM == 0) ? V : 1];
-      if (VISC) {
-        auto side_visc = [&](bool own, double *G) {
-          double Us[V];
-          if (own) {
-            own_state(Us);
-            visc_normal<D>(Us, [&](int j, double *dUj) {
-#pragma unroll
-              for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
-            }, n, a, G);
-          } else {
-            par_state(Us);
-            visc_normal<D>(Us, [&](int j, double *dUj) {
-#pragma unroll
-              for (int v = 0; v < V; v++) dUj[v] = a.Gtr[((ep * D + j) * V + v) * nF + fp];
-            }, n, a, G);
-          }
-        };
-        if (BM == 0) {
-          side_visc(sl == 0, G1);
-          side_visc(sl == 1, G2);
-        } else {
-          side_visc((BM == 1) ? (sl == 0) : (sl == 1), G1);
-        }
-      }
-      double UL[V], UR[V];
-      if (sl) { par_state(UL); own_state(UR); } else { own_state(UL); par_state(UR); }
-      double Fo[V];
-      common_flux<D, VISC, BM, FC>(UL, UR, G1, G2, n, dAs, a, Fo);
-#pragma unroll
-      for (int v = 0; v < V; v++) c[v * nF + f] = Fo[v];
-    }
-  }
-  if (VISC) __syncthreads();  // the gradient field in lG is consumed
-
-  // pointwise flux at the spt, contracted with the adjugate rows (src/elements.cpp:1588-1699);
-  // each thread reads and overwrites only its own spt column of lG
-  if (valid) {
-    double ij[D][D], detJ;
-    load_metric<D>(a, e, t, nS, ij, detJ, false);
-    double Us[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
-    const Prim<D> p = primitives<D>(Us, a.gamma);
-    double tF[D][V];
-#pragma unroll
-    for (int d1 = 0; d1 < D; d1++) conv_proj<D>(Us, p, ij[d1], tF[d1]);
-    if (VISC) {
-      double g[V][D];
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int v = 0; v < V; v++) g[v][d] = g_l[(d * V + v) * nS + t];
-      const Visc<D> vs = viscous<D>(Us, p, g, a);
-#pragma unroll
-      for (int d1 = 0; d1 < D; d1++) visc_proj_sub<D>(vs, ij[d1], tF[d1]);
-    }
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[d][v];
-  }
-  __syncthreads();
-  double unew[V];
-  if (valid) {
-    // divergence: oppDiv tF + oppDiv_fpts Fcomm (src/elements.cpp:1449-1501)
-    const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-    double acc[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) acc[v] = 0.0;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-      const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
-      const int base = t - id * str;
-      const double *Dr = tab_Dss<N>(lops, id);
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        double s = acc[v];
-#pragma unroll
-        for (int m = 0; m < N; m++) s += Dr[m] * g_l[(d * V + v) * nS + base + m * str];
-        acc[v] = s;
-      }
-    }
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-      const double *Df = tab_Dsf<N>(lops, id);
-      const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        acc[v] += (-Df[0]) * c[v * nF + f0];
-        acc[v] += Df[1] * c[v * nF + f1];
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
-    rk_epilogue<D>(a, e, t, nS, acc, unew);
-  }
-  // the new state's full trace for the next stage, into a buffer other than the Utr this
-  // launch reads; the element's U block is no longer read
-  if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, lU + el * V * nS, e, t, valid, unew, -1);
-}
-
-template <int D, int N, bool VISC, int BM, int FC, bool SELF = !VISC>
-__global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *__restrict__ U, double *__restrict__ divF,
-                                                            KArgs a) {
-  static_assert(!(VISC && SELF), "the viscous divergence reads the gradient traces, hence the traces too");
-  divergence_block<D, N, VISC, BM, FC, SELF>(U, divF, a, blockIdx.x);
-}
-
-template <int D, int N, bool VISC, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) void k_divergence_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
-  const int c = find_case(starts, nc, blockIdx.x);
-  divergence_block<D, N, VISC, BM, FC, !VISC>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
-}
-
-// ---------------------------------------------------------------- one-sided LDG path
-// With LDG beta = +1/2 (BM 1) the common viscous flux uses only the left (slot-0)
-// side's gradient, with beta = -1/2 (BM 2) only the right side's (src/faces.cpp:1520-1545),
-// and the common solution is the other side's state.  The element on that "owner" side
-// therefore has everything for the complete common flux (Rusanov + LDG) right after its
-// own corrected gradient: it writes the unscaled common normal flux to Ftr at its owned
-// flux points, and no gradient trace is needed.  The partner's solution trace at an owned
-// point is extrapolated from the partner's U inside the kernel.  A residual is then
-//   k_visc_onesided (gradient, volume flux divergence, owned common fluxes)
-//   -> k_face_correction (oppDiv_fpts Fcomm, both sides).
-
-// Register budget: for hex p=3 four waves per SIMD (<= 128 VGPRs) is what the kernel's
-// LDS allows per CU and the Rusanov face phase fits it without spilling, so the allocator
-// is held to it.  The Roe face phase needs more: three waves (<= 168 VGPRs) measured
-// fastest (four spill, 2.76 ms; three 1.52 ms; uncapped, two waves, 1.80 ms at 64^3).
-// Other shapes would spill under a cap and keep the default (quads at p=4: three waves,
-// 168 VGPRs with 15 spilled, measured 0.165 ms against 0.128 ms uncapped at 256^2).
-template <int D, int N, int FC>
-struct OneSidedWaves {
-  static constexpr int value = (D == 3 && N == 4) ? (FC == 1 ? 3 : 4) : 1;
-};
-
-// SELF: extrapolate the partner traces this element needs from the partner's U (no trace
-// launch before this kernel; they are parked in Utr at this element's own owned points
-// for the face phase); otherwise read them from Utr at the partner's points (written by a
-// trace launch or by the previous RK stage's update).  A template parameter: as a
-// run-time branch the two prologues together spill the hex p=3 kernel.
-template <int D, int N, int BM, int FC, bool SELF>
-__device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U, double *__restrict__ divF,
-                                                    const KArgs &a, long bid) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  constexpr int owner = (BM == 1) ? 0 : 1;  // slot whose gradient enters the common flux
-  // one LDS region per element holds the common solution, then (after a barrier) the
-  // gradient, then the transformed flux
-  constexpr int W = (V * nF > D * V * nS) ? V * nF : D * V * nS;
-  extern __shared__ double lds[];
-  const int EB = a.EB;
-  double *lops = lds;
-  double *lU = lops + S::NOPS_PAD;
-  double *lW = lU + EB * V * nS;
-  unsigned *lmask = reinterpret_cast<unsigned *>(lW + EB * W);  // owned faces per element
-  const long e0 = bid * EB;
-  const long neb = min((long)EB, (long)a.E - e0);
-  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  const bool valid = el < neb;
-  // the slots of this thread's flux points and, where this element owns the face, the
-  // partner's trace: with SELF extrapolated here from the partner's solution line
-  // (no trace launch runs before this kernel), else read from the partner's Utr.  Issued
-  // before the staging barrier so the loads overlap it.
-  int slot[S::FPT];
-  double cp[S::FPT][V];
-  constexpr bool self = SELF;
-#pragma unroll
-  for (int k = 0; k < S::FPT; k++) {
-    const int f = t + k * nS;
-    slot[k] = -1;
-    if (valid && f < nF) {
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      slot[k] = sl;
-      if (sl == owner) {
-        if (self) {
-          partner_trace<D, N>(U, a.ops, ep, fp, cp[k]);
-        } else {
-#pragma unroll
-          for (int v = 0; v < V; v++) cp[k][v] = a.Utr[(ep * V + v) * nF + fp];
-        }
-      }
-    }
-  }
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
-  __syncthreads();
-  const double *u = lU + el * V * nS;
-  double *c = lW + el * W;
-  double *g_l = lW + el * W;
-  // common solution (= the non-owner side's state) and the mask of owned faces (all flux
-  // points of a face share its slot, so ownership is per face)
-  if (valid) {
-#pragma unroll
-    for (int k = 0; k < S::FPT; k++) {
-      const int f = t + k * nS;
-      if (f >= nF) continue;
-      if (slot[k] == owner) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-          c[v * nF + f] = cp[k][v];
-          if (self) a.Utr[(e * V + v) * nF + f] = cp[k][v];  // own slot: read back in the face phase
-        }
-        if (f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
-      } else {
-        int dim, side, base, str;
-        fpt_info<D, N>(f, dim, side, base, str);
-        const double *lE = tab_lE<N>(lops, side);
-#pragma unroll
-        for (int v = 0; v < V; v++) c[v * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
-      }
-    }
-  }
-  __syncthreads();
-  {
-    double g[V][D];
-    if (valid) {
-      double ij[D][D], detJ;
-      load_metric<D>(a, e, t, nS, ij, detJ, true);
-      spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
-    }
-    __syncthreads();  // the common solution is consumed
-    if (valid) {
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
-    }
-  }
-  __syncthreads();
-  // common normal flux (unscaled) at the owned flux points -> Ftr
-  if (valid) {
-    const unsigned mask = lmask[el];
-    const int cnt = __popc(mask) * S::nFF;
-#pragma unroll 1
-    for (int k = t; k < cnt; k += nS) {
-      const int f = owned_fpt<D, N>(mask, k);  // face-major: neighbouring lanes, neighbouring points
-      int dim, side, base, str;
-      fpt_info<D, N>(f, dim, side, base, str);
-      const double *lE = tab_lE<N>(lops, side);
-      const double *fg = a.fgeo + e * (D + 1) * nF + f;
-      double n[D];
-#pragma unroll
-      for (int d = 0; d < D; d++) n[d] = fg[d * nF];
-      double Uo[V], Up[V];
-#pragma unroll
-      for (int v = 0; v < V; v++) Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
-      if constexpr (FC == 1) {
-        // Roe: the convective flux first, so that its many temporaries are dead before the
-        // gradient's lines are gathered (formed in the Rusanov order it needs ~350 registers)
-        {
-          long ep; int sl, fp;
-          unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-          const long ip = self ? (e * V * nF + f) : (ep * V * nF + fp);
-#pragma unroll
-          for (int v = 0; v < V; v++) Up[v] = a.Utr[ip + v * nF];
-        }
-        double Fo[V], G[V];
-        if (owner == 0) conv_common_flux<D, FC>(Uo, Up, n, a, Fo);
-        else conv_common_flux<D, FC>(Up, Uo, n, a, Fo);
-        asm volatile("" ::: "memory");
-        visc_normal<D>(Uo, [&](int j, double *dUj) {
-#pragma unroll
-          for (int v = 0; v < V; v++) {
-            dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
-            asm volatile("" ::: "memory");
-          }
-        }, n, a, G);
-        if (owner == 0) ldg_common_add<D, BM>(Uo, Up, G, G, a, Fo);
-        else ldg_common_add<D, BM>(Up, Uo, G, G, a, Fo);
-#pragma unroll
-        for (int v = 0; v < V; v++) a.Ftr[(e * V + v) * nF + f] = Fo[v];
-        continue;
-      }
-      double G[V];
-      // one variable's line at a time (a compiler barrier after each): 128 VGPRs instead
-      // of 160, i.e. one more wave per SIMD, worth more than the loads' overlap
-      visc_normal<D>(Uo, [&](int j, double *dUj) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-          dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
-          asm volatile("" ::: "memory");
-        }
-      }, n, a, G);
-      asm volatile("" ::: "memory");  // load the partner's state only now: fewer live registers
-      {
-        long ep; int sl, fp;
-        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-        const long ip = self ? (e * V * nF + f) : (ep * V * nF + fp);  // cached since phase 1
-#pragma unroll
-        for (int v = 0; v < V; v++) Up[v] = a.Utr[ip + v * nF];
-      }
-      double Fo[V];
-      if (owner == 0) common_flux<D, true, BM, FC>(Uo, Up, G, G, n, 1.0, a, Fo);
-      else common_flux<D, true, BM, FC>(Up, Uo, G, G, n, 1.0, a, Fo);
-#pragma unroll
-      for (int v = 0; v < V; v++) a.Ftr[(e * V + v) * nF + f] = Fo[v];
-    }
-  }
-  __syncthreads();  // the gradient field is consumed
-  // pointwise flux contracted with the adjugate rows, over the own spt column of lG
-  if (valid) {
-    double ij[D][D], detJ;
-    load_metric<D>(a, e, t, nS, ij, detJ, false);
-    double Us[V], g[V][D];
-#pragma unroll
-    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int v = 0; v < V; v++) g[v][d] = g_l[(d * V + v) * nS + t];
-    const Prim<D> p = primitives<D>(Us, a.gamma);
-    const Visc<D> vs = viscous<D>(Us, p, g, a);
-    double tF[D][V];
-#pragma unroll
-    for (int d1 = 0; d1 < D; d1++) {
-      conv_proj<D>(Us, p, ij[d1], tF[d1]);
-      visc_proj_sub<D>(vs, ij[d1], tF[d1]);
-    }
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[d][v];
-  }
-  __syncthreads();
-  if (!valid) return;
-  // volume part of the divergence: oppDiv tF (src/elements.cpp:1449-1474)
-  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-  double acc[V];
-#pragma unroll
-  for (int v = 0; v < V; v++) acc[v] = 0.0;
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-    const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
-    const int base = t - id * str;
-    const double *Dr = tab_Dss<N>(lops, id);
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      double s = acc[v];
-#pragma unroll
-      for (int m = 0; m < N; m++) s += Dr[m] * g_l[(d * V + v) * nS + base + m * str];
-      acc[v] = s;
-    }
-  }
-#pragma unroll
-  for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
-}
-
-template <int D, int N, int BM, int FC, bool SELF>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
-void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
-  visc_onesided_block<D, N, BM, FC, SELF>(U, divF, a, blockIdx.x);
-}
-
-// The same kernel for workgroups of one wave (64 threads: one hex p=3 element per
-// workgroup, the tile the autotuner picks there): the compiler knows the workgroup is a
-// single wave, so its barriers need no cross-wave synchronisation.
-template <int D, int N, int BM, int FC, bool SELF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
-void k_visc_onesided_w64(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
-  visc_onesided_block<D, N, BM, FC, SELF>(U, divF, a, blockIdx.x);
-}
-
-template <int D, int N, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
-void k_visc_onesided_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
-  const int c = find_case(starts, nc, blockIdx.x);
-  visc_onesided_block<D, N, BM, FC, true>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
-}
-
-
-// divF += oppDiv_fpts Fcomm (src/elements.cpp:1476-1501) with Fcomm = Ftr(owner) * signed
-// dA of this side; the owner's Ftr is this element's own or its partner's.
-template <int D, int N, int BM>
-__device__ __forceinline__ void face_correction_block(double *__restrict__ divF, const KArgs &a, long bid) {
-  using S = Shape<D, N>;
-  constexpr int nS = S::nS, nF = S::nF, V = S::V;
-  constexpr int owner = (BM == 1) ? 0 : 1;
-  // per-element LDS block: the common fluxes (V * nF), then the new state (V * nS) when the
-  // RK update writes the next trace
-  constexpr int W = V * (nF > nS ? nF : nS);
-  extern __shared__ double lds[];
-  const int EB = a.EB;
-  double *lops = lds;
-  double *lF = lops + S::NOPS_PAD;
-  const int tid = threadIdx.x, NT = blockDim.x;
-  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const long e0 = bid * EB;
-  const long neb = min((long)EB, (long)a.E - e0);
-  const int el = tid / nS, t = tid - el * nS;
-  const long e = e0 + el;
-  const bool valid = el < neb;
-  double *c = lF + el * W;
-  if (valid) {
-#pragma unroll
-    for (int k = 0; k < S::FPT; k++) {
-      const int f = t + k * nS;
-      if (f >= nF) continue;
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      const double dAs = a.fgeo[(e * (D + 1) + D) * nF + f];
-      const double *src = (sl == owner) ? a.Ftr + e * V * nF + f : a.Ftr + ep * V * nF + fp;
-#pragma unroll
-      for (int v = 0; v < V; v++) c[v * nF + f] = src[v * nF] * dAs;
-    }
-  }
-  __syncthreads();
-  double unew[V];
-  if (valid) {
-    const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
-    double acc[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) acc[v] = divF[(e * V + v) * nS + t];
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
-      const double *Df = tab_Dsf<N>(lops, id);
-      const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        acc[v] += (-Df[0]) * c[v * nF + f0];
-        acc[v] += Df[1] * c[v * nF + f1];
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
-    rk_epilogue<D>(a, e, t, nS, acc, unew);
-  }
-  // the new state's partner-owned trace for the next stage
-  if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, c, e, t, valid, unew, owner);
-}
-
-template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restrict__ divF, KArgs a) {
-  face_correction_block<D, N, BM>(divF, a, blockIdx.x);
-}
-
-template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_face_correction_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
-  const int c = find_case(starts, nc, blockIdx.x);
-  face_correction_block<D, N, BM>(cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
-}
-
-// ---------------------------------------------------------------- reverse mode
-#include "adjoint.hpp"
 
 // ---------------------------------------------------------------- RK + norm
 // src/solver.cpp:2897-2941 (stage), 2997-3062 (final), element-major layout.  src, when
@@ -1211,179 +186,6 @@ __global__ void k_norm_final(const double *__restrict__ part, int nblk, int V, i
   out[v] = r;
 }
 
-// ---------------------------------------------------------------- dispatch
-// LDS budget per workgroup; kept at 64 KiB so several workgroups share a CU.
-constexpr size_t kMaxLds = 64 * 1024;
-
-// Elements per workgroup: the requested tile, limited by the thread count and the LDS
-// the kernel needs per element (per_ele doubles).
-inline int eles_per_block(const zfr_ele_desc_t *d, int defEB, size_t fixed, size_t per_ele) {
-  int eb = d->eles_per_block > 0 ? d->eles_per_block : defEB;
-  if (d->eles_per_block <= 0) {
-    // small meshes (coarse multigrid levels, small cases): keep >= 1024 workgroups, four
-    // per CU, so that every SIMD holds a wave; the kernels are latency-bound there, and
-    // partly filled waves cost less than idle SIMDs
-    const long per_cu4 = ((long)d->nEles + 1023) / 1024;
-    if (per_cu4 < eb) eb = per_cu4 < 1 ? 1 : (int)per_cu4;
-  }
-  while (eb > 1 && (eb * d->nSpts > kMaxThreads || sizeof(double) * (fixed + eb * per_ele) > kMaxLds)) eb--;
-  return eb < 1 ? 1 : eb;
-}
-
-inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
-  KArgs a;
-  a.E = d->nEles; a.EB = EB; a.viscous = d->viscous; a.fix_vis = d->fix_vis;
-  a.gamma = d->gamma; a.prandtl = d->prandtl; a.mu = d->mu; a.rt = d->rt; a.c_sth = d->c_sth;
-  a.rus_k = d->rus_k; a.ldg_b = d->ldg_b; a.ldg_tau = d->ldg_tau;
-  a.ops = d->ops1d; a.metric = d->metric; a.fgeo = d->fgeo; a.nbr = d->nbr; a.Utr = d->Utr; a.Gtr = d->Gtr;
-  a.Ftr = d->Ftr;
-  a.trace_sel = 0;
-  a.rk = KArgs::Rk{};
-  return a;
-}
-
-enum class Op { Extrap, ExtrapPartner, Grad, Div, OneSided, OneSidedFromTrace, FaceCorr, AdjPrep, AdjMain, AdjGather };
-
-// k_divergence for one convective flux (FC: 0 Rusanov, 1 Roe) and the LDG beta branch.
-// Euler: the standalone residual extrapolates the partner traces itself (SELF); with a
-// fused RK update it must read them from Utr, since its epilogue overwrites U in place
-// while other workgroups may still read it.
-template <int D, int N, int FC>
-void launch_div(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds, hipStream_t st, const double *U,
-                double *out, const KArgs &a) {
-  if (!d->viscous && a.rk.mode != 0)
-    hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, false>), grid, block, lds, st, U, out, a);
-  else if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, true>), grid, block, lds, st, U, out, a);
-  else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 1, FC>), grid, block, lds, st, U, out, a);
-  else if (d->ldg_b == -0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 2, FC>), grid, block, lds, st, U, out, a);
-  else hipLaunchKernelGGL((k_divergence<D, N, true, 0, FC>), grid, block, lds, st, U, out, a);
-}
-
-// Extra operands of the reverse-mode launches.
-struct AdjPtrs {
-  const double *w = nullptr;  // cotangent of divF
-  double *Wtr = nullptr;      // [E][V][nF] face weights of w
-  double *Xtr = nullptr;      // [E][V][nF] partner trace cotangent at owned points
-  const KArgs::Rk *rk = nullptr;  // fused RK update for Div / FaceCorr
-};
-
-// k_adj_main for one convective flux (FC: 0 Rusanov, 1 Roe) and the LDG beta branch.
-template <int D, int N, int FC>
-void launch_adj_main(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds, hipStream_t st, const double *U,
-                     const AdjPtrs &j, double *out, const KArgs &a) {
-  if (!d->viscous)
-    hipLaunchKernelGGL((k_adj_main<D, N, false, 0, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-  else if (block.x <= 64) {
-    if (d->ldg_b == 0.5)
-      hipLaunchKernelGGL((k_adj_main_w64<D, N, true, 1, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-    else
-      hipLaunchKernelGGL((k_adj_main_w64<D, N, true, 2, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-  } else if (d->ldg_b == 0.5)
-    hipLaunchKernelGGL((k_adj_main<D, N, true, 1, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-  else
-    hipLaunchKernelGGL((k_adj_main<D, N, true, 2, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
-}
-
-// LDS doubles per element of each kernel (the owned-point list of the one-sided kernel is
-// nF + 1 ints).
-template <int D, int N>
-size_t lds_per_element(Op op, bool viscous) {
-  using S = Shape<D, N>;
-  constexpr size_t kUC = (size_t)S::V * S::nS + S::V * S::nF, kG = (size_t)D * S::V * S::nS;
-  size_t per_ele = 0;
-  switch (op) {
-    case Op::Extrap:
-    case Op::ExtrapPartner: per_ele = (size_t)S::V * S::nS; break;
-    case Op::Grad: per_ele = kUC > kG ? kUC : kG; break;
-    case Op::Div: per_ele = kUC + kG; break;
-    case Op::OneSided:
-    case Op::OneSidedFromTrace:
-      per_ele = (size_t)S::V * S::nS + (S::V * S::nF > (int)kG ? S::V * S::nF : kG) + 1;
-      break;
-    case Op::FaceCorr: per_ele = (size_t)S::V * (S::nF > S::nS ? S::nF : S::nS); break;
-    case Op::AdjPrep: per_ele = (size_t)S::V * S::nS; break;
-    case Op::AdjMain:
-      per_ele = viscous ? (size_t)AdjLds<D, N, true>::per_ele : (size_t)AdjLds<D, N, false>::per_ele;
-      break;
-    case Op::AdjGather: per_ele = (size_t)S::V * S::nF; break;
-  }
-  return per_ele;
-}
-
-template <int D, int N>
-int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j = AdjPtrs()) {
-  using S = Shape<D, N>;
-  if (d->nSpts != S::nS || d->nFpts != S::nF || d->nVars != S::V)
-    return zfr::fail("kernel launch: element descriptor does not match (nDims, order)");
-  if (d->nEles <= 0) return 0;
-  const size_t per_ele = lds_per_element<D, N>(op, d->viscous != 0);
-  const bool adj = op == Op::AdjPrep || op == Op::AdjMain || op == Op::AdjGather;
-  if (adj && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5)
-    return zfr::fail("the residual VJP supports LDG beta = +-0.5 (and Euler)");
-  if (d->fconv_type != 0 && d->fconv_type != 2) return zfr::fail("fconv_type must be 0 (Rusanov) or 2 (Roe)");
-  if ((op == Op::OneSided || op == Op::OneSidedFromTrace || op == Op::FaceCorr) && d->ldg_b != 0.5 &&
-      d->ldg_b != -0.5)
-    return zfr::fail("one-sided LDG kernels need ldg_b = +-0.5");
-  // the viscous adjoint runs one element per workgroup: its LDS then allows ~7
-  // workgroups per CU, more than any larger tile
-  const int EB = (op == Op::AdjMain && d->viscous) ? 1 : eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele);
-  if (sizeof(double) * (S::NOPS_PAD + EB * per_ele) > kMaxLds)
-    return zfr::fail("kernel launch: one element needs more LDS than a workgroup may use (order too high)");
-  KArgs a = make_args(d, EB);
-  if (j.rk) {
-    if (op != Op::Div && op != Op::FaceCorr) return zfr::fail("kernel launch: RK fusion is for the last kernel only");
-    a.rk = *j.rk;
-  }
-  const dim3 grid((unsigned)((d->nEles + EB - 1) / EB)), block((unsigned)(EB * S::nS));
-  const size_t lds = sizeof(double) * (S::NOPS_PAD + EB * per_ele);
-  switch (op) {
-    case Op::Extrap:
-    case Op::ExtrapPartner:
-      a.trace_sel = (op == Op::ExtrapPartner) ? 1 : 0;
-      if (block.x <= 64) hipLaunchKernelGGL((k_extrapolate_w64<D, N>), grid, block, lds, st, U, a);
-      else hipLaunchKernelGGL((k_extrapolate<D, N>), grid, block, lds, st, U, a);
-      break;
-    case Op::Grad:
-      hipLaunchKernelGGL((k_gradient<D, N>), grid, block, lds, st, U, out, a);
-      break;
-    case Op::Div:
-      if (d->fconv_type == 2) launch_div<D, N, 1>(d, grid, block, lds, st, U, out, a);
-      else launch_div<D, N, 0>(d, grid, block, lds, st, U, out, a);
-      break;
-    case Op::OneSided:
-    case Op::OneSidedFromTrace: {
-#define ZK_OS(K_, B_, F_, S_) hipLaunchKernelGGL((K_<D, N, B_, F_, S_>), grid, block, lds, st, U, out, a)
-#define ZK_OS2(K_, B_, F_) do { if (op == Op::OneSided) ZK_OS(K_, B_, F_, true); else ZK_OS(K_, B_, F_, false); } while (0)
-      if (block.x <= 64) {
-        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided_w64, 1, 1); else ZK_OS2(k_visc_onesided_w64, 2, 1); }
-        else { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided_w64, 1, 0); else ZK_OS2(k_visc_onesided_w64, 2, 0); }
-      } else {
-        if (d->fconv_type == 2) { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided, 1, 1); else ZK_OS2(k_visc_onesided, 2, 1); }
-        else { if (d->ldg_b == 0.5) ZK_OS2(k_visc_onesided, 1, 0); else ZK_OS2(k_visc_onesided, 2, 0); }
-      }
-#undef ZK_OS2
-#undef ZK_OS
-      break;
-    }
-    case Op::FaceCorr:
-      if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_face_correction<D, N, 1>), grid, block, lds, st, out, a);
-      else hipLaunchKernelGGL((k_face_correction<D, N, 2>), grid, block, lds, st, out, a);
-      break;
-    case Op::AdjPrep:
-      hipLaunchKernelGGL((k_adj_prep<D, N>), grid, block, lds, st, j.w, j.Wtr, a);
-      break;
-    case Op::AdjMain:
-      if (d->fconv_type == 2) launch_adj_main<D, N, 1>(d, grid, block, lds, st, U, j, out, a);
-      else launch_adj_main<D, N, 0>(d, grid, block, lds, st, U, j, out, a);
-      break;
-    case Op::AdjGather:
-      if (d->viscous && d->ldg_b == -0.5) hipLaunchKernelGGL((k_adj_gather<D, N, 2>), grid, block, lds, st, j.Xtr, out, a);
-      else hipLaunchKernelGGL((k_adj_gather<D, N, 0>), grid, block, lds, st, j.Xtr, out, a);
-      break;
-  }
-  return zfr::check_hip(hipGetLastError(), "kernel launch");
-}
-
 int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j = AdjPtrs()) {
   const int N = d->order + 1;
   if (d->nDims == 2) {
@@ -1408,53 +210,6 @@ int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipSt
   return zfr::fail("kernel launch: supported shapes are quads and hexes with order 0..5");
 }
 
-// One launch of a multi-case plan: every case of one kernel, shape and flux branch.
-struct MultiGroup {
-  Op op = Op::Extrap;
-  int D = 0, N = 0, visc = 0, bm = 0, fc = 0;  // bm: LDG branch (0 general, 1 beta = 1/2, 2 beta = -1/2)
-  int EB = 1, nc = 0;
-  unsigned nblk = 0;
-  size_t lds = 0;
-  MultiCase *cases = nullptr;  // device [nc]
-  long *starts = nullptr;      // device [nc]
-};
-
-template <int D, int N>
-void launch_multi(const MultiGroup &g, hipStream_t st) {
-  const dim3 grid(g.nblk), block((unsigned)(g.EB * Shape<D, N>::nS));
-  const MultiCase *c = g.cases;
-  const long *s0 = g.starts;
-  switch (g.op) {
-    case Op::Extrap:
-    case Op::ExtrapPartner:
-      hipLaunchKernelGGL((k_extrapolate_multi<D, N>), grid, block, g.lds, st, c, s0, g.nc);
-      break;
-    case Op::Grad:
-      hipLaunchKernelGGL((k_gradient_multi<D, N>), grid, block, g.lds, st, c, s0, g.nc);
-      break;
-    case Op::Div:
-#define ZK_DIV(V_, B_, F_) hipLaunchKernelGGL((k_divergence_multi<D, N, V_, B_, F_>), grid, block, g.lds, st, c, s0, g.nc)
-      if (!g.visc) { if (g.fc) ZK_DIV(false, 0, 1); else ZK_DIV(false, 0, 0); }
-      else if (g.bm == 1) { if (g.fc) ZK_DIV(true, 1, 1); else ZK_DIV(true, 1, 0); }
-      else if (g.bm == 2) { if (g.fc) ZK_DIV(true, 2, 1); else ZK_DIV(true, 2, 0); }
-      else { if (g.fc) ZK_DIV(true, 0, 1); else ZK_DIV(true, 0, 0); }
-#undef ZK_DIV
-      break;
-    case Op::OneSided:
-#define ZK_OSM(B_, F_) hipLaunchKernelGGL((k_visc_onesided_multi<D, N, B_, F_>), grid, block, g.lds, st, c, s0, g.nc)
-      if (g.bm == 1) { if (g.fc) ZK_OSM(1, 1); else ZK_OSM(1, 0); }
-      else { if (g.fc) ZK_OSM(2, 1); else ZK_OSM(2, 0); }
-#undef ZK_OSM
-      break;
-    case Op::FaceCorr:
-      if (g.bm == 1) hipLaunchKernelGGL((k_face_correction_multi<D, N, 1>), grid, block, g.lds, st, c, s0, g.nc);
-      else hipLaunchKernelGGL((k_face_correction_multi<D, N, 2>), grid, block, g.lds, st, c, s0, g.nc);
-      break;
-    default:
-      break;
-  }
-}
-
 template <int D>
 void dispatch_multi_n(const MultiGroup &g, hipStream_t st) {
   switch (g.N) {
@@ -1467,19 +222,6 @@ void dispatch_multi_n(const MultiGroup &g, hipStream_t st) {
   }
 }
 
-template <int D>
-size_t lds_per_element_n(int N, Op op, bool viscous) {
-  switch (N) {
-    case 1: return lds_per_element<D, 1>(op, viscous);
-    case 2: return lds_per_element<D, 2>(op, viscous);
-    case 3: return lds_per_element<D, 3>(op, viscous);
-    case 4: return lds_per_element<D, 4>(op, viscous);
-    case 5: return lds_per_element<D, 5>(op, viscous);
-    case 6: return lds_per_element<D, 6>(op, viscous);
-  }
-  return 0;
-}
-
 inline unsigned grid_for(long total) {
   long g = (total + kMaxThreads - 1) / kMaxThreads;
   if (g > 8192) g = 8192;
@@ -1488,13 +230,15 @@ inline unsigned grid_for(long total) {
 
 }  // namespace zk
 
-extern "C" {
-
 namespace {
 // Buffers a launch reads or writes through the descriptor: a null one is an error here,
 // not a fault on the device.
 int need(const void *p, const char *what) {
   return p ? 0 : zfr::fail(std::string("kernel launch: the descriptor's ") + what + " is NULL");
+}
+// A pointer argument of the entry point itself.
+int need_arg(const void *p, const char *what) {
+  return p ? 0 : zfr::fail(std::string("kernel launch: ") + what + " is NULL");
 }
 int need_geometry(const zfr_ele_desc_t *d) {
   if (!d) return zfr::fail("kernel launch: NULL descriptor");
@@ -1506,8 +250,11 @@ int need_geometry(const zfr_ele_desc_t *d) {
 }
 }  // namespace
 
+extern "C" {
+
 int zfr_extrapolate(const zfr_ele_desc_t *d, const double *U, void *stream) {
   int rc = need_geometry(d);
+  if (!rc) rc = need_arg(U, "U");
   if (!rc) rc = need(d->Utr, "Utr");
   if (rc) return rc;
   return zk::dispatch(zk::Op::Extrap, d, U, nullptr, (hipStream_t)stream);
@@ -1515,6 +262,7 @@ int zfr_extrapolate(const zfr_ele_desc_t *d, const double *U, void *stream) {
 
 int zfr_extrapolate_partner(const zfr_ele_desc_t *d, const double *U, void *stream) {
   int rc = need_geometry(d);
+  if (!rc) rc = need_arg(U, "U");
   if (!rc) rc = need(d->Utr, "Utr");
   if (rc) return rc;
   if (!d->viscous || (d->ldg_b != 0.5 && d->ldg_b != -0.5))
@@ -1524,6 +272,7 @@ int zfr_extrapolate_partner(const zfr_ele_desc_t *d, const double *U, void *stre
 
 int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void *stream) {
   int rc = need_geometry(d);
+  if (!rc) rc = need_arg(U, "U");
   if (!rc) rc = need(d->Utr, "Utr");
   if (!rc) rc = need(d->Gtr, "Gtr");
   if (rc) return rc;
@@ -1533,6 +282,8 @@ int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void 
 
 int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
   int rc = need_geometry(d);
+  if (!rc) rc = need_arg(U, "U");
+  if (!rc) rc = need_arg(divF, "divF");
   if (!rc && d->viscous) rc = need(d->Utr, "Utr");
   if (!rc && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5) rc = need(d->Gtr, "Gtr");
   if (rc) return rc;
@@ -1541,6 +292,8 @@ int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void 
 
 int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
   int rc = need_geometry(d);
+  if (!rc) rc = need_arg(U, "U");
+  if (!rc) rc = need_arg(divF, "divF");
   if (!rc) rc = need(d->Utr, "Utr");
   if (!rc) rc = need(d->Ftr, "Ftr");
   if (rc) return rc;
@@ -1550,6 +303,8 @@ int zfr_gradient_flux(const zfr_ele_desc_t *d, const double *U, double *divF, vo
 
 int zfr_gradient_flux_traced(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream) {
   int rc = need_geometry(d);
+  if (!rc) rc = need_arg(U, "U");
+  if (!rc) rc = need_arg(divF, "divF");
   if (!rc) rc = need(d->Utr, "Utr");
   if (!rc) rc = need(d->Ftr, "Ftr");
   if (rc) return rc;
@@ -1559,11 +314,14 @@ int zfr_gradient_flux_traced(const zfr_ele_desc_t *d, const double *U, double *d
 
 int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream) {
   int rc = need_geometry(d);
+  if (!rc) rc = need_arg(divF, "divF");
   if (!rc) rc = need(d->Ftr, "Ftr");
   if (rc) return rc;
   if (!d->viscous) return zfr::fail("zfr_face_correction: descriptor is inviscid");
   return zk::dispatch(zk::Op::FaceCorr, d, nullptr, divF, (hipStream_t)stream);
 }
+
+}  // extern "C"
 
 namespace {
 int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *r,
@@ -1600,13 +358,20 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
 }
 }  // namespace
 
+extern "C" {
+
 int zfr_face_correction_rk(const zfr_ele_desc_t *d, double *divF, const zfr_rk_args_t *rk, void *stream) {
+  if (int rc = need_geometry(d)) return rc;
+  if (int rc = need_arg(divF, "divF")) return rc;
   if (!d->viscous) return zfr::fail("zfr_face_correction_rk: descriptor is inviscid");
   return rk_fused(zk::Op::FaceCorr, d, nullptr, divF, rk, (hipStream_t)stream);
 }
 
 int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *rk,
                       void *stream) {
+  if (int rc = need_geometry(d)) return rc;
+  if (int rc = need_arg(U, "U")) return rc;
+  if (int rc = need_arg(divF, "divF")) return rc;
   return rk_fused(zk::Op::Div, d, U, divF, rk, (hipStream_t)stream);
 }
 
@@ -1686,22 +451,52 @@ int zfr_axpby(long n, double a, const double *x, double b, const double *y, doub
   return zfr::check_hip(hipGetLastError(), "zfr_axpby");
 }
 
-int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work, double *res_out,
-                      void *stream) {
-  if (res_type < 0 || res_type > 2) return zfr::fail("zfr_residual_norm: res_type must be 0, 1 or 2");
+}  // extern "C"
+
+namespace {
+// Block partials of the residual norm: at most kNormMaxBlocks workgroups, V <= kNormMaxVars.
+constexpr int kNormMaxBlocks = 1024, kNormMaxVars = 8;
+
+int norm_checks(const zfr_ele_desc_t *d, const double *divF, int res_type, const double *res_out, const char *who) {
+  if (!d) return zfr::fail(std::string(who) + ": NULL descriptor");
+  if (res_type < 0 || res_type > 2) return zfr::fail(std::string(who) + ": res_type must be 0, 1 or 2");
+  if (d->nDims != 2 && d->nDims != 3) return zfr::fail(std::string(who) + ": nDims must be 2 or 3");
+  if (d->nVars != d->nDims + 2) return zfr::fail(std::string(who) + ": nVars must be nDims + 2");
+  if (d->nEles < 0 || d->nSpts < 1) return zfr::fail(std::string(who) + ": bad sizes");
+  if (!divF || !res_out || !d->metric) return zfr::fail(std::string(who) + ": missing buffer (divF, res_out or metric)");
+  return 0;
+}
+
+// Partials into part[nblk][V] (every entry written before it is read), then the final
+// reduction into res_out[V].
+int norm_launch(const zfr_ele_desc_t *d, const double *divF, int res_type, double *part, double *res_out,
+                hipStream_t st) {
   const long npts = (long)d->nEles * d->nSpts;
   unsigned nblk = zk::grid_for(npts);
-  if (nblk > 1024) nblk = 1024;
-  hipStream_t st = (hipStream_t)stream;
+  if (nblk > (unsigned)kNormMaxBlocks) nblk = kNormMaxBlocks;
   if (d->nDims == 2)
     hipLaunchKernelGGL(zk::k_norm_partial<2>, dim3(nblk), dim3(zk::kMaxThreads), 0, st, divF, d->metric,
-                       (long)d->nEles, d->nSpts, res_type, work);
+                       (long)d->nEles, d->nSpts, res_type, part);
   else
     hipLaunchKernelGGL(zk::k_norm_partial<3>, dim3(nblk), dim3(zk::kMaxThreads), 0, st, divF, d->metric,
-                       (long)d->nEles, d->nSpts, res_type, work);
-  hipLaunchKernelGGL(zk::k_norm_final, dim3(1), dim3(64), 0, st, work, (int)nblk, d->nVars, res_type, res_out);
+                       (long)d->nEles, d->nSpts, res_type, part);
+  hipLaunchKernelGGL(zk::k_norm_final, dim3(1), dim3(64), 0, st, part, (int)nblk, d->nVars, res_type, res_out);
   return zfr::check_hip(hipGetLastError(), "zfr_residual_norm");
 }
+}  // namespace
+
+extern "C" {
+
+int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work, double *res_out,
+                      void *stream) {
+  if (int rc = norm_checks(d, divF, res_type, res_out, "zfr_residual_norm")) return rc;
+  if (!work) return zfr::fail("zfr_residual_norm: missing buffer (work)");
+  return norm_launch(d, divF, res_type, work, res_out, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, double *v, double *Wtr,
                      double *Xtr, void *stream) {

@@ -214,9 +214,10 @@ void PMGrid::drop_graph() {
 }
 
 std::vector<double> PMGrid::graph_key(const FRSolver &solver) const {
-  std::vector<double> k = {input->dt, input->rel_fac, (double)solver.eles->tile_key(), (double)solver.fuse_rk};
+  std::vector<double> k = {input->dt, input->rel_fac, (double)solver.fuse_rk};
+  for (int t : solver.eles->tile_key()) k.push_back((double)t);
   for (int n = 1; n < nLevels; n++) {
-    k.push_back((double)grids[n]->eles->tile_key());
+    for (int t : grids[n]->eles->tile_key()) k.push_back((double)t);
     k.push_back((double)grids[n]->fuse_rk);
   }
   for (int n = 0; n < nLevels; n++) k.push_back((double)input->mg_steps[n]);

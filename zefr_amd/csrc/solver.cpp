@@ -679,7 +679,9 @@ int FRSolver::autotune(int reps) {
       for (int k = 0; k < 3; k++)
         if (ms[k] > 0.0 && ms[k] < best[k]) { best[k] = ms[k]; best_eb[k] = eb; }
     }
-  if (best[1] == 1e300) best_eb[1] = best_eb[2];  // inviscid: no middle launch
+  // an inviscid residual has no middle launch: phase 1 timed only the gap between two
+  // events, so its tile is the divergence launch's
+  if (!input->viscous) best_eb[1] = best_eb[2];
   // one-sided and Euler residuals have no trace launch, so phase 0 timed nothing; its tile
   // (used by a fused RK step's stage-0 extrapolation) falls back to the solver-wide tile
   if (!(input->viscous && !onesided())) best_eb[0] = 0;
