@@ -4,14 +4,15 @@
 Workload (one "step" = one FRSolver::compute_residual over the whole mesh, inputs
 resident in HBM): periodic 64^3 uniform hex mesh on [0, 2pi]^3, p = 3 (64 solution
 points per element, 16.8M points x 5 variables = 83.9M DoF), Re = 1600, Ma = 0.1,
-Pr = 0.71, Rusanov + LDG (beta = 0.5, tau = 0.1), FP64, Taylor-Green initial state.
+Pr = 0.71, Roe + LDG (beta = 0.5, tau = 0.1), FP64, Taylor-Green initial state.
 
 Prints one JSON line with the throughput in Gdof-residuals/s, the roofline of the
 dominant kernel (live HIP-event timing on the solver's stream), the whole-residual
-roofline against the per-element byte model, a stream-copy bandwidth measured on the
-same device, and a CPU baseline: the CPU oracle (oracle/, OpenMP) on a bounded sample.
+byte model, a stream-copy bandwidth measured on the same device, the residual's VJP
+ratio on the 32^3 adjoint case, and a CPU baseline: the CPU oracle (oracle/, OpenMP, in
+a child process) on the same 64^3 mesh and state.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--n 64] [--order 3]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--n 64] [--order 3] [--flux Roe]
 """
 
 from __future__ import annotations
@@ -39,6 +40,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--n", type=int, default=64, help="elements per direction")
     p.add_argument("--order", type=int, default=3)
+    p.add_argument("--flux", default="Roe", choices=["Roe", "Rusanov"],
+                   help="convective common flux of the headline (the other one is timed as a leg)")
     p.add_argument("--tile", type=int, default=-1, help="elements per workgroup (-1: autotune)")
     p.add_argument("--tiles", default=None,
                    help="per-launch tiles 'trace,flux,correction' (overrides --tile; default: autotune)")
@@ -46,12 +49,15 @@ def parse():
     p.add_argument("--no-vjp", action="store_true", help="skip the VJP timing")
     p.add_argument("--no-steps", action="store_true", help="skip the RK4 step timings")
     p.add_argument("--no-pmg", action="store_true", help="skip the p-multigrid timing")
-    p.add_argument("--no-roe", action="store_true", help="skip the Roe+LDG variant timing")
+    p.add_argument("--no-other-flux", action="store_true", help="skip the leg with the other convective flux")
+    p.add_argument("--no-c5", action="store_true", help="skip the 32^3 adjoint-case VJP ratio")
     p.add_argument("--no-batch", action="store_true", help="skip the batched small-case timing")
     p.add_argument("--no-c2", action="store_true", help="skip the 2D Navier-Stokes 256^2 p=4 timing")
-    p.add_argument("--cpu-n", type=int, default=32, help="elements per direction of the CPU sample")
-    p.add_argument("--cpu-steps", type=int, default=150)
+    p.add_argument("--cpu-warmup", type=int, default=2)
+    p.add_argument("--cpu-steps", type=int, default=10, help="timed CPU residuals (median reported)")
     p.add_argument("--workdir", default=None)
+    p.add_argument("--cpu-worker", nargs=4, metavar=("MESH", "ORDER", "FLUX", "OUT"), default=None,
+                   help=argparse.SUPPRESS)  # internal: the CPU baseline's child process
     return p.parse_args()
 
 
@@ -113,7 +119,19 @@ def kernel_bytes(D, N):
     return {"gradient_flux": gflux, "face_correction": fcorr}
 
 
-KERNEL_NAMES = {"gradient_flux": "k_visc_onesided<3,{N},1>", "face_correction": "k_face_correction<3,{N},1>"}
+def kernel_bytes_read_once(D, N):
+    """The same launches counted with every value read once (the scope table's rule): the
+    partner's solution enters gradient_flux as one trace value per owned flux point and
+    variable (nF/2 V), as if a trace launch had formed it, instead of its N-point line."""
+    V = D + 2
+    nF = 2 * D * N ** (D - 1)
+    b = kernel_bytes(D, N)
+    b["gradient_flux"] -= 8 * (nF // 2) * (N - 1) * V
+    return b
+
+
+FLUX_CODE = {"Rusanov": 0, "Roe": 1}  # the kernels' convective-flux template parameter (FC)
+KERNEL_NAMES = {"gradient_flux": "k_visc_onesided<3,{N},1,{FC}>", "face_correction": "k_face_correction<3,{N},1>"}
 
 
 def residual_model_bytes(D, N, viscous):
@@ -464,44 +482,58 @@ def batch_timing(workdir, count=64, reps=20, seed=2024):
             "graph_Gdof_res_per_s": round(dof / (out["graph_ms"] * 1e-3) / 1e9, 3)}
 
 
-def roe_timing(workdir, mesh, order, steps, with_vjp=True):
-    """The Roe+LDG variant of the same Taylor-Green workload (the reference has no Roe
-    flux; it is timed separately from the Rusanov+LDG headline): residual throughput and
-    the three kernels' average durations, HIP events on the solver's stream, and the Roe
-    residual's vector-Jacobian product against its forward."""
-    from zefr_amd import zefr
+def residual_timing(z, steps):
+    """Wall time of `steps` back-to-back residuals on the solver's stream (bracketed by
+    device synchronisation) and the average duration of each of its kernels (HIP events
+    recorded on that stream)."""
     from zefr_amd._lib import lib
-    deck = write_deck(os.path.join(workdir, "tgv_roe.deck"), mesh, order, fconv="Roe")
+    import torch
+    z.synchronize()
+    torch.cuda.synchronize()
+    ms3 = (C.c_double * 3)()
+    t = time.perf_counter()
+    rc = lib().zefr_time_residual_kernels(z._h, steps, ms3)
+    z.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    if rc != 0:
+        raise RuntimeError(lib().zefr_last_error().decode())
+    return wall / steps * 1e3, {"gradient_flux": ms3[1], "face_correction": ms3[2]}
+
+
+def tgv_solver(workdir, mesh, order, flux, tag):
+    from zefr_amd import zefr
+    deck = write_deck(os.path.join(workdir, f"tgv_{tag}.deck"), mesh, order, fconv=flux)
     z = zefr.Zefr()
     z.read_input(deck)
     z.setup_solver()
+    return z
+
+
+def flux_leg(workdir, mesh, order, steps, flux):
+    """The same Taylor-Green workload with the other convective flux: residual
+    throughput, its kernels' average durations and the VJP against the forward."""
+    z = tgv_solver(workdir, mesh, order, flux, flux.lower())
     z.autotune(3)
     d = z.dims()
     dof = d["nEles"] * d["nSpts"] * d["nVars"]
     for _ in range(3):
         z.compute_residual(0)
-    z.synchronize()
-    ms3 = (C.c_double * 3)()
-    t = time.perf_counter()
-    rc = lib().zefr_time_residual_kernels(z._h, steps, ms3)
-    z.synchronize()
-    wall = time.perf_counter() - t
-    if rc != 0:
-        raise RuntimeError(lib().zefr_last_error().decode())
-    vjp = vjp_timing(z, steps) if with_vjp else None
+    ms, kms = residual_timing(z, steps)
+    vjp = vjp_timing(z, steps)
+    tiles = z.tiles()
     del z
-    return {"workload": "3D Navier-Stokes Taylor-Green residual with Roe+LDG (Harten entropy fix 0.1 a), same mesh",
-            "value": round(dof / (wall / steps) / 1e9, 4), "unit": "Gdof-residuals/s",
-            "ms_per_step": round(wall / steps * 1e3, 4),
-            "kernels_ms": {"gradient_flux": round(ms3[1], 4),
-                           "face_correction": round(ms3[2], 4)},
-            "vjp": vjp}
+    return {"workload": f"3D Navier-Stokes Taylor-Green residual with {flux}+LDG(beta=0.5,tau=0.1), same mesh",
+            "value": round(dof / (ms * 1e-3) / 1e9, 4), "unit": "Gdof-residuals/s", "ms_per_step": round(ms, 4),
+            "tiles_eles_per_block": tiles,
+            "kernels_ms": {k: round(v, 4) for k, v in kms.items()}, "vjp": vjp}
 
 
 def vjp_timing(z, reps):
-    """The residual's vector-Jacobian product on the same mesh and state: mean time of
-    `reps` back-to-back calls (HIP events on the torch stream both run on) against the
-    mean time of as many forward residual_at calls on that stream."""
+    """The residual's vector-Jacobian product on the solver's mesh and state against the
+    forward residual (residual_at), both on the torch stream, each call bracketed by HIP
+    events: medians over `reps` calls (BASELINE's VJP ratio is median over median)."""
+    import statistics
     import torch
     U = z.state_tensor()
     w = torch.randn_like(U)
@@ -512,53 +544,126 @@ def vjp_timing(z, reps):
         z.residual_at(U, R)
         z.residual_vjp(U, w, v)
     torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    ev[0].record(st)
-    for _ in range(reps):
+    ef = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ea = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ef[0].record(st)
+    for i in range(reps):
         z.residual_at(U, R)
-    ev[1].record(st)
-    for _ in range(reps):
+        ef[i + 1].record(st)
+    ea[0].record(st)
+    for i in range(reps):
         z.residual_vjp(U, w, v)
-    ev[2].record(st)
-    ev[2].synchronize()
-    fwd = ev[0].elapsed_time(ev[1]) / reps
-    adj = ev[1].elapsed_time(ev[2]) / reps
+        ea[i + 1].record(st)
+    ea[-1].synchronize()
+    fwd = statistics.median(ef[i].elapsed_time(ef[i + 1]) for i in range(reps))
+    adj = statistics.median(ea[i].elapsed_time(ea[i + 1]) for i in range(reps))
     ok = bool(torch.isfinite(v).all())
     del U, w, v, R
     torch.cuda.empty_cache()
     return {"forward_ms": round(fwd, 4), "vjp_ms": round(adj, 4), "vjp_residual_ratio": round(adj / fwd, 3),
-            "finite": ok}
+            "statistic": f"median of {reps}", "finite": ok}
 
 
-def cpu_baseline(workdir, n, order, steps):
-    """The CPU oracle on an n^3 sample of the same workload (OpenMP, all host threads)."""
-    from oracle.oracle import Oracle, make_phys
+def c5_vjp(workdir, order, reps, flux):
+    """The adjoint configuration's VJP/residual ratio: 32^3 hexes p=3 Taylor-Green."""
     from zefr_amd.mesh import write_box
-    import numpy as np
+    mesh = write_box(os.path.join(workdir, "hex32.msh"), (32, 32, 32), (0.0,) * 3, (2 * math.pi,) * 3)
+    out = {"workload": f"3D Navier-Stokes Taylor-Green, 32^3 hexes on [0,2pi]^3, p={order}"}
+    for fl in (flux, "Rusanov" if flux == "Roe" else "Roe"):
+        z = tgv_solver(workdir, mesh, order, fl, f"c5_{fl.lower()}")
+        z.autotune(3)
+        out[fl] = vjp_timing(z, reps)
+        del z
+    out["vjp_residual_ratio"] = out[flux]["vjp_residual_ratio"]
+    return out
+
+
+def cpu_topology():
+    """Host CPU description: lscpu's model and counts, the CPUs this process may run on and
+    the physical cores among them (sysfs topology), and the OpenMP thread count to use:
+    one per available physical core, capped by an inherited OMP_NUM_THREADS."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, ValueError):
+        pass
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    phys = len(cores)
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(phys, cap) if cap > 0 else phys
+    info.update({"cpus_available": len(cpus), "physical_cores_available": phys,
+                 "omp_num_threads_inherited": cap or None})
+    return threads, info
+
+
+def cpu_worker(mesh, order, flux, out_path, warm, reps):
+    """Child process of the CPU baseline: the CPU oracle on the benchmark's mesh and state,
+    `warm` untimed then `reps` timed residuals; writes the per-residual times (s)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle.oracle import Oracle, make_phys
     from common import tgv3d_state
-    mesh = write_box(os.path.join(workdir, f"cpu_hex{n}.msh"), (n, n, n), (0.0,) * 3, (2 * math.pi,) * 3)
     U0 = 0.1 * math.sqrt(1.4)
     mu = 1.0 * U0 * 1.0 / 1600.0
+    t0 = time.perf_counter()
     o = Oracle(mesh, 3, order, make_phys(viscous=True, gamma=1.4, prandtl=0.71, mu=mu, fix_vis=True,
-                                         rus_k=0.0, ldg_b=0.5, ldg_tau=0.1, dt=2e-4))
+                                         rus_k=0.0, ldg_b=0.5, ldg_tau=0.1, dt=2e-4, fconv=2 if flux == "Roe" else 0))
     x = o.coord_spts()
     o.set_U(tgv3d_state(x[:, 0, :], x[:, 1, :], x[:, 2, :]))
-    o.compute_residual(0)
-    t = time.perf_counter()
-    for _ in range(steps):
+    setup = time.perf_counter() - t0
+    for _ in range(warm):
         o.compute_residual(0)
-    dt = (time.perf_counter() - t) / steps
-    dof = o.nEles * o.nSpts * o.nVars
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    return {"value": dof / dt / 1e9, "unit": "Gdof-residuals/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (oracle/zefr_oracle.cpp, OpenMP) on the same TGV case at {n}^3 hexes "
-                      f"p={order} ({dof / 1e6:.2f}M DoF), mean of {steps} back-to-back residuals "
-                      f"({dt * 1e3:.1f} ms each, {dt * steps:.1f} s total)", "ms_per_residual": dt * 1e3}
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        o.compute_residual(0)
+        ts.append(time.perf_counter() - t)
+    with open(out_path, "w") as f:
+        json.dump({"times_s": ts, "setup_s": setup, "dof": o.nEles * o.nSpts * o.nVars, "nEles": o.nEles}, f)
+
+
+def cpu_baseline(workdir, mesh, n, order, flux, warm, reps):
+    """The CPU oracle (oracle/zefr_oracle.cpp, OpenMP) on the same mesh, state and flux as
+    the headline, run in a child process so that its OpenMP settings take effect:
+    OMP_NUM_THREADS = the physical cores available, OMP_PROC_BIND=close, OMP_PLACES=cores;
+    median of `reps` residuals after `warm` untimed ones."""
+    import statistics
+    import subprocess
+    threads, topo = cpu_topology()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out_path = os.path.join(workdir, "cpu_baseline.json")
+    subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", mesh, str(order), flux, out_path,
+                    "--cpu-warmup", str(warm), "--cpu-steps", str(reps)], env=env, check=True)
+    with open(out_path) as f:
+        r = json.load(f)
+    med = statistics.median(r["times_s"])
+    return {"value": r["dof"] / med / 1e9, "unit": "Gdof-residuals/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (oracle/zefr_oracle.cpp, OpenMP, {threads} threads, OMP_PROC_BIND=close, "
+                      f"OMP_PLACES=cores) on the headline workload itself: {n}^3 hexes p={order}, {flux}+LDG, "
+                      f"{r['dof'] / 1e6:.1f}M DoF; median of {reps} residuals after {warm} warm-up "
+                      f"({med * 1e3:.0f} ms each, {sum(r['times_s']):.1f} s timed)",
+            "ms_per_residual": round(med * 1e3, 2), "times_ms": [round(t * 1e3, 1) for t in r["times_s"]],
+            "setup_s": round(r["setup_s"], 1), "host": topo}
 
 
 def main():
     args = parse()
+    if args.cpu_worker:
+        mesh, order, flux, out_path = args.cpu_worker
+        cpu_worker(mesh, int(order), flux, out_path, args.cpu_warmup, args.cpu_steps)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo exchange is a design, see the sharding notes)", file=sys.stderr)
@@ -567,19 +672,16 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    from zefr_amd import zefr
     from zefr_amd._lib import lib
     from zefr_amd.mesh import write_box
 
     workdir = args.workdir or tempfile.mkdtemp(prefix="zefr_bench_")
-    n, order = args.n, args.order
-    D, N = 3, order + 1
+    n, order, flux = args.n, args.order, args.flux
+    other = "Rusanov" if flux == "Roe" else "Roe"
+    D, N, FC = 3, order + 1, FLUX_CODE[flux]
     t0 = time.perf_counter()
     mesh = write_box(os.path.join(workdir, f"hex{n}.msh"), (n, n, n), (0.0,) * 3, (2 * math.pi,) * 3)
-    deck = write_deck(os.path.join(workdir, "tgv.deck"), mesh, order)
-    z = zefr.Zefr()
-    z.read_input(deck)
-    z.setup_solver()
+    z = tgv_solver(workdir, mesh, order, flux, "main")
     setup_s = time.perf_counter() - t0
     d = z.dims()
     E, nS, V = d["nEles"], d["nSpts"], d["nVars"]
@@ -595,37 +697,31 @@ def main():
 
     for _ in range(args.warmup):
         z.compute_residual(0)
-    z.synchronize()
-    torch.cuda.synchronize()
-    ms3 = (C.c_double * 3)()
-    t = time.perf_counter()
-    rc = lib().zefr_time_residual_kernels(z._h, args.steps, ms3)
-    z.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t
-    if rc != 0:
-        raise RuntimeError(lib().zefr_last_error().decode())
-    ms_step = wall / args.steps * 1e3
-    value = dof / (wall / args.steps) / 1e9
+    ms_step, kms = residual_timing(z, args.steps)
+    value = dof / (ms_step * 1e-3) / 1e9
 
     kb = kernel_bytes(D, N)
     names = ["gradient_flux", "face_correction"]  # the one-sided residual has no trace launch
-    kms = dict(zip(names, list(ms3)[1:]))
     dom = max(names, key=lambda k: kms[k])
+    kname = {k: KERNEL_NAMES[k].format(N=N, FC=FC) for k in names}
     achieved = kb[dom] * E / (kms[dom] * 1e-3) / 1e9
-    per_kernel = {KERNEL_NAMES[k].format(N=N): {"avg_ms": kms[k], "alg_bytes_per_element": kb[k],
-                      "achieved_GBs": kb[k] * E / (kms[k] * 1e-3) / 1e9 if kms[k] > 0 else None}
+    per_kernel = {kname[k]: {"avg_ms": round(kms[k], 4), "alg_bytes_per_element": kb[k],
+                             "achieved_GBs": round(kb[k] * E / (kms[k] * 1e-3) / 1e9, 1) if kms[k] > 0 else None}
                   for k in names}
+    read_once = kernel_bytes_read_once(D, N)
     model = residual_model_bytes(D, N, True)
     res_gbs = model * E / (ms_step * 1e-3) / 1e9
     copy_gbs, copy_ok = stream_copy_gbs(lib())
     vjp = None if args.no_vjp else vjp_timing(z, args.steps)
     rk = None
     if not args.no_steps:
-        rk = {"tgv_hex64_p3": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
+        rk = {f"tgv_hex{n}_p{order}": step_timing(z, 5), "vortex_quad32_p3": small_case_steps(workdir)}
+    del z
+    torch.cuda.empty_cache()
+    c5 = None if (args.no_c5 or args.no_vjp) else c5_vjp(workdir, order, args.steps, flux)
     c2 = None if args.no_c2 else c2_timing(workdir, args.steps)
     pmg = None if args.no_pmg else pmg_timing(workdir)
-    roe = None if args.no_roe else roe_timing(workdir, mesh, order, args.steps, not args.no_vjp)
+    oth = None if args.no_other_flux else flux_leg(workdir, mesh, order, args.steps, other)
     batch = None if args.no_batch else batch_timing(workdir)
 
     rec = {
@@ -643,27 +739,31 @@ def main():
         "data": "synthetic: generated periodic hex mesh, Taylor-Green initial state",
         "config": {"workload": f"3D Navier-Stokes Taylor-Green residual, {n}^3 hexes on [0,2pi]^3, p={order}",
                    "elements": E, "spts_per_element": nS, "vars": V, "dof": dof, "Re": 1600, "Ma": 0.1,
-                   "Pr": 0.71, "flux": "Rusanov+LDG(beta=0.5,tau=0.1)", "tiles_eles_per_block": tile,
+                   "Pr": 0.71, "flux": f"{flux}+LDG(beta=0.5,tau=0.1)", "tiles_eles_per_block": tile,
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(KERNEL_NAMES[dom].format(N=N), n, order),
-                     "kernel": KERNEL_NAMES[dom].format(N=N),
-                     "alg_bytes_per_launch": kb[dom] * E, "avg_launch_ms": round(kms[dom], 4)},
-        "roofline_residual": {"bytes_per_element_model": model, "achieved": round(res_gbs, 1),
-                              "frac_of_peak": round(res_gbs / HBM_PEAK_GBS, 4),
-                              "stream_copy_GBs": round(copy_gbs, 1), "stream_copy_ok": copy_ok,
-                              "frac_of_stream_copy": round(res_gbs / copy_gbs, 4)},
+                     "traffic": profiled_traffic(kname[dom], n, order),
+                     "kernel": kname[dom], "timing": "live: HIP events on the solver's stream, mean launch",
+                     "alg_bytes_per_launch": kb[dom] * E, "avg_launch_ms": round(kms[dom], 4),
+                     "read_once": {"alg_bytes_per_launch": read_once[dom] * E,
+                                   "achieved": round(read_once[dom] * E / (kms[dom] * 1e-3) / 1e9, 1),
+                                   "frac": round(read_once[dom] * E / (kms[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+        "roofline_residual": {"bytes_per_element_model": model, "model_GBs": round(res_gbs, 1),
+                              "model_frac_of_peak": round(res_gbs / HBM_PEAK_GBS, 4),
+                              "note": "byte model of the reference's eleven-launch data flow, not a bandwidth",
+                              "stream_copy_GBs": round(copy_gbs, 1), "stream_copy_ok": copy_ok},
         "kernels": per_kernel,
         "vjp": vjp,
+        "vjp_c5": c5,
         "rk4_step": rk,
         "c2_quad256_p4": c2,
         "pmg": pmg,
-        "roe": roe,
+        other.lower(): oth,
         "batch": batch,
     }
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(workdir, args.cpu_n, order, args.cpu_steps)
+        rec["cpu_baseline"] = cpu_baseline(workdir, mesh, n, order, flux, args.cpu_warmup, args.cpu_steps)
     print(json.dumps(rec), flush=True)
 
 

@@ -437,38 +437,45 @@ struct Oracle {
   }
 
   // geometry.cpp:1159-1255
+  // Each periodic face is paired with the first other periodic face (in the map's order)
+  // of the same node count whose centroid matches in nDims-1 coordinates that the face
+  // itself does not lie flat in.  The centroids and the per-face "flat in d" flags are
+  // the reference's per-pair quantities, computed once per face with the same arithmetic;
+  // the scan per face is independent, so the faces run in parallel.
   void couple_periodic_bnds() {
-    for (auto &bf : bnd_faces) {
-      if (bf.second != BC_PERIODIC) continue;
-      const auto &face1 = bf.first;
-      unsigned nn = face1.size();
-      std::vector<real> c1(nDims, 0.0);
+    std::vector<const std::vector<unsigned> *> pf;
+    for (auto &bf : bnd_faces)
+      if (bf.second == BC_PERIODIC) pf.push_back(&bf.first);
+    const long np = (long)pf.size();
+    std::vector<real> cen(np * nDims, 0.0);
+    std::vector<char> flat(np * nDims, 1);
+    for (long i = 0; i < np; i++) {
+      const auto &f = *pf[i];
+      const unsigned nn = f.size();
       for (int d = 0; d < nDims; d++) {
-        for (unsigned k = 0; k < nn; k++) c1[d] += coord_nodes[face1[k] * nDims + d];
-        c1[d] /= nn;
+        real c = 0.0;
+        for (unsigned k = 0; k < nn; k++) c += coord_nodes[f[k] * nDims + d];
+        cen[i * nDims + d] = c / nn;
+        const real c0 = coord_nodes[f[0] * nDims + d];
+        for (unsigned k = 1; k < nn; k++)
+          if (std::abs(c0 - coord_nodes[f[k] * nDims + d]) > 1.e-6) flat[i * nDims + d] = 0;
       }
-      bool paired = false;
-      for (auto &bf2 : bnd_faces) {
-        const auto &face2 = bf2.first;
-        if (face2 == face1 || face2.size() != nn || bf2.second != BC_PERIODIC) continue;
-        std::vector<real> c2(nDims, 0.0);
-        for (int d = 0; d < nDims; d++) {
-          for (unsigned k = 0; k < nn; k++) c2[d] += coord_nodes[face2[k] * nDims + d];
-          c2[d] /= nn;
-        }
+    }
+    std::vector<long> match(np, -1);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < np; i++) {
+      const size_t nn = pf[i]->size();
+      for (long j = 0; j < np; j++) {
+        if (j == i || pf[j]->size() != nn) continue;
         int count = 0;
-        for (int d = 0; d < nDims; d++) {
-          if (std::abs(c1[d] - c2[d]) < 1.e-6) {
-            bool onPlane = true;
-            real c0 = coord_nodes[face1[0] * nDims + d];
-            for (unsigned k = 1; k < nn; k++)
-              if (std::abs(c0 - coord_nodes[face1[k] * nDims + d]) > 1.e-6) onPlane = false;
-            if (!onPlane) count++;
-          }
-        }
-        if (count == nDims - 1) { paired = true; per_bnd_pairs[face1] = face2; break; }
+        for (int d = 0; d < nDims; d++)
+          if (std::abs(cen[i * nDims + d] - cen[j * nDims + d]) < 1.e-6 && !flat[i * nDims + d]) count++;
+        if (count == nDims - 1) { match[i] = j; break; }
       }
-      if (!paired) fail("oracle: unpaired periodic face");
+    }
+    for (long i = 0; i < np; i++) {
+      if (match[i] < 0) fail("oracle: unpaired periodic face");
+      per_bnd_pairs[*pf[i]] = *pf[match[i]];
     }
   }
 
@@ -692,6 +699,7 @@ struct Oracle {
 
   // cblas_dgemm(RowMajor, NoTrans, NoTrans, alpha=1, beta=0) as used for geometry
   static void dgemm_dense(int M, int N, int K, const real *A, const real *B, real *C) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < M; i++)
       for (int j = 0; j < N; j++) {
         real s = 0.0;
@@ -718,11 +726,12 @@ struct Oracle {
       }
     std::vector<unsigned> idxL(nGfpts_int), idxR(nGfpts_int), idxsort(nGfpts_int);
     for (unsigned i = 0; i < nGfpts_int; i++) idxL[i] = idxR[i] = idxsort[i] = i;
-    unsigned shift = 0;
-    for (unsigned ff = 0; ff < nIntFaces; ff++) {
+    // each face's points are sorted on their own (disjoint slices), so the faces run in parallel
+#pragma omp parallel for schedule(static)
+    for (long ff = 0; ff < (long)nIntFaces; ff++) {
+      const size_t shift = (size_t)ff * nFptsPerFace;
       fuzzysort_ind(cL, nGfpts, idxL.data() + shift, nFptsPerFace, D);
       fuzzysort_ind(cR, nGfpts, idxR.data() + shift, nFptsPerFace, D);
-      shift += nFptsPerFace;
     }
     std::sort(idxsort.begin(), idxsort.end(), [&](unsigned a, unsigned b) { return ptrL[a] < ptrL[b]; });
     auto cpL = idxL, cpR = idxR;
@@ -754,6 +763,8 @@ struct Oracle {
     const int D = nDims;
     auto J = [&](int a, unsigned p, int b, int e) { return jac[(((size_t)a * nPts + p) * D + b) * nEles + e]; };
     auto I = [&](int a, unsigned p, int b, int e) -> real & { return inv[(((size_t)a * nPts + p) * D + b) * nEles + e]; };
+    bool negative = false;
+#pragma omp parallel for schedule(static) reduction(|| : negative)
     for (int e = 0; e < nEles; e++)
       for (unsigned p = 0; p < nPts; p++) {
         real dt;
@@ -772,9 +783,10 @@ struct Oracle {
         }
         if (det) {
           (*det)[(size_t)p * nEles + e] = dt;
-          if (dt < 0) fail("oracle: negative Jacobian at solution points");
+          if (dt < 0) negative = true;
         }
       }
+    if (negative) fail("oracle: negative Jacobian at solution points");
   }
 
   void calc_transforms() {
