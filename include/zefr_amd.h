@@ -321,6 +321,15 @@ int zfr_rk_final_source(const zfr_ele_desc_t *d, double *U, const double *U_ini,
  * sum squares (res_type 0/1/2), reduced on the device; work holds >= 1024*V doubles. */
 int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type, double *work,
                       double *res_out_device, void *stream);
+/* zfr_residual_norm without the work argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  The block partials go to a fixed array in each
+ * device's global memory (1024 x 8 doubles, 64 KiB, allocated with the library's code
+ * object and held until the process ends); this call allocates nothing.  Rejects nVars
+ * above 8.  Every call on a device shares that array: calls that overlap on one device
+ * (other host threads or streams) race, so serialise them. */
+int zfr_residual_norm_static(const zfr_ele_desc_t *d, const double *divF, int res_type,
+                             double *res_out_device, void *stream);
 /* RK update fused into the last kernel of a residual (the stage update of
  * src/solver.cpp:2897-2941 or the final combination of :2997-3062, same arithmetic as
  * zfr_rk_stage / zfr_rk_final): after divF of `stage` is formed, mode 1 writes

@@ -146,6 +146,7 @@ SIGNATURES = {
     "zfr_rk_stage": (_I, [C.POINTER(EleDesc), _P, _P, _P, C.c_double, C.c_double, _P]),
     "zfr_rk_final": (_I, [C.POINTER(EleDesc), _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_residual_norm": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P, _P]),
+    "zfr_residual_norm_static": (_I, [C.POINTER(EleDesc), _P, _I, _P, _P]),
     "zfr_residual_vjp": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, _P, _P]),
     "zfr_face_correction_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P]),
     "zfr_divergence_rk": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P]),

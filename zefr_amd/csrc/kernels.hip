@@ -4,6 +4,9 @@
 
 #include "kernels.hpp"
 
+#include <algorithm>
+#include <mutex>
+
 namespace zk {
 
 // ---------------------------------------------------------------- RK + norm
@@ -492,6 +495,58 @@ int zfr_residual_norm(const zfr_ele_desc_t *d, const double *divF, int res_type,
   if (int rc = norm_checks(d, divF, res_type, res_out, "zfr_residual_norm")) return rc;
   if (!work) return zfr::fail("zfr_residual_norm: missing buffer (work)");
   return norm_launch(d, divF, res_type, work, res_out, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+namespace {
+// The block partials of zfr_residual_norm_static: a fixed array in every device's global
+// memory (64 KiB, held until the process ends), so that call allocates nothing.  Its
+// host-side name is not a device address: host code uses only the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr long kScratchDoubles = (long)kNormMaxBlocks * kNormMaxVars;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int scratch_for_current_device(double **out) {
+  int dev = -1;
+  if (int rc = zfr::check_hip(hipGetDevice(&dev), "zfr_residual_norm_static: hipGetDevice")) return rc;
+  if (dev < 0 || dev >= kMaxDevices)
+    return zfr::fail("zfr_residual_norm_static: device index " + std::to_string(dev) + " beyond " +
+                     std::to_string(kMaxDevices));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (int rc = zfr::check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                                "zfr_residual_norm_static: hipGetSymbolAddress(scratch)"))
+      return rc;
+    if (!p) return zfr::fail("zfr_residual_norm_static: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int zfr_residual_norm_static(const zfr_ele_desc_t *d, const double *divF, int res_type, double *res_out,
+                             void *stream) {
+  if (int rc = norm_checks(d, divF, res_type, res_out, "zfr_residual_norm_static")) return rc;
+  if (d->nVars > kNormMaxVars)
+    return zfr::fail("zfr_residual_norm_static: nVars " + std::to_string(d->nVars) + " exceeds the scratch limit " +
+                     std::to_string(kNormMaxVars));
+  const long need = (long)std::min<long>(zk::grid_for((long)d->nEles * d->nSpts), kNormMaxBlocks) * d->nVars;
+  if (need > kScratchDoubles)
+    return zfr::fail("zfr_residual_norm_static: " + std::to_string(need) + " partials exceed the scratch array of " +
+                     std::to_string(kScratchDoubles) + " doubles");
+  double *part = nullptr;
+  if (int rc = scratch_for_current_device(&part)) return rc;
+  return norm_launch(d, divF, res_type, part, res_out, (hipStream_t)stream);
 }
 
 }  // extern "C"
