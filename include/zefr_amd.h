@@ -356,8 +356,11 @@ int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, co
                       void *stream);
 /* Vector-Jacobian product of one residual (new; no reference counterpart): for the state
  * U [E][V][nS] and a cotangent w of divF (same layout) writes v = (d divF / dU)^T w.
- * Euler, or Navier-Stokes with LDG beta = +-1/2.  d->Utr, Wtr and Xtr are [E][V][nF]
- * scratch buffers (the partner traces are extrapolated from U inside). */
+ * Euler and Navier-Stokes with any LDG beta.  d->Utr, Wtr and Xtr are [E][V][nF]
+ * scratch buffers (the partner traces are extrapolated from U inside).  With beta other
+ * than +-1/2 both sides' gradients enter every common flux: the VJP then also takes
+ * d->Gtr ([E][D][V][nF]) as scratch and runs six launches (forward traces and gradient
+ * traces, face weights, the two-sided face reverse, the element reverse, the gather). */
 int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, double *v, double *Wtr,
                      double *Xtr, void *stream);
 /* Multi-case residual plan (new; no reference counterpart): the residuals of n independent

@@ -48,7 +48,7 @@ def test_roe_rk4_steps_vs_oracle(case):
     assert rel_err(z.state("U"), o.U()) <= 1e-10
 
 
-@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2]], ids=lambda c: c["name"])
+@pytest.mark.parametrize("case", CASES[:4], ids=lambda c: c["name"])
 def test_roe_vjp_vs_oracle_finite_differences(case):
     """<J^T w, d> from the residual VJP with Roe against fourth-order central differences of
     the oracle's Roe residual along d (states away from the flux's kinks)."""

@@ -4,7 +4,7 @@ against finite differences of the residual: the oracle's (CPU) and the product's
 * directional test: <w, J d> by fourth-order central differences of R along d vs.
   <J^T w, d> from the VJP, for random w and d;
 * a Jacobian-transpose assembled column by column by finite differences on a tiny mesh,
-  compared with VJP rows;
+  compared with VJP rows (one-sided LDG beta = +-1/2 and general beta);
 * torch autograd through R and through RK4 steps of the kinetic energy vs. central
   finite differences (eps = 1e-7) of the same functional.
 Test states are random perturbations of a uniform flow, so interface jumps keep the
@@ -62,6 +62,14 @@ CASES = [
          perturb=0.03),
     dict(name="vjp_h_ns_p3", n=(2, 2, 2), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=3, viscous=True, state="smooth"),
     dict(name="vjp_h_euler_p1", n=(3, 3, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=1, viscous=False, state="smooth"),
+    # general LDG beta: both sides' gradients in every common flux, the common solution
+    # a weighted mix of both traces (src/faces.cpp:1366-1474, 1520-1545)
+    dict(name="vjp_q_ns_p3_beta02_curved", n=(3, 4), lo=(0.0, 0.0), hi=(1.0, 1.5), order=3, viscous=True,
+         state="smooth", perturb=0.04, extra={"ldg_b": 0.2}),
+    dict(name="vjp_h_ns_p2_beta_m03_sutherland", n=(2, 2, 3), lo=(0.0,) * 3, hi=(1.0,) * 3, order=2, viscous=True,
+         state="smooth", extra={"ldg_b": -0.3, "fix_vis": 0, "disable_nondim": 0, "rus_k": 0.1}),
+    dict(name="vjp_h_ns_p3_beta0_roe", n=(2, 2, 2), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=3, viscous=True,
+         state="smooth", extra={"ldg_b": 0.0, "ldg_tau": 0.5, "fconv_type": "Roe"}),
 ]
 
 
@@ -100,7 +108,7 @@ def test_forward_residual_at_matches_solver(case):
     assert np.abs(to_ref(got) - want).max() <= 1e-11 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("viscous,beta", [(False, 0.5), (True, 0.5), (True, -0.5)])
+@pytest.mark.parametrize("viscous,beta", [(False, 0.5), (True, 0.5), (True, -0.5), (True, 0.2), (True, 0.0)])
 def test_vjp_equals_column_assembled_jacobian_transpose(viscous, beta):
     """Tiny mesh (2x2 quads, p=1: 64 unknowns): J assembled column by column from central
     differences of the product's own forward residual; every VJP row e_i^T J^T ... is
@@ -126,6 +134,27 @@ def test_vjp_equals_column_assembled_jacobian_transpose(viscous, beta):
         w[i] = 1.0
         JT[:, i] = z.residual_vjp(Ut, torch.from_numpy(w.reshape(Ud.shape)).cuda()).cpu().numpy().ravel()
     assert np.abs(JT - J.T).max() <= 1e-7 * np.abs(J).max()
+
+
+@pytest.mark.parametrize("beta", [0.5, 0.2])
+def test_vjp_dot_product_with_forward_linearisation(beta):
+    """<w, J d> = <J^T w, d> with J d from central differences of the product's own
+    forward residual (eps 1e-6, second order) on 3^3 hexes at p=2: the VJP is the exact
+    transpose of the forward linearisation, whatever the LDG branch."""
+    import torch
+    case = dict(name=f"vjp_dot_{beta}", n=(3, 3, 3), lo=(0.0,) * 3, hi=(1.0,) * 3, order=2, viscous=True,
+                state="smooth", perturb=0.02, extra={"ldg_b": beta})
+    z, o, _ = paired(case)
+    U = to_dev(noisy_state(o, 11))
+    R = lambda x: z.residual_at(torch.from_numpy(x).cuda()).cpu().numpy()  # noqa: E731
+    rng = np.random.default_rng(12)
+    for _ in range(2):
+        w = rng.standard_normal(U.shape)
+        d = rng.standard_normal(U.shape)
+        jd = fd4(R, U, d, 1e-5)
+        v = z.residual_vjp(torch.from_numpy(U).cuda(), torch.from_numpy(w).cuda()).cpu().numpy()
+        lhs, rhs = np.sum(v * d), np.sum(w * jd)
+        assert abs(lhs - rhs) <= 1e-8 * (np.sum(np.abs(w * jd)) + 1e-300), (lhs, rhs)
 
 
 def test_autograd_residual_backward():

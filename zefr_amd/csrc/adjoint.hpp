@@ -17,6 +17,9 @@
 //   k_adj_gather : v += E^T (partner Xtr) at the points the partner owns.
 #pragma once
 
+// k_adj_gather's BM for general LDG beta: the partner's cotangent at every flux point.
+constexpr int kAdjAllPoints = 3;
+
 // Stage EB consecutive [V][nS] blocks of x into LDS (stride V*nS).
 template <int D, int N>
 __device__ __forceinline__ void stage_block(const double *__restrict__ x, double *lx, long e0, long neb) {
@@ -74,16 +77,20 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restri
 }
 
 // LDS per element: U [V][nS]; one region that holds the common solution, then the forward
-// gradient [D][V][nS]; the gradient cotangent [D][V][nS] (viscous); the own trace
-// cotangent [V][nF]; the owned-face mask.  w is read from global memory (each value by
-// the few threads of its lines, from cache).  The partner's share of the trace cotangent
-// goes straight to Xtr; __syncthreads orders this block's global accesses too.
+// gradient [D][V][nS], then (viscous, once the face adjoint has read the gradient) the own
+// trace cotangent [V][nF]; the gradient cotangent [D][V][nS] (viscous); the own trace
+// cotangent in a region of its own (Euler); the owned-face mask.  w is read from global
+// memory (each value by the few threads of its lines, from cache).  The partner's share
+// of the trace cotangent goes straight to Xtr; __syncthreads orders this block's global
+// accesses too.
 template <int D, int N, bool VISC>
 struct AdjLds {
   using S = Shape<D, N>;
-  static constexpr int R = VISC ? ((S::V * S::nF > D * S::V * S::nS) ? S::V * S::nF : D * S::V * S::nS) : 0;
+  static constexpr int R0 = (S::V * S::nF > D * S::V * S::nS) ? S::V * S::nF : D * S::V * S::nS;
+  static constexpr int R = VISC ? R0 : 0;
   static constexpr int GB = VISC ? D * S::V * S::nS : 0;
-  static constexpr int per_ele = S::V * S::nS + R + GB + S::V * S::nF + 1;  // doubles
+  static constexpr int TO = VISC ? 0 : S::V * S::nF;
+  static constexpr int per_ele = S::V * S::nS + R + GB + TO + 1;  // doubles
 };
 
 template <int D, int N, bool VISC, int BM, int FC>
@@ -101,7 +108,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
   double *lR = lU + EB * V * nS;
   double *lGB = lR + EB * L::R;
   double *lTo = lGB + EB * L::GB;
-  unsigned *lmask = reinterpret_cast<unsigned *>(lTo + EB * V * nF);  // owned faces per element
+  unsigned *lmask = reinterpret_cast<unsigned *>(lTo + EB * L::TO);  // owned faces per element
   const long e0 = bid * EB;
   const long neb = min((long)EB, (long)a.E - e0);
   stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
@@ -113,7 +120,8 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
   const double *u = lU + el * V * nS;
   const double *we = w + e * V * nS;
   double *c = lR + el * L::R, *g_l = lR + el * L::R;
-  double *gb = lGB + el * L::GB, *to = lTo + el * V * nF;
+  // viscous: the trace cotangent takes the gradient's region after the face adjoint
+  double *gb = lGB + el * L::GB, *to = VISC ? lR + el * L::R : lTo + el * L::TO;
   const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
   // 1. owned list, zeroed own-trace cotangent, common solution (viscous)
   if (valid) {
@@ -142,8 +150,10 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
           for (int v = 0; v < V; v++) c[v * nF + f] = line_extrap<N>(lE, u + v * nS, base, str);
         }
       }
+      if (!VISC) {
 #pragma unroll
-      for (int v = 0; v < V; v++) to[v * nF + f] = 0.0;
+        for (int v = 0; v < V; v++) to[v * nF + f] = 0.0;
+      }
     }
   }
   __syncthreads();
@@ -208,6 +218,10 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
   //    accumulation order is fixed)
   const unsigned mask = valid ? lmask[el] : 0u;
   const int cnt = __popc(mask) * S::nFF;
+  double uob[S::FPT][V];  // viscous: the owned points' trace cotangents, stored once the gradient is read
+  int fown[S::FPT];
+#pragma unroll
+  for (int kk = 0; kk < S::FPT; kk++) fown[kk] = -1;
 #pragma unroll
   for (int kk = 0; kk < S::FPT; kk++) {
     // viscous runs use one element per workgroup, so cnt is the same for every thread and
@@ -264,8 +278,14 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
             },
             a, Uob);
       }
+      if (VISC) {
+        fown[kk] = f;
 #pragma unroll
-      for (int v = 0; v < V; v++) to[v * nF + f] = Uob[v];
+        for (int v = 0; v < V; v++) uob[kk][v] = Uob[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) to[v * nF + f] = Uob[v];
+      }
     }
     if (VISC) {
       const int face = has ? f / S::nFF : -1;
@@ -286,6 +306,14 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
   }
   __syncthreads();
   if (VISC) {
+    // the forward gradient is consumed: its region takes the trace cotangent, owned points
+    // from the face adjoint, the others in phase 6
+#pragma unroll
+    for (int kk = 0; kk < S::FPT; kk++) {
+      if (fown[kk] < 0) continue;
+#pragma unroll
+      for (int v = 0; v < V; v++) to[v * nF + fown[kk]] = uob[kk][v];
+    }
     // 5. reference-space gradient cotangent: grb[d2][v] = sum_d1 adj[d2][d1] gb[v][d1] / detJ
     if (valid) {
       double gv[V][D];
@@ -328,7 +356,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
         for (int v = 0; v < V; v++) {
           const double cb = line_Dsf<N>(lops, side, gb + (dim * V + v) * nS, base, str);
           if (sl == owner) Xtr[(e * V + v) * nF + f] += cb;
-          else to[v * nF + f] += cb;
+          else to[v * nF + f] = cb;  // this point's only contribution (the face adjoint wrote owned ones)
         }
       }
     }
@@ -391,7 +419,8 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_gather(const double *__rest
       long ep; int sl, fp;
       unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
 #pragma unroll
-      for (int v = 0; v < V; v++) T[v * nF + f] = (sl == owner) ? 0.0 : Xtr[(ep * V + v) * nF + fp];
+      for (int v = 0; v < V; v++)
+        T[v * nF + f] = (BM != kAdjAllPoints && sl == owner) ? 0.0 : Xtr[(ep * V + v) * nF + fp];
     }
   }
   __syncthreads();
@@ -410,4 +439,292 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_gather(const double *__rest
   }
 #pragma unroll
   for (int v = 0; v < V; v++) vout[(e * V + v) * nS + t] = acc[v];
+}
+
+// ---------------------------------------------------------------- general LDG beta
+// With beta not +-1/2 both sides' gradients enter every common flux (src/faces.cpp:
+// 1476-1614) and the common solution mixes both traces, Ucomm = (1/2 - beta) UL +
+// (1/2 + beta) UR (src/faces.cpp:1366-1474).  The reverse runs after the forward trace
+// and gradient-trace launches (Utr, Gtr at every flux point) and k_adj_prep:
+//   k_adj_face     : per flux-point pair, by its slot-0 side: the common flux reversed for
+//                    both sides' states and gradient traces; the cotangents overwrite the
+//                    traces in place (Utr -> d/dUtr, Gtr -> d/dGtr);
+//   k_adj_main_gen : per element: volume adjoint, the gradient-trace cotangents spread
+//                    back along the normal lines, the gradient reversed into U and into the
+//                    common solution, whose cotangent splits (1/2 -+ beta) between this
+//                    element's trace (with the face part, E^T into v) and the partner's
+//                    (Xtr, at every point);
+//   k_adj_gather   : v += E^T Xtr(partner) at every point (BM = kAdjAllPoints).
+
+template <int D, int N, int FC>
+__global__ __launch_bounds__(kMaxThreads) void k_adj_face(const double *__restrict__ Wtr, KArgs a) {
+  using S = Shape<D, N>;
+  constexpr int nF = S::nF, V = S::V;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)a.E * nF) return;
+  const long e = i / nF;
+  const int f = (int)(i - e * nF);
+  long ep; int sl, fp;
+  unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+  if (sl != 0) return;  // each pair is reversed once, by its slot-0 (left) side
+  const double *fg = a.fgeo + e * (D + 1) * nF + f;
+  double n[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) n[d] = fg[d * nF];
+  double UL[V], UR[V], Fb[V], ULb[V], URb[V], G1b[V], G2b[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    UL[v] = a.Utr[(e * V + v) * nF + f];
+    UR[v] = a.Utr[(ep * V + v) * nF + fp];
+    // both sides' copies of the flux are the same function of (UL, UR, GL, GR)
+    Fb[v] = Wtr[(e * V + v) * nF + f] + Wtr[(ep * V + v) * nF + fp];
+    ULb[v] = URb[v] = G1b[v] = G2b[v] = 0.0;
+  }
+  common_flux_vjp<D, true, 0, FC>(UL, UR, n, 1.0, a, Fb, ULb, URb, G1b, G2b);
+  // each side's viscous normal flux G = -T(U, grad) . n, reversed (gradient from Gtr)
+  double GLb[V][D], GRb[V][D];
+  {
+    const Prim<D> pL = primitives<D>(UL, a.gamma);
+    viscous_vjp<D>(
+        UL, pL,
+        [&](int j, double *dUj) {
+#pragma unroll
+          for (int v = 0; v < V; v++) dUj[v] = a.Gtr[((e * D + j) * V + v) * nF + f];
+        },
+        [&](int v, int j) { return -G1b[v] * n[j]; },
+        [&](int j, const double *gbj) {
+#pragma unroll
+          for (int v = 0; v < V; v++) GLb[v][j] = gbj[v];
+        },
+        a, ULb);
+  }
+  {
+    const Prim<D> pR = primitives<D>(UR, a.gamma);
+    viscous_vjp<D>(
+        UR, pR,
+        [&](int j, double *dUj) {
+#pragma unroll
+          for (int v = 0; v < V; v++) dUj[v] = a.Gtr[((ep * D + j) * V + v) * nF + fp];
+        },
+        [&](int v, int j) { return -G2b[v] * n[j]; },
+        [&](int j, const double *gbj) {
+#pragma unroll
+          for (int v = 0; v < V; v++) GRb[v][j] = gbj[v];
+        },
+        a, URb);
+  }
+  // in place: this pair's two entries are read and written by this thread only
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    a.Utr[(e * V + v) * nF + f] = ULb[v];
+    a.Utr[(ep * V + v) * nF + fp] = URb[v];
+  }
+#pragma unroll
+  for (int j = 0; j < D; j++)
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      a.Gtr[((e * D + j) * V + v) * nF + f] = GLb[v][j];
+      a.Gtr[((ep * D + j) * V + v) * nF + fp] = GRb[v][j];
+    }
+}
+
+// LDS per element: U [V][nS]; one region for the common solution, then the forward
+// gradient [D][V][nS], then the trace cotangent [V][nF]; the gradient cotangent [D][V][nS].
+template <int D, int N>
+struct AdjGenLds {
+  using S = Shape<D, N>;
+  static constexpr int R = (S::V * S::nF > D * S::V * S::nS) ? S::V * S::nF : D * S::V * S::nS;
+  static constexpr int per_ele = S::V * S::nS + R + D * S::V * S::nS;  // doubles
+};
+
+template <int D, int N>
+__device__ __forceinline__ void adj_main_general_block(const double *__restrict__ U, const double *__restrict__ w,
+                                                       double *__restrict__ Xtr, double *__restrict__ vout,
+                                                       const KArgs &a, long bid) {
+  using S = Shape<D, N>;
+  using L = AdjGenLds<D, N>;
+  constexpr int nS = S::nS, nF = S::nF, V = S::V;
+  extern __shared__ double lds[];
+  const int EB = a.EB;
+  double *lops = lds;
+  double *lU = lops + S::NOPS_PAD;
+  double *lR = lU + EB * V * nS;
+  double *lGB = lR + EB * L::R;
+  const long e0 = bid * EB;
+  const long neb = min((long)EB, (long)a.E - e0);
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  __syncthreads();
+  const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
+  const long e = e0 + el;
+  const bool valid = el < neb;
+  const double *u = lU + el * V * nS;
+  const double *we = w + e * V * nS;
+  double *c = lR + el * L::R, *g_l = c, *to = c;
+  double *gb = lGB + el * D * V * nS;
+  const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
+  const double beta = a.ldg_b;
+  // 1. common solution, the partner's trace extrapolated from its U (Utr now holds cotangents)
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++) {
+      const int f = t + k * nS;
+      if (f >= nF) continue;
+      int dim, side, base, str;
+      fpt_info<D, N>(f, dim, side, base, str);
+      const double *lE = tab_lE<N>(lops, side);
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      double pt[V];
+      partner_trace<D, N>(U, a.ops, ep, fp, pt);
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        const double own = line_extrap<N>(lE, u + v * nS, base, str);
+        const double l = sl ? pt[v] : own, r = sl ? own : pt[v];
+        c[v * nF + f] = 0.5 * (l + r) - beta * (l - r);
+      }
+    }
+  }
+  __syncthreads();
+  double ij[D][D], detJ = 1.0;
+  if (valid) load_metric<D>(a, e, t, nS, ij, detJ, true);
+  // 2. forward physical gradient, into the common solution's region
+  {
+    double g[V][D];
+    if (valid) spt_gradient<D, N>(lops, u, c, t, ij, detJ, g);
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = g[v][d];
+    }
+    __syncthreads();
+  }
+  // 3. volume adjoint: tFb[d1] = Dss^T w along the d1 lines; F(U, grad) reversed
+  double Ub[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) Ub[v] = 0.0;
+  if (valid) {
+    double Us[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
+    const Prim<D> p = primitives<D>(Us, a.gamma);
+    double tFb[D][V];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+      const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
+      const int base = t - id * str;
+#pragma unroll
+      for (int v = 0; v < V; v++) tFb[d][v] = line_T_Dss<N>(lops, id, we + v * nS, base, str);
+      conv_proj_vjp<D>(Us, p, ij[d], tFb[d], a.gamma, Ub);
+    }
+    viscous_vjp<D>(
+        Us, p,
+        [&](int j, double *dUj) {
+#pragma unroll
+          for (int v = 0; v < V; v++) dUj[v] = g_l[(j * V + v) * nS + t];
+        },
+        [&](int v, int j) {
+          double s = 0.0;
+#pragma unroll
+          for (int d1 = 0; d1 < D; d1++) s += tFb[d1][v] * ij[d1][j];
+          return -s;
+        },
+        [&](int j, const double *gbj) {
+#pragma unroll
+          for (int v = 0; v < V; v++) gb[(j * V + v) * nS + t] = gbj[v];
+        },
+        a, Ub);
+  }
+  // 4. the gradient-trace cotangents (k_adj_face, in Gtr) spread back along the normal
+  //    lines, face by face (lines of one face never meet, so the order is fixed)
+#pragma unroll 1
+  for (int q = 0; q < 2 * D; q++) {
+    __syncthreads();
+    if (!valid) continue;
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++) {
+      const int f = t + k * nS;
+      if (f >= nF || f / S::nFF != q) continue;
+      int dim, side, base, str;
+      fpt_info<D, N>(f, dim, side, base, str);
+      const double *lE = tab_lE<N>(lops, side);
+#pragma unroll
+      for (int j = 0; j < D; j++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          const double gt = a.Gtr[((e * D + j) * V + v) * nF + f];
+#pragma unroll
+          for (int m = 0; m < N; m++) gb[(j * V + v) * nS + base + m * str] += lE[m] * gt;
+        }
+    }
+  }
+  __syncthreads();
+  // 5. reference-space gradient cotangent: grb[d2][v] = sum_d1 adj[d2][d1] gb[v][d1] / detJ
+  if (valid) {
+    double gv[V][D];
+#pragma unroll
+    for (int j = 0; j < D; j++)
+#pragma unroll
+      for (int v = 0; v < V; v++) gv[v][j] = gb[(j * V + v) * nS + t];
+    const double inv_det = 1.0 / detJ;
+#pragma unroll
+    for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double s = 0.0;
+#pragma unroll
+        for (int d1 = 0; d1 < D; d1++) s += ij[d2][d1] * gv[v][d1];
+        gb[(d2 * V + v) * nS + t] = s * inv_det;
+      }
+  }
+  __syncthreads();
+  // 6. Dss^T into U; Dsf^T into the common solution's cotangent, split between this
+  //    element's trace (added to its face cotangent from Utr) and the partner's (Xtr)
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+      const int str = (d == 0) ? 1 : ((d == 1) ? N : N * N);
+      const int base = t - id * str;
+#pragma unroll
+      for (int v = 0; v < V; v++) Ub[v] += line_T_Dss<N>(lops, id, gb + (d * V + v) * nS, base, str);
+    }
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++) {
+      const int f = t + k * nS;
+      if (f >= nF) continue;
+      int dim, side, base, str;
+      fpt_info<D, N>(f, dim, side, base, str);
+      const int sl = (int)((a.nbr[e * nF + f] >> 8) & 1u);
+      const double ao = sl ? (0.5 + beta) : (0.5 - beta), ap = sl ? (0.5 - beta) : (0.5 + beta);
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        const double cb = line_Dsf<N>(lops, side, gb + (dim * V + v) * nS, base, str);
+        to[v * nF + f] = a.Utr[(e * V + v) * nF + f] + ao * cb;
+        Xtr[(e * V + v) * nF + f] = ap * cb;
+      }
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  // 7. own trace cotangent E^T into v
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int id = (d == 0) ? ia : ((d == 1) ? ib : ic);
+    const double l0 = tab_lE<N>(lops, 0)[id], l1 = tab_lE<N>(lops, 1)[id];
+    const int f0 = fpt_end<D, N>(ia, ib, ic, d, 0), f1 = fpt_end<D, N>(ia, ib, ic, d, 1);
+#pragma unroll
+    for (int v = 0; v < V; v++) Ub[v] += l0 * to[v * nF + f0] + l1 * to[v * nF + f1];
+  }
+#pragma unroll
+  for (int v = 0; v < V; v++) vout[(e * V + v) * nS + t] = Ub[v];
+}
+
+template <int D, int N>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void k_adj_main_gen(const double *__restrict__ U, const double *__restrict__ w, double *__restrict__ Xtr,
+                    double *__restrict__ vout, KArgs a) {
+  adj_main_general_block<D, N>(U, w, Xtr, vout, a, xcd_block(blockIdx.x, gridDim.x));
 }

@@ -562,6 +562,17 @@ int zfr_residual_vjp(const zfr_ele_desc_t *d, const double *U, const double *w, 
   j.Wtr = Wtr;
   j.Xtr = Xtr;
   const hipStream_t st = (hipStream_t)stream;
+  if (d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5) {
+    // general LDG beta: forward traces and gradient traces, then the two-sided reverse
+    if (!d->Gtr) return zfr::fail("zfr_residual_vjp: general LDG beta needs the gradient trace buffer Gtr");
+    int rc = zk::dispatch(zk::Op::Extrap, d, U, nullptr, st);
+    if (!rc) rc = zk::dispatch(zk::Op::Grad, d, U, nullptr, st);
+    if (!rc) rc = zk::dispatch(zk::Op::AdjPrep, d, U, nullptr, st, j);
+    if (!rc) rc = zk::dispatch(zk::Op::AdjFace, d, U, nullptr, st, j);
+    if (!rc) rc = zk::dispatch(zk::Op::AdjMainGen, d, U, v, st, j);
+    if (!rc) rc = zk::dispatch(zk::Op::AdjGather, d, U, v, st, j);
+    return rc;
+  }
   int rc = zk::dispatch(zk::Op::AdjPrep, d, U, nullptr, st, j);
   if (!rc) rc = zk::dispatch(zk::Op::AdjMain, d, U, v, st, j);
   if (!rc) rc = zk::dispatch(zk::Op::AdjGather, d, U, v, st, j);

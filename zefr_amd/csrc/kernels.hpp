@@ -1077,7 +1077,11 @@ inline KArgs make_args(const zfr_ele_desc_t *d, int EB) {
   return a;
 }
 
-enum class Op { Extrap, ExtrapPartner, Grad, Div, OneSided, OneSidedFromTrace, FaceCorr, AdjPrep, AdjMain, AdjGather };
+enum class Op {
+  Extrap, ExtrapPartner, Grad, Div, OneSided, OneSidedFromTrace, FaceCorr,
+  AdjPrep, AdjMain, AdjGather,  // reverse mode (Euler, LDG beta = +-1/2; AdjPrep and AdjGather also general beta)
+  AdjFace, AdjMainGen           // reverse mode, general LDG beta
+};
 
 // Extra operands of the reverse-mode launches.
 struct AdjPtrs {
@@ -1109,6 +1113,8 @@ size_t lds_per_element(Op op, bool viscous) {
       per_ele = viscous ? (size_t)AdjLds<D, N, true>::per_ele : (size_t)AdjLds<D, N, false>::per_ele;
       break;
     case Op::AdjGather: per_ele = (size_t)S::V * S::nF; break;
+    case Op::AdjFace: per_ele = 0; break;
+    case Op::AdjMainGen: per_ele = (size_t)AdjGenLds<D, N>::per_ele; break;
   }
   return per_ele;
 }

@@ -48,9 +48,12 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
     return zfr::fail("kernel launch: element descriptor does not match (nDims, order)");
   if (d->nEles <= 0) return 0;
   const size_t per_ele = lds_per_element<D, N>(op, d->viscous != 0);
-  const bool adj = op == Op::AdjPrep || op == Op::AdjMain || op == Op::AdjGather;
-  if (adj && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5)
-    return zfr::fail("the residual VJP supports LDG beta = +-0.5 (and Euler)");
+  const bool general = d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5;
+  if (op == Op::AdjMain && general) return zfr::fail("k_adj_main is the one-sided LDG adjoint (beta = +-0.5)");
+  if ((op == Op::AdjFace || op == Op::AdjMainGen) && !general)
+    return zfr::fail("the general-beta adjoint launches need a viscous descriptor with beta != +-0.5");
+  if ((op == Op::AdjFace || op == Op::AdjMainGen) && (!d->Utr || !d->Gtr))
+    return zfr::fail("the general-beta adjoint needs the trace buffers Utr and Gtr");
   if (d->fconv_type != 0 && d->fconv_type != 2) return zfr::fail("fconv_type must be 0 (Rusanov) or 2 (Roe)");
   if ((op == Op::OneSided || op == Op::OneSidedFromTrace || op == Op::FaceCorr) && d->ldg_b != 0.5 &&
       d->ldg_b != -0.5)
@@ -108,8 +111,19 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       else launch_adj_main<D, N, 0>(d, grid, block, lds, st, U, j, out, a);
       break;
     case Op::AdjGather:
-      if (d->viscous && d->ldg_b == -0.5) hipLaunchKernelGGL((k_adj_gather<D, N, 2>), grid, block, lds, st, j.Xtr, out, a);
+      if (general) hipLaunchKernelGGL((k_adj_gather<D, N, kAdjAllPoints>), grid, block, lds, st, j.Xtr, out, a);
+      else if (d->viscous && d->ldg_b == -0.5) hipLaunchKernelGGL((k_adj_gather<D, N, 2>), grid, block, lds, st, j.Xtr, out, a);
       else hipLaunchKernelGGL((k_adj_gather<D, N, 0>), grid, block, lds, st, j.Xtr, out, a);
+      break;
+    case Op::AdjFace: {  // one thread per (element, flux point); slot-0 points work
+      const long total = (long)d->nEles * S::nF;
+      const dim3 fgrid((unsigned)((total + kMaxThreads - 1) / kMaxThreads)), fblock(kMaxThreads);
+      if (d->fconv_type == 2) hipLaunchKernelGGL((k_adj_face<D, N, 1>), fgrid, fblock, 0, st, j.Wtr, a);
+      else hipLaunchKernelGGL((k_adj_face<D, N, 0>), fgrid, fblock, 0, st, j.Wtr, a);
+      break;
+    }
+    case Op::AdjMainGen:
+      hipLaunchKernelGGL((k_adj_main_gen<D, N>), grid, block, lds, st, U, j.w, j.Xtr, out, a);
       break;
   }
   return zfr::check_hip(hipGetLastError(), "kernel launch");

@@ -427,7 +427,6 @@ void FRSolver::residual_at(const double *U, double *divF, hipStream_t st) {
 
 void FRSolver::residual_vjp(const double *U, const double *w, double *v, hipStream_t st) {
   need_device(*this);
-  if (input->viscous && !onesided()) raise("the residual VJP needs LDG beta = +-0.5 (or an inviscid run)");
   Elements &el = *eles;
   const size_t tr = (size_t)el.nEles * el.nVars * el.nFpts * sizeof(double);
   if (!Wtr_d.p) Wtr_d.alloc(tr);
