@@ -3,7 +3,8 @@
 * The transfer kernel (zfr_pmg_transfer) for every pair of orders 0..5 in 2D and 3D against
   the oracle's dense tensor operators (oppRes / oppPro as Quads/Hexas::setup_PMG build them).
 * V-cycles on quads (the 4 -> 3 -> 2 -> 1 -> 0 level list of the 128^2 steady case, on a
-  16^2 mesh with the seeded noisy freestream), on quads with skipped orders and rel_fac < 1,
+  16^2 mesh and on the stated 128^2 mesh, with the seeded noisy freestream), on quads with
+  skipped orders and rel_fac < 1,
   on hexes (Navier-Stokes), and one FMG cycle: the fine solution after each cycle against
   the oracle (1e-12 norm-wise per variable), and every coarse level's source term, stored
   solution, correction and state.
@@ -67,6 +68,10 @@ CASES = [
     _pmg_case("pmg_hex_ns", (3, 3, 3), (0.0,) * 3, (TWO_PI,) * 3, 3, True, "tgv3d", [3, 2, 0], [1, 2, 2]),
     _pmg_case("pmg_hex_euler", (3, 3, 3), (0.0,) * 3, (1.0,) * 3, 2, False, "noisy:3", [2, 1], [1, 3],
               dt=2e-4),
+    # the quad half of the steady multigrid configuration at its stated size: 128^2 quads,
+    # p = 4, levels 4 -> 3 -> 2 -> 1 -> 0, one smoothing step per level
+    _pmg_case("pmg_quad_p4_128", (128, 128), (0.0, 0.0), (1.0, 1.0), 4, False, "noisy:7", [4, 3, 2, 1, 0],
+              [1, 1, 1, 1, 1], dt=1e-4),
 ]
 
 

@@ -36,3 +36,22 @@ def test_sweep_residuals_vs_oracle():
         worst = max(worst, err)
         del z, o
     print(f"[sweep] worst relative L-inf {worst:.3e}", flush=True)
+
+
+def test_sweep_batch_graph_vs_oracle():
+    """The same 64 cases through the batch graph (zefr.Batch: the members' residuals as a
+    multi-case plan, one launch for all five orders): every member against its oracle."""
+    from zefr_amd import zefr
+    zs, os_ = [], []
+    for case in sweep_cases():
+        z, o, _ = paired(case)
+        o.compute_residual(0)
+        zs.append(z)
+        os_.append(o)
+    b = zefr.Batch(zs)
+    print(f"[sweep] batch of {len(zs)} cases in {b.launches} launch(es)", flush=True)
+    assert b.launches == 1
+    b.residual()
+    for case, z, o in zip(sweep_cases(), zs, os_):
+        err, kappa, ok = residual_parity(z.state("divF", 0), o)
+        assert ok, f"batch {case['name']}: relative L-inf {err:.3e} (cancellation factor {kappa:.1e})"

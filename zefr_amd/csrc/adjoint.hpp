@@ -269,7 +269,10 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
             Uo, po,
             [&](int j, double *dUj) {
 #pragma unroll
-              for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+              for (int v = 0; v < V; v++) {
+                dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+                asm volatile("" : "+v"(dUj[v]) : : "memory");  // formed here, before the next line is loaded
+              }
             },
             [&](int v, int j) { return -Gb[v] * n[j]; },
             [&](int j, const double *gbj) {

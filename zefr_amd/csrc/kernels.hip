@@ -215,6 +215,10 @@ int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipSt
 
 template <int D>
 void dispatch_multi_n(const MultiGroup &g, hipStream_t st) {
+  if (g.N == 0) {
+    launch_mixed<D>(g, st);
+    return;
+  }
   switch (g.N) {
     case 1: launch_multi<D, 1>(g, st); break;
     case 2: launch_multi<D, 2>(g, st); break;
@@ -620,13 +624,14 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
     if (d->nEles <= 0) continue;
     if (visc && bm != 0) {
       if (!d->Ftr) return zfr::fail("zfr_multi_create: the one-sided path needs Ftr");
-      phase[1][{Op::OneSided, D, N, 1, bm, fc}].push_back(i);  // extrapolates the partner traces itself
-      phase[2][{Op::FaceCorr, D, N, 1, bm, 0}].push_back(i);
+      // one launch per phase for every order (N = 0: mixed-order launch)
+      phase[1][{Op::OneSided, D, 0, 1, bm, fc}].push_back(i);  // extrapolates the partner traces itself
+      phase[2][{Op::FaceCorr, D, 0, 1, bm, 0}].push_back(i);
     } else {
       if (visc && !d->Gtr) return zfr::fail("zfr_multi_create: the general LDG path needs Gtr");
       if (visc) phase[0][{Op::Extrap, D, N, 0, 0, 0}].push_back(i);  // Euler: extrapolated in the divergence
       if (visc) phase[1][{Op::Grad, D, N, 1, 0, 0}].push_back(i);
-      phase[2][{Op::Div, D, N, visc, bm, fc}].push_back(i);
+      phase[2][{Op::Div, D, visc ? N : 0, visc, bm, fc}].push_back(i);
     }
   }
   auto m = new zfr_multi_s();
@@ -637,34 +642,47 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
       zk::MultiGroup g;
       g.op = k.op; g.D = k.D; g.N = k.N; g.visc = k.visc; g.bm = k.bm; g.fc = k.fc;
       g.nc = (int)ids.size();
-      const size_t per_ele = (k.D == 2) ? zk::lds_per_element_n<2>(k.N, k.op, k.visc != 0)
-                                        : zk::lds_per_element_n<3>(k.N, k.op, k.visc != 0);
-      const int N = k.N;
-      const int nS = (k.D == 2) ? N * N : N * N * N;
-      const int nops = 2 * N + N * N + 2 * N, nops_pad = (nops + 7) & ~7;
-      // one tile for the group, chosen for its total element count as for one solver
-      zfr_ele_desc_t agg = descs[ids[0]];
-      agg.nEles = 0;
-      for (int i : ids) agg.nEles += descs[i].nEles;
-      agg.eles_per_block = 0;
-      const int defEB = (zk::kMaxThreads / nS) > 0 ? zk::kMaxThreads / nS : 1;
-      g.EB = zk::eles_per_block(&agg, defEB, nops_pad, per_ele);
-      g.lds = sizeof(double) * (nops_pad + g.EB * per_ele);
-      if (g.lds > zk::kMaxLds) {
-        delete m;
-        return zfr::fail("zfr_multi_create: one element needs more LDS than a workgroup may use");
+      // one tile per order, chosen for that order's total element count in the group, as
+      // for one solver; the launch takes the largest LDS of its orders
+      std::map<int, int> eb_of;
+      g.lds = 0;
+      g.EB = 0;
+      for (int i : ids) {
+        const int N = descs[i].order + 1;
+        if (eb_of.count(N)) continue;
+        const size_t per_ele = (k.D == 2) ? zk::lds_per_element_n<2>(N, k.op, k.visc != 0)
+                                          : zk::lds_per_element_n<3>(N, k.op, k.visc != 0);
+        const int nS = (k.D == 2) ? N * N : N * N * N;
+        const int nops = 2 * N + N * N + 2 * N, nops_pad = (nops + 7) & ~7;
+        zfr_ele_desc_t agg = descs[i];
+        agg.nEles = 0;
+        for (int i2 : ids)
+          if (descs[i2].order + 1 == N) agg.nEles += descs[i2].nEles;
+        agg.eles_per_block = 0;
+        const int defEB = (zk::kMaxThreads / nS) > 0 ? zk::kMaxThreads / nS : 1;
+        const int eb = zk::eles_per_block(&agg, defEB, nops_pad, per_ele);
+        eb_of[N] = eb;
+        const size_t lds = sizeof(double) * (nops_pad + eb * per_ele);
+        if (lds > zk::kMaxLds) {
+          delete m;
+          return zfr::fail("zfr_multi_create: one element needs more LDS than a workgroup may use");
+        }
+        g.lds = std::max(g.lds, lds);
+        g.EB = std::max(g.EB, eb);
       }
       std::vector<zk::MultiCase> hc(ids.size());
       std::vector<long> hs(ids.size());
       long blocks = 0;
       for (size_t q = 0; q < ids.size(); q++) {
         const int i = ids[q];
-        hc[q].a = zk::make_args(&descs[i], g.EB);
+        const int N = descs[i].order + 1, eb = eb_of[N];
+        hc[q].a = zk::make_args(&descs[i], eb);
         hc[q].a.trace_sel = (k.op == Op::ExtrapPartner) ? 1 : 0;
         hc[q].U = U[i];
         hc[q].out = (k.op == Op::Div || k.op == Op::OneSided || k.op == Op::FaceCorr) ? divF[i] : nullptr;
+        hc[q].N = N;
         hs[q] = blocks;
-        blocks += (descs[i].nEles + g.EB - 1) / g.EB;
+        blocks += (descs[i].nEles + eb - 1) / eb;
       }
       if (blocks > 0x7fffffffL) {
         delete m;

@@ -111,6 +111,7 @@ struct MultiCase {
   KArgs a;
   const double *U;
   double *out;
+  int N;  // points per direction of the case's elements (the order of a mixed-order launch)
 };
 
 // The case of workgroup b: binary search over the cases' first workgroups (uniform across
@@ -603,7 +604,10 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
             own_state(Us);
             visc_normal<D>(Us, [&](int j, double *dUj) {
 #pragma unroll
-              for (int v = 0; v < V; v++) dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+              for (int v = 0; v < V; v++) {
+                dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
+                asm volatile("" : "+v"(dUj[v]) : : "memory");  // formed here, before the next line is loaded
+              }
             }, n, a, G);
           } else {
             par_state(Us);
@@ -723,14 +727,14 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence_multi(const MultiCas
 //   -> k_face_correction (oppDiv_fpts Fcomm, both sides).
 
 // Register budget: for hex p=3 four waves per SIMD (<= 128 VGPRs) is what the kernel's
-// LDS allows per CU and the Rusanov face phase fits it without spilling, so the allocator
-// is held to it.  The Roe face phase needs more: three waves (<= 168 VGPRs) measured
-// fastest (four spill, 2.76 ms; three 1.52 ms; uncapped, two waves, 1.80 ms at 64^3).
-// Other shapes would spill under a cap and keep the default (quads at p=4: three waves,
-// 168 VGPRs with 15 spilled, measured 0.165 ms against 0.128 ms uncapped at 256^2).
+// LDS allows per CU, and both face phases fit it (Rusanov 110 VGPRs; Roe at the cap) once
+// each gradient line's sum is pinned before the next line is loaded (the value-tied asm
+// fences in the face phase: without them the scheduler issued all the face phase's LDS
+// loads before their FMAs, 168 VGPRs for Roe, 196 for quads at p=4).  Other shapes keep
+// the default allocation (quads at p=4: 90 VGPRs, five waves).
 template <int D, int N, int FC>
 struct OneSidedWaves {
-  static constexpr int value = (D == 3 && N == 4) ? (FC == 1 ? 3 : 4) : 1;
+  static constexpr int value = (D == 3 && N == 4) ? 4 : 1;
 };
 
 // SELF: extrapolate the partner traces this element needs from the partner's U (no trace
@@ -864,7 +868,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
 #pragma unroll
           for (int v = 0; v < V; v++) {
             dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
-            asm volatile("" ::: "memory");
+            asm volatile("" : "+v"(dUj[v]) : : "memory");  // formed here, before the next line is loaded
           }
         }, n, a, G);
         if (owner == 0) ldg_common_add<D, BM>(Uo, Up, G, G, a, Fo);
@@ -880,7 +884,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
 #pragma unroll
         for (int v = 0; v < V; v++) {
           dUj[v] = line_extrap<N>(lE, g_l + (j * V + v) * nS, base, str);
-          asm volatile("" ::: "memory");
+          asm volatile("" : "+v"(dUj[v]) : : "memory");  // formed here, before the next line is loaded
         }
       }, n, a, G);
       asm volatile("" ::: "memory");  // load the partner's state only now: fewer live registers
@@ -1123,6 +1127,7 @@ size_t lds_per_element(Op op, bool viscous) {
 struct MultiGroup {
   Op op = Op::Extrap;
   int D = 0, N = 0, visc = 0, bm = 0, fc = 0;  // bm: LDG branch (0 general, 1 beta = 1/2, 2 beta = -1/2)
+                                               // N = 0: cases of several orders (kernels_mixed.hip)
   int EB = 1, nc = 0;
   unsigned nblk = 0;
   size_t lds = 0;
@@ -1148,5 +1153,10 @@ template <int D, int N>
 int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j);
 template <int D, int N>
 void launch_multi(const MultiGroup &g, hipStream_t st);
+// Defined and explicitly instantiated per nDims in kernels_mixed.hip.
+template <int D>
+void launch_mixed(const MultiGroup &g, hipStream_t st);
+// The ops a mixed-order launch covers (the Euler divergence and the one-sided LDG pair).
+inline bool mixed_op(Op op, int visc) { return (op == Op::Div && !visc) || op == Op::OneSided || op == Op::FaceCorr; }
 
 }  // namespace zk

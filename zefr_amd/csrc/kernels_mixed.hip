@@ -1,0 +1,81 @@
+// kernels_mixed.hip -- multi-case launches over cases of different orders: one launch per
+// (kernel, nDims, flux branch) for a whole multi-case plan.  Compiled once per nDims with
+// -DZK_D=<2|3>.  Workgroup b finds its case by a binary search over the cases' first
+// workgroups, then switches on that case's order (uniform across the workgroup) into the
+// single-shape block function, with the case's own tile; threads beyond its EB * nS idle.
+// The launch takes the largest LDS any of its cases needs.
+#include "kernels.hpp"
+
+#if !defined(ZK_D)
+#error "kernels_mixed.hip is compiled with -DZK_D=<nDims>"
+#endif
+
+namespace zk {
+
+#define ZK_SWITCH_N(CALL)                \
+  switch (m.N) {                         \
+    case 1: { constexpr int N = 1; CALL; } break; \
+    case 2: { constexpr int N = 2; CALL; } break; \
+    case 3: { constexpr int N = 3; CALL; } break; \
+    case 4: { constexpr int N = 4; CALL; } break; \
+    case 5: { constexpr int N = 5; CALL; } break; \
+    case 6: { constexpr int N = 6; CALL; } break; \
+  }
+
+template <int D, bool VISC, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) void k_divergence_mixed(const MultiCase *__restrict__ cs,
+                                                                  const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  const MultiCase &m = cs[c];
+  const long b = (long)blockIdx.x - starts[c];
+  ZK_SWITCH_N((divergence_block<D, N, VISC, BM, FC, !VISC>(m.U, m.out, m.a, b)))
+}
+
+template <int D, int BM, int FC>
+__global__ __launch_bounds__(kMaxThreads) void k_visc_onesided_mixed(const MultiCase *__restrict__ cs,
+                                                                     const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  const MultiCase &m = cs[c];
+  const long b = (long)blockIdx.x - starts[c];
+  ZK_SWITCH_N((visc_onesided_block<D, N, BM, FC, true>(m.U, m.out, m.a, b)))
+}
+
+template <int D, int BM>
+__global__ __launch_bounds__(kMaxThreads) void k_face_correction_mixed(const MultiCase *__restrict__ cs,
+                                                                       const long *__restrict__ starts, int nc) {
+  const int c = find_case(starts, nc, blockIdx.x);
+  const MultiCase &m = cs[c];
+  const long b = (long)blockIdx.x - starts[c];
+  ZK_SWITCH_N((face_correction_block<D, N, BM>(m.out, m.a, b)))
+}
+#undef ZK_SWITCH_N
+
+template <int D>
+void launch_mixed(const MultiGroup &g, hipStream_t st) {
+  const dim3 grid(g.nblk), block(kMaxThreads);
+  const MultiCase *c = g.cases;
+  const long *s0 = g.starts;
+  switch (g.op) {
+    case Op::Div:
+#define ZK_DIV(V_, B_, F_) hipLaunchKernelGGL((k_divergence_mixed<D, V_, B_, F_>), grid, block, g.lds, st, c, s0, g.nc)
+      if (!g.visc) { if (g.fc) ZK_DIV(false, 0, 1); else ZK_DIV(false, 0, 0); }
+#undef ZK_DIV
+      break;
+    case Op::OneSided:
+#define ZK_OSM(B_, F_) hipLaunchKernelGGL((k_visc_onesided_mixed<D, B_, F_>), grid, block, g.lds, st, c, s0, g.nc)
+      if (g.bm == 1) { if (g.fc) ZK_OSM(1, 1); else ZK_OSM(1, 0); }
+      else { if (g.fc) ZK_OSM(2, 1); else ZK_OSM(2, 0); }
+#undef ZK_OSM
+      break;
+    case Op::FaceCorr:
+      if (g.bm == 1) hipLaunchKernelGGL((k_face_correction_mixed<D, 1>), grid, block, g.lds, st, c, s0, g.nc);
+      else hipLaunchKernelGGL((k_face_correction_mixed<D, 2>), grid, block, g.lds, st, c, s0, g.nc);
+      break;
+    default:
+      break;
+  }
+}
+
+template void launch_mixed<ZK_D>(const MultiGroup &, hipStream_t);
+
+}  // namespace zk
