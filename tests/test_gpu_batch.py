@@ -1,7 +1,7 @@
 """Batched residuals (zefr_batch_*): the residuals of independent solvers as one multi-case
-plan (zfr_multi_*: one launch per kernel, shape and flux branch over all members) captured as
-one HIP graph.  The members cover Euler, the one-sided LDG path, general beta and Roe on quads
-and hexes of several orders.  Each replay must reproduce every member's eager residual to
+plan (zfr_multi_*: one launch per kernel and flux branch over all members, whatever their
+order and element type) captured as one HIP graph.  The members cover Euler, the one-sided
+LDG path, general beta and Roe on quads and hexes of several orders.  Each replay must reproduce every member's eager residual to
 rounding (1e-15 relative: the multi-case kernels are separate compilations of the same
 arithmetic, and the hex p=3 one-sided member differs from its eager residual in one ulp of an
 entry); the graph is replayed several times with eager work and synchronisation in between."""
@@ -26,6 +26,12 @@ def _members():
              state="smooth", extra={"ldg_b": 0.0}),
         dict(name="batch_h_roe", n=(2, 2, 2), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=2, viscous=True,
              state="tgv3d", extra={"fconv_type": "Roe"}),
+        # quads and hexes in the same groups: a quad one-sided member beside the hex p=3 one,
+        # a hex Euler member beside the quad Euler ones
+        dict(name="batch_q_ns_p4", n=(6, 5), lo=(0.0, 0.0), hi=(TWO_PI, TWO_PI), order=4, viscous=True,
+             state="tgv2d"),
+        dict(name="batch_h_euler_p2", n=(3, 2, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=2, viscous=False,
+             state="noisy:9"),
     ]
     return cases
 
@@ -42,7 +48,11 @@ def test_batch_graph_matches_eager_residuals():
         refs.append(z.state("divF", 0, layout="device").copy())
         zs.append(z)
     b = Batch(zs)
-    assert 0 < b.launches < 2 * len(zs)  # grouped: fewer launches than one per member and kernel
+    # one launch per kernel and flux branch over every order and element type: the Euler
+    # divergence (quads p 1..5 and a hex), the Rusanov one-sided kernel (quad p=4, hex p=3),
+    # the Roe one-sided kernel, the face correction of all three one-sided members, and the
+    # general-beta member's extrapolate / gradient / divergence
+    assert b.launches == 7, b.launches
     for rep in range(3):
         for z in zs:  # clear every residual eagerly between replays
             d = z.dims()

@@ -624,14 +624,15 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
     if (d->nEles <= 0) continue;
     if (visc && bm != 0) {
       if (!d->Ftr) return zfr::fail("zfr_multi_create: the one-sided path needs Ftr");
-      // one launch per phase for every order (N = 0: mixed-order launch)
-      phase[1][{Op::OneSided, D, 0, 1, bm, fc}].push_back(i);  // extrapolates the partner traces itself
-      phase[2][{Op::FaceCorr, D, 0, 1, bm, 0}].push_back(i);
+      // one launch per phase for every order and element type (N = 0, D = 0: mixed launch;
+      // narrowed to one nDims below when the group holds only one)
+      phase[1][{Op::OneSided, 0, 0, 1, bm, fc}].push_back(i);  // extrapolates the partner traces itself
+      phase[2][{Op::FaceCorr, 0, 0, 1, bm, 0}].push_back(i);
     } else {
       if (visc && !d->Gtr) return zfr::fail("zfr_multi_create: the general LDG path needs Gtr");
       if (visc) phase[0][{Op::Extrap, D, N, 0, 0, 0}].push_back(i);  // Euler: extrapolated in the divergence
       if (visc) phase[1][{Op::Grad, D, N, 1, 0, 0}].push_back(i);
-      phase[2][{Op::Div, D, visc ? N : 0, visc, bm, fc}].push_back(i);
+      phase[2][{Op::Div, visc ? D : 0, visc ? N : 0, visc, bm, fc}].push_back(i);
     }
   }
   auto m = new zfr_multi_s();
@@ -642,26 +643,32 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
       zk::MultiGroup g;
       g.op = k.op; g.D = k.D; g.N = k.N; g.visc = k.visc; g.bm = k.bm; g.fc = k.fc;
       g.nc = (int)ids.size();
+      if (g.D == 0) {  // a mixed group: its own nDims' kernel when every case has one nDims
+        const int d0 = descs[ids[0]].nDims;
+        bool one = true;
+        for (int i : ids) one = one && descs[i].nDims == d0;
+        if (one) g.D = d0;
+      }
       // one tile per order, chosen for that order's total element count in the group, as
       // for one solver; the launch takes the largest LDS of its orders
-      std::map<int, int> eb_of;
+      std::map<int, int> eb_of;  // tile per shape, keyed nDims * 8 + N
       g.lds = 0;
       g.EB = 0;
       for (int i : ids) {
-        const int N = descs[i].order + 1;
-        if (eb_of.count(N)) continue;
-        const size_t per_ele = (k.D == 2) ? zk::lds_per_element_n<2>(N, k.op, k.visc != 0)
-                                          : zk::lds_per_element_n<3>(N, k.op, k.visc != 0);
-        const int nS = (k.D == 2) ? N * N : N * N * N;
+        const int N = descs[i].order + 1, cD = descs[i].nDims, shape = cD * 8 + N;
+        if (eb_of.count(shape)) continue;
+        const size_t per_ele = (cD == 2) ? zk::lds_per_element_n<2>(N, k.op, k.visc != 0)
+                                         : zk::lds_per_element_n<3>(N, k.op, k.visc != 0);
+        const int nS = (cD == 2) ? N * N : N * N * N;
         const int nops = 2 * N + N * N + 2 * N, nops_pad = (nops + 7) & ~7;
         zfr_ele_desc_t agg = descs[i];
         agg.nEles = 0;
         for (int i2 : ids)
-          if (descs[i2].order + 1 == N) agg.nEles += descs[i2].nEles;
+          if (descs[i2].order + 1 == N && descs[i2].nDims == cD) agg.nEles += descs[i2].nEles;
         agg.eles_per_block = 0;
         const int defEB = (zk::kMaxThreads / nS) > 0 ? zk::kMaxThreads / nS : 1;
         const int eb = zk::eles_per_block(&agg, defEB, nops_pad, per_ele);
-        eb_of[N] = eb;
+        eb_of[shape] = eb;
         const size_t lds = sizeof(double) * (nops_pad + eb * per_ele);
         if (lds > zk::kMaxLds) {
           delete m;
@@ -675,12 +682,13 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
       long blocks = 0;
       for (size_t q = 0; q < ids.size(); q++) {
         const int i = ids[q];
-        const int N = descs[i].order + 1, eb = eb_of[N];
+        const int N = descs[i].order + 1, eb = eb_of[descs[i].nDims * 8 + N];
         hc[q].a = zk::make_args(&descs[i], eb);
         hc[q].a.trace_sel = (k.op == Op::ExtrapPartner) ? 1 : 0;
         hc[q].U = U[i];
         hc[q].out = (k.op == Op::Div || k.op == Op::OneSided || k.op == Op::FaceCorr) ? divF[i] : nullptr;
         hc[q].N = N;
+        hc[q].D = descs[i].nDims;
         hs[q] = blocks;
         blocks += (descs[i].nEles + eb - 1) / eb;
       }
@@ -714,7 +722,8 @@ int zfr_multi_residual(zfr_multi_t *plan, void *stream) {
   const hipStream_t st = (hipStream_t)stream;
   for (const zk::MultiGroup &g : plan->groups) {
     if (g.nblk == 0) continue;
-    if (g.D == 2) zk::dispatch_multi_n<2>(g, st);
+    if (g.D == 0) zk::launch_mixed<0>(g, st);  // quads and hexes in one launch
+    else if (g.D == 2) zk::dispatch_multi_n<2>(g, st);
     else zk::dispatch_multi_n<3>(g, st);
     const int rc = zfr::check_hip(hipGetLastError(), "zfr_multi_residual");
     if (rc) return rc;

@@ -112,6 +112,7 @@ struct MultiCase {
   const double *U;
   double *out;
   int N;  // points per direction of the case's elements (the order of a mixed-order launch)
+  int D;  // nDims of the case (a launch over quads and hexes together)
 };
 
 // The case of workgroup b: binary search over the cases' first workgroups (uniform across
@@ -1127,7 +1128,8 @@ size_t lds_per_element(Op op, bool viscous) {
 struct MultiGroup {
   Op op = Op::Extrap;
   int D = 0, N = 0, visc = 0, bm = 0, fc = 0;  // bm: LDG branch (0 general, 1 beta = 1/2, 2 beta = -1/2)
-                                               // N = 0: cases of several orders (kernels_mixed.hip)
+                                               // N = 0: cases of several orders (kernels_mixed.hip);
+                                               // with D = 0 also of both nDims
   int EB = 1, nc = 0;
   unsigned nblk = 0;
   size_t lds = 0;
@@ -1153,7 +1155,7 @@ template <int D, int N>
 int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j);
 template <int D, int N>
 void launch_multi(const MultiGroup &g, hipStream_t st);
-// Defined and explicitly instantiated per nDims in kernels_mixed.hip.
+// Defined and explicitly instantiated in kernels_mixed.hip for D = 2, 3 and 0 (both).
 template <int D>
 void launch_mixed(const MultiGroup &g, hipStream_t st);
 // The ops a mixed-order launch covers (the Euler divergence and the one-sided LDG pair).

@@ -1,28 +1,37 @@
-// kernels_mixed.hip -- multi-case launches over cases of different orders: one launch per
-// (kernel, nDims, flux branch) for a whole multi-case plan.  Compiled once per nDims with
-// -DZK_D=<2|3>.  Workgroup b finds its case by a binary search over the cases' first
-// workgroups, then switches on that case's order (uniform across the workgroup) into the
-// single-shape block function, with the case's own tile; threads beyond its EB * nS idle.
-// The launch takes the largest LDS any of its cases needs.
+// kernels_mixed.hip -- multi-case launches over cases of different orders, and of different
+// element types (quads and hexes): one launch per (kernel, flux branch) for a whole
+// multi-case plan.  Compiled with -DZK_D=2 or 3 (cases of that nDims, any order) and with
+// -DZK_D=0 (quads and hexes together).  Workgroup b finds its case by a binary search over
+// the cases' first workgroups, then switches on that case's shape (uniform across the
+// workgroup) into the single-shape block function, with the case's own tile; threads beyond
+// its EB * nS idle.  The launch takes the largest LDS any of its cases needs; the register
+// budget is the largest of the shapes compiled in, which is why single-nDims groups keep
+// their own kernel.
 #include "kernels.hpp"
 
 #if !defined(ZK_D)
-#error "kernels_mixed.hip is compiled with -DZK_D=<nDims>"
+#error "kernels_mixed.hip is compiled with -DZK_D=<nDims, or 0 for quads and hexes together>"
 #endif
 
 namespace zk {
 
-#define ZK_SWITCH_N(CALL)                \
-  switch (m.N) {                         \
-    case 1: { constexpr int N = 1; CALL; } break; \
-    case 2: { constexpr int N = 2; CALL; } break; \
-    case 3: { constexpr int N = 3; CALL; } break; \
-    case 4: { constexpr int N = 4; CALL; } break; \
-    case 5: { constexpr int N = 5; CALL; } break; \
-    case 6: { constexpr int N = 6; CALL; } break; \
+#define ZK_SWITCH_N_OF(DD, CALL)                         \
+  switch (m.N) {                                         \
+    case 1: { constexpr int N = 1, D = DD; CALL; } break; \
+    case 2: { constexpr int N = 2, D = DD; CALL; } break; \
+    case 3: { constexpr int N = 3, D = DD; CALL; } break; \
+    case 4: { constexpr int N = 4, D = DD; CALL; } break; \
+    case 5: { constexpr int N = 5, D = DD; CALL; } break; \
+    case 6: { constexpr int N = 6, D = DD; CALL; } break; \
   }
+// DIMS 2 or 3: that nDims; 0: the case's own nDims (quads and hexes in one launch)
+#define ZK_SWITCH_N(CALL)                                \
+  if constexpr (DIMS == 2) { ZK_SWITCH_N_OF(2, CALL) }   \
+  else if constexpr (DIMS == 3) { ZK_SWITCH_N_OF(3, CALL) } \
+  else if (m.D == 2) { ZK_SWITCH_N_OF(2, CALL) }         \
+  else { ZK_SWITCH_N_OF(3, CALL) }
 
-template <int D, bool VISC, int BM, int FC>
+template <int DIMS, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) void k_divergence_mixed(const MultiCase *__restrict__ cs,
                                                                   const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
@@ -31,7 +40,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence_mixed(const MultiCas
   ZK_SWITCH_N((divergence_block<D, N, VISC, BM, FC, !VISC>(m.U, m.out, m.a, b)))
 }
 
-template <int D, int BM, int FC>
+template <int DIMS, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided_mixed(const MultiCase *__restrict__ cs,
                                                                      const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
@@ -40,7 +49,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_visc_onesided_mixed(const Multi
   ZK_SWITCH_N((visc_onesided_block<D, N, BM, FC, true>(m.U, m.out, m.a, b)))
 }
 
-template <int D, int BM>
+template <int DIMS, int BM>
 __global__ __launch_bounds__(kMaxThreads) void k_face_correction_mixed(const MultiCase *__restrict__ cs,
                                                                        const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
@@ -49,6 +58,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction_mixed(const Mul
   ZK_SWITCH_N((face_correction_block<D, N, BM>(m.out, m.a, b)))
 }
 #undef ZK_SWITCH_N
+#undef ZK_SWITCH_N_OF
 
 template <int D>
 void launch_mixed(const MultiGroup &g, hipStream_t st) {
