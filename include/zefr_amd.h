@@ -285,7 +285,9 @@ int zfr_gradient(const zfr_ele_desc_t *d, const double *U, double *dU_out, void 
  * divF[E][V][nS] from U and the traces.  Euler: from U alone (the partner traces are
  * extrapolated from the partner's U inside, replacing extrapolate_U; d->Utr is not read);
  * viscous: reads the solution trace Utr and the gradient trace Gtr (zfr_extrapolate,
- * zfr_gradient).  zfr_divergence_rk reads Utr in both cases (its update overwrites U). */
+ * zfr_gradient).  zfr_divergence_rk reads Utr in both cases when an RK epilogue writes
+ * the U it reads (rk->U, U_ini_out or corr_out equal to U); otherwise Euler extrapolates
+ * from U as here. */
 int zfr_divergence(const zfr_ele_desc_t *d, const double *U, double *divF, void *stream);
 /* Navier-Stokes with LDG ldg_b = +-1/2, where one side's gradient enters each common
  * flux: corrected gradient + compute_F + compute_divF_spts (volume part of divF) and the
@@ -339,8 +341,12 @@ int zfr_residual_norm_static(const zfr_ele_desc_t *d, const double *divF, int re
  * there ([E][V][nF]), as the next stage's extrapolation would (zfr_extrapolate_partner for
  * zfr_face_correction_rk, read by zfr_gradient_flux_traced; zfr_extrapolate for
  * zfr_divergence_rk).  zfr_face_correction_rk reads no trace and may write d->Utr itself;
- * zfr_divergence_rk reads the partners' d->Utr while it writes, so its trace_out must be
- * another buffer (the next stage then runs with Utr = trace_out). */
+ * a zfr_divergence_rk that reads the partners' d->Utr while it writes needs its trace_out
+ * in another buffer (the next stage then runs with Utr = trace_out).
+ * rk->U need not be the U the residual is formed from: the update writes U from U_ini,
+ * so a step may form stage 0's residual from its starting state, write the stage states
+ * to a second buffer, and have the final combination write the starting state's buffer
+ * (returning the starting state into the second one with U_ini_out). */
 typedef struct zfr_rk_args {
   double *U;
   const double *U_ini;
@@ -348,8 +354,21 @@ typedef struct zfr_rk_args {
   int mode, stage, nStages;
   double alpha, dt;
   double beta[4];
-  const double *source;  /* p-multigrid source term added to each divF_s, or NULL */
+  const double *source;  /* p-multigrid source term added to each divF_s (modes 1, 2, 4), or NULL */
   double *trace_out;     /* mode 1: also write the new state's trace here, or NULL  */
+  /* modes 1, 2: also store a copy of U_ini (the state the update starts from, read before
+   * U is written) here, or NULL.  U_ini and U may then be different buffers at every stage:
+   * a step may keep its starting state in U and the stage states elsewhere. */
+  double *U_ini_out;
+  /* mode 2: also write corr_out = (new U) - corr_base, or NULL (p-multigrid's coarse
+   * correction, src/multigrid.cpp:236-256) */
+  double *corr_out;
+  const double *corr_base;
+  /* Residual epilogues without an update, for p-multigrid (PMGrid::compute_source_term
+   * and the restricted residual, src/multigrid.cpp:105-261, 474-518): mode 3 writes
+   * source_out -= divF (source_out holding the restricted residual); mode 4 writes
+   * divF += source.  U and U_ini are not read or written in these modes. */
+  double *source_out;
 } zfr_rk_args_t;
 int zfr_face_correction_rk(const zfr_ele_desc_t *d, double *divF, const zfr_rk_args_t *rk, void *stream);
 int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *rk,

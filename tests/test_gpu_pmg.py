@@ -108,16 +108,32 @@ def test_fmg_cycle_vs_oracle():
     _check_cycle(z, o, p, case, 1)
 
 
-def test_fused_and_separate_source_updates_agree():
-    case = _pmg_case("pmg_fuse", (4, 4), (0.0, 0.0), (TWO_PI, TWO_PI), 3, True, "smooth", [3, 1, 0], [1, 2, 2])
+FUSE_CASES = [
+    _pmg_case("pmg_fuse", (4, 4), (0.0, 0.0), (TWO_PI, TWO_PI), 3, True, "smooth", [3, 1, 0], [1, 2, 2]),
+    _pmg_case("pmg_fuse_euler", (5, 4), (0.0, 0.0), (1.0, 1.0), 4, False, "noisy:5", [4, 2, 1, 0], [1, 2, 1, 3],
+              dt=2e-4),
+    _pmg_case("pmg_fuse_beta", (4, 4), (0.0, 0.0), (TWO_PI, TWO_PI), 3, True, "smooth", [3, 2, 0], [2, 1, 2],
+              rel_fac=0.9),
+]
+FUSE_CASES[2]["extra"]["ldg_b"] = 0.3  # the general-beta LDG path
+
+
+@pytest.mark.parametrize("case", FUSE_CASES, ids=[c["name"] for c in FUSE_CASES])
+def test_fused_and_separate_source_updates_agree(case):
+    """Fused: the stored solution, the correction, the source term and the residual plus
+    source term are written by the smoothing steps' and residuals' own kernels.  Separate:
+    copies, axpby launches and the plain update kernels.  Every level bit-identical."""
+    print("[pmg]", case["name"], flush=True)
     out = {}
     for fuse in (True, False):
         z, o, U0 = paired(case)
         z.set_step_mode(fuse, False)
         for _ in range(2):
             z.do_step()
-        out[fuse] = z.state("U")
-    assert np.array_equal(out[True], out[False])
+        out[fuse] = [z.state("U")] + [z.pmg_state(lv, w) for lv in range(1, len(case["levels"]))
+                                      for w in ("U", "source", "solution", "correction")]
+    for a, b in zip(out[True], out[False]):
+        assert np.array_equal(a, b)
 
 
 def test_transfer_kernel_all_orders():

@@ -193,10 +193,22 @@ class FRSolver {
   // FRSolver::update(sourceBT) (src/solver.cpp:2599-2646): one step with a p-multigrid
   // source term [E][V][nS] added to every stage residual (nullptr: plain update)
   void update(const double *source);
+  // The same, also storing the starting state in save and, after the step, writing
+  // corr = U - corr_base (each nullptr for none).  With fuse_rk and several stages the
+  // copy and the difference are written by the step's own kernels; otherwise by a copy
+  // and an axpby around the step.
+  void update(const double *source, double *save, double *corr, const double *corr_base);
+  // The stage-0 residual into divF(0) with a p-multigrid epilogue: mode 3 subtracts it
+  // from source_out (source_out -= divF), mode 4 adds the source term to it
+  // (divF += source).  With fuse_rk the last residual kernel does it; otherwise an axpby.
+  void residual_source(int mode, double *source_out, const double *source);
   bool own_stream = true;  // false for p-multigrid levels, which run on the fine stream
   // The trace buffer the residual phases read, when not Utr_d (fused RK steps on the
   // divergence path alternate two buffers; see step_RK_eager)
   double *trace_in = nullptr;
+  // The state the residual phases read, when not U_spts_d (stages 1.. of a fused step,
+  // whose stage states live in U_ini_d; see step_RK_eager)
+  const double *state_in = nullptr;
   // PyFR-format solution files (src/solver.cpp:4275-4837): write returns the file name
   std::string write_solution_pyfr(const std::string &prefix);
   void restart_pyfr(const std::string &restart_case, unsigned restart_iter);
@@ -205,7 +217,8 @@ class FRSolver {
  private:
   void rk_stage_update(int stage);
   void rk_final_update();
-  void finish_with_rk(int stage, double *trace_out = nullptr);
+  void finish_with_rk(int stage, double *trace_out, bool split);
+  const double *cur_state() const;
   void step_RK_eager();
   void drop_graph();
   hipGraph_t graph = nullptr;
@@ -214,7 +227,11 @@ class FRSolver {
   std::array<int, 4> graph_tile = {-1, -1, -1, -1};
   bool graph_fused = false;
   const double *graph_source = nullptr;
+  std::array<const void *, 3> graph_extra = {nullptr, nullptr, nullptr};
   const double *rk_source = nullptr;  // source term of the step in progress
+  double *rk_save = nullptr;          // store the step's starting state here
+  double *rk_corr = nullptr;          // after the step: rk_corr = U - rk_corr_base
+  const double *rk_corr_base = nullptr;
 };
 
 // p-multigrid (include/multigrid.hpp:17-44, src/multigrid.cpp:18-518) on quads/hexes:
@@ -226,7 +243,7 @@ class PMGrid {
   void setup(zefr_input_t *input, FRSolver &solver);
   void cycle(FRSolver &solver);
   void v_cycle(FRSolver &solver, int level);
-  void restrict_pmg(FRSolver &grid_f, FRSolver &grid_c);
+  void restrict_pmg(FRSolver &grid_f, FRSolver &grid_c, double *res_out);
   void prolong_err(FRSolver &grid_c, const double *correction, FRSolver &grid_f);
   void prolong_U(FRSolver &grid_c, FRSolver &grid_f);
   void compute_source_term(FRSolver &grid, double *source);

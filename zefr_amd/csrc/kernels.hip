@@ -333,16 +333,19 @@ int zfr_face_correction(const zfr_ele_desc_t *d, double *divF, void *stream) {
 namespace {
 int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *r,
              hipStream_t st) {
-  if (!r || !r->U || !r->U_ini || (r->mode == 2 && r->nStages > 1 && !r->divF0))
+  if (!r) return zfr::fail("RK fusion: missing buffer");
+  if (r->mode < 1 || r->mode > 4)
+    return zfr::fail("RK fusion: mode must be 1 (stage), 2 (final), 3 (source -= divF) or 4 (divF += source)");
+  const bool update = r->mode == 1 || r->mode == 2;
+  if (update && (!r->U || !r->U_ini || (r->mode == 2 && r->nStages > 1 && !r->divF0)))
     return zfr::fail("RK fusion: missing buffer");
-  int rc = need_geometry(d);
-  if (!rc && op == zk::Op::Div) rc = need(d->Utr, "Utr");  // the fused divergence reads the trace
-  if (!rc && op == zk::Op::Div && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5) rc = need(d->Gtr, "Gtr");
-  if (!rc && op == zk::Op::FaceCorr) rc = need(d->Ftr, "Ftr");
-  if (rc) return rc;
-  if (r->mode != 1 && r->mode != 2) return zfr::fail("RK fusion: mode must be 1 (stage) or 2 (final)");
-  if (r->nStages < 1 || r->nStages > 4 || r->stage < 0 || r->stage >= r->nStages)
-    return zfr::fail("RK fusion: stage out of range");
+  if (r->mode == 3 && !r->source_out) return zfr::fail("RK fusion: mode 3 needs source_out");
+  if (r->mode == 4 && !r->source) return zfr::fail("RK fusion: mode 4 needs source");
+  if (r->U_ini_out && !update) return zfr::fail("RK fusion: U_ini_out is for RK updates (modes 1, 2)");
+  if (r->U_ini_out && r->U_ini_out == r->U_ini) return zfr::fail("RK fusion: U_ini_out must not be U_ini");
+  if (r->corr_out && (r->mode != 2 || !r->corr_base))
+    return zfr::fail("RK fusion: corr_out is for final combinations (mode 2) and needs corr_base");
+  if (r->trace_out && r->mode != 1) return zfr::fail("RK fusion: trace_out is for stage updates (mode 1)");
   zk::KArgs::Rk rk{};
   rk.U = r->U;
   rk.Uini = r->U_ini;
@@ -355,10 +358,23 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
   rk.nStages = r->nStages;
   rk.stage = r->stage;
   rk.mode = r->mode;
-  if (r->trace_out && r->mode != 1) return zfr::fail("RK fusion: trace_out is for stage updates (mode 1)");
-  if (r->trace_out && op == zk::Op::Div && r->trace_out == d->Utr)
-    return zfr::fail("RK fusion: trace_out must not be the trace the divergence kernel reads (d->Utr)");
   rk.trace = r->trace_out;
+  rk.Uini_out = r->U_ini_out;
+  rk.corr = r->corr_out;
+  rk.corr_base = r->corr_base;
+  rk.srcw = r->source_out;
+  // the divergence reads the trace when viscous, or when its epilogue writes the state it
+  // reads (Euler then reads the partners' traces instead of their U)
+  const bool reads_tr = op == zk::Op::Div && (d->viscous || zk::rk_writes_input(rk, U));
+  int rc = need_geometry(d);
+  if (!rc && reads_tr) rc = need(d->Utr, "Utr");
+  if (!rc && op == zk::Op::Div && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5) rc = need(d->Gtr, "Gtr");
+  if (!rc && op == zk::Op::FaceCorr) rc = need(d->Ftr, "Ftr");
+  if (rc) return rc;
+  if (update && (r->nStages < 1 || r->nStages > 4 || r->stage < 0 || r->stage >= r->nStages))
+    return zfr::fail("RK fusion: stage out of range");
+  if (r->trace_out && reads_tr && r->trace_out == d->Utr)
+    return zfr::fail("RK fusion: trace_out must not be the trace the divergence kernel reads (d->Utr)");
   zk::AdjPtrs j;
   j.rk = &rk;
   return zk::dispatch(op, d, U, divF, st, j);

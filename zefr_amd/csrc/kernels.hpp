@@ -92,15 +92,21 @@ struct KArgs {
   // k_extrapolate: 0 = the trace at every flux point; 1 = only where the partner owns the
   // face (the one-sided LDG path reads nothing else)
   int trace_sel;
-  // RK update fused into the last kernel of a residual (mode 0: none, 1: stage, 2: final)
+  // Epilogue of the last kernel of a residual: mode 0 none; 1 RK stage update; 2 final RK
+  // combination; 3 srcw -= divF (p-multigrid source term: srcw holds the restricted
+  // residual); 4 divF += src (a residual plus the source term)
   struct Rk {
     double *U;
     const double *Uini, *divF0;  // divF0: stage 0 of the [nStages][E][V][nS] residuals
-    const double *src;           // p-multigrid source term added to every divF_s, or null
+    const double *src;           // p-multigrid source term added to every divF_s (modes 1, 2, 4), or null
     long total;
     double alpha, dt, beta[4];
     int nStages, stage, mode;
     double *trace;  // mode 1: also write the new state's trace here (the next stage's Utr), or null
+    double *Uini_out;         // modes 1, 2: a copy of Uini (read before U is written), or null
+    double *corr;             // mode 2: corr = new U - corr_base, or null
+    const double *corr_base;
+    double *srcw;             // mode 3
   } rk;
 };
 
@@ -233,28 +239,46 @@ __device__ __forceinline__ void partner_trace(const double *__restrict__ U, cons
 // :2997-3062, with the same arithmetic as k_rk_stage / k_rk_final).  acc holds this
 // stage's divF at the spt; the other stages' come from memory.  With a source term
 // (src/solver.cpp:2946-2962, 3061-3073) every divF_s is replaced by divF_s + src.
+// Side outputs: a copy of the starting state (Uini_out), and after a final combination
+// the change against a stored state (corr: p-multigrid's coarse correction,
+// src/multigrid.cpp:236-256).  Mode 3 subtracts the residual from a p-multigrid source
+// term in place (src/multigrid.cpp:474-518).
+// Whether a fused epilogue writes the array the residual reads its state from.
+inline bool rk_writes_input(const KArgs::Rk &r, const double *U) {
+  if (r.mode != 1 && r.mode != 2) return false;
+  return r.U == U || r.Uini_out == U || r.corr == U;
+}
+
 template <int D>
 __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc,
                                             double *unew = nullptr) {
   constexpr int V = D + 2, nM = D * D + 1;
-  if (a.rk.mode == 0) return;
+  if (a.rk.mode == 0 || a.rk.mode == 4) return;
+  if (a.rk.mode == 3) {
+#pragma unroll
+    for (int v = 0; v < V; v++) a.rk.srcw[(e * V + v) * nS + t] -= acc[v];
+    return;
+  }
   const double detJ = a.metric[(e * nM + D * D) * nS + t];
 #pragma unroll
   for (int v = 0; v < V; v++) {
     const long i = (e * V + v) * nS + t;
+    const double u0 = a.rk.Uini[i];
+    if (a.rk.Uini_out) a.rk.Uini_out[i] = u0;
     if (a.rk.mode == 1) {
       const double r = a.rk.src ? acc[v] + a.rk.src[i] : acc[v];
-      const double u = a.rk.Uini[i] - a.rk.alpha * a.rk.dt / detJ * r;
+      const double u = u0 - a.rk.alpha * a.rk.dt / detJ * r;
       a.rk.U[i] = u;
       if (unew) unew[v] = u;
     } else {
-      double u = a.rk.Uini[i];
+      double u = u0;
       for (int s = 0; s < a.rk.nStages; s++) {
         double r = (s == a.rk.stage) ? acc[v] : a.rk.divF0[s * a.rk.total + i];
         if (a.rk.src) r += a.rk.src[i];
         u -= a.rk.beta[s] * a.rk.dt / detJ * r;
       }
       a.rk.U[i] = u;
+      if (a.rk.corr) a.rk.corr[i] = u - a.rk.corr_base[i];
     }
   }
 }
@@ -695,6 +719,10 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
         acc[v] += Df[1] * c[v * nF + f1];
       }
     }
+    if (a.rk.mode == 4) {
+#pragma unroll
+      for (int v = 0; v < V; v++) acc[v] += a.rk.src[(e * V + v) * nS + t];
+    }
 #pragma unroll
     for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
     rk_epilogue<D>(a, e, t, nS, acc, unew);
@@ -1028,6 +1056,10 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
         acc[v] += (-Df[0]) * c[v * nF + f0];
         acc[v] += Df[1] * c[v * nF + f1];
       }
+    }
+    if (a.rk.mode == 4) {
+#pragma unroll
+      for (int v = 0; v < V; v++) acc[v] += a.rk.src[(e * V + v) * nS + t];
     }
 #pragma unroll
     for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];

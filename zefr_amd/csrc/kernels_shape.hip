@@ -10,13 +10,13 @@
 namespace zk {
 
 // k_divergence for one convective flux (FC: 0 Rusanov, 1 Roe) and the LDG beta branch.
-// Euler: the standalone residual extrapolates the partner traces itself (SELF); with a
-// fused RK update it must read them from Utr, since its epilogue overwrites U in place
-// while other workgroups may still read it.
+// Euler: the residual extrapolates the partner traces from U itself (SELF), unless a
+// fused epilogue writes U in place (the RK update, or the copy of the starting state):
+// other workgroups may then still be reading it, so the traces come from Utr.
 template <int D, int N, int FC>
 void launch_div(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds, hipStream_t st, const double *U,
                 double *out, const KArgs &a) {
-  if (!d->viscous && a.rk.mode != 0)
+  if (!d->viscous && rk_writes_input(a.rk, U))
     hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, false>), grid, block, lds, st, U, out, a);
   else if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, true>), grid, block, lds, st, U, out, a);
   else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 1, FC>), grid, block, lds, st, U, out, a);

@@ -138,11 +138,13 @@ static int level_of(const std::vector<std::shared_ptr<FRSolver>> &grids, const F
   return 0;
 }
 
-// PMGrid::restrict_pmg (src/multigrid.cpp:335-392): solution and stage-0 residual.
-void PMGrid::restrict_pmg(FRSolver &grid_f, FRSolver &grid_c) {
+// PMGrid::restrict_pmg (src/multigrid.cpp:335-392): solution and stage-0 residual.  The
+// restricted residual goes to res_out (the coarse divF in the reference; the cycle passes
+// the coarse level's source buffer, where compute_source_term consumes it).
+void PMGrid::restrict_pmg(FRSolver &grid_f, FRSolver &grid_c, double *res_out) {
   const DevBuf &R = oppRes_d[level_of(grids, grid_f)];
   transfer(grid_f, grid_c, R, 1.0, grid_f.eles->U_spts_d.as<double>(), 0.0, grid_c.eles->U_spts_d.as<double>());
-  transfer(grid_f, grid_c, R, 1.0, grid_f.eles->divF_d.as<double>(), 0.0, grid_c.eles->divF_d.as<double>());
+  transfer(grid_f, grid_c, R, 1.0, grid_f.eles->divF_d.as<double>(), 0.0, res_out);
 }
 
 // PMGrid::prolong_err (src/multigrid.cpp:394-433): U_f += rel_fac * P correction_c.
@@ -158,48 +160,53 @@ void PMGrid::prolong_U(FRSolver &grid_c, FRSolver &grid_f) {
 }
 
 // PMGrid::compute_source_term (src/multigrid.cpp:474-518): source = restricted residual
-// minus the coarse residual at the restricted state.
-void PMGrid::compute_source_term(FRSolver &grid, double *source) {
-  Elements &e = *grid.eles;
-  const size_t n = e.state_size();
-  hip_or_throw(hipMemcpyAsync(source, e.divF_d.p, n * sizeof(double), hipMemcpyDeviceToDevice, grid.stream),
-               "hipMemcpyAsync");
-  grid.compute_residual(0);
-  status_or_throw(zfr_axpby((long)n, 1.0, source, -1.0, e.divF_d.as<double>(), source, grid.stream));
-}
+// minus the coarse residual at the restricted state.  source holds the restricted
+// residual on entry (restrict_pmg writes it there); the residual's last kernel subtracts.
+void PMGrid::compute_source_term(FRSolver &grid, double *source) { grid.residual_source(3, source, nullptr); }
 
 // PMGrid::v_cycle (src/multigrid.cpp:105-261) from `level` (0 = the fine solver) down.
+// Each coarse level's stored solution is written by its first smoothing step (the state
+// it starts from), its correction (U - stored solution, the CPU path's form; the GPU
+// path of the reference leaves the difference in U as well, which nothing reads
+// afterwards) by its last one, and the residual restricted to the next level carries the
+// source term added by the residual's last kernel.
 void PMGrid::v_cycle(FRSolver &solver, int lvl) {
   if (lvl != nLevels - 1) {
     solver.compute_residual(0);
-    restrict_pmg(solver, *grids[lvl + 1]);
+    restrict_pmg(solver, *grids[lvl + 1], sourcesBT[lvl + 1].as<double>());
   }
   for (int n = lvl + 1; n < nLevels; n++) {
     FRSolver &g = *grids[n];
     Elements &e = *g.eles;
     const size_t bytes = e.state_size() * sizeof(double);
-    double *src = sourcesBT[n].as<double>();
+    double *src = sourcesBT[n].as<double>(), *sol = solutionsBT[n].as<double>(),
+           *corr = correctionsBT[n].as<double>();
     compute_source_term(g, src);
-    hip_or_throw(hipMemcpyAsync(solutionsBT[n].p, e.U_spts_d.p, bytes, hipMemcpyDeviceToDevice, g.stream),
-                 "hipMemcpyAsync");
-    for (unsigned s = 0; s < input->mg_steps[n]; s++) g.update(src);
-    if (n + 1 < nLevels) {
-      g.compute_residual(0);
-      status_or_throw(zfr_axpby((long)e.state_size(), 1.0, e.divF_d.as<double>(), 1.0, src, e.divF_d.as<double>(),
-                                g.stream));
-      restrict_pmg(g, *grids[n + 1]);
+    const unsigned steps = input->mg_steps[n];
+    if (steps == 0)
+      hip_or_throw(hipMemcpyAsync(sol, e.U_spts_d.p, bytes, hipMemcpyDeviceToDevice, g.stream), "hipMemcpyAsync");
+    const bool coarsest = n == nLevels - 1;
+    for (unsigned s = 0; s < steps; s++) {
+      // the coarsest level is not smoothed again on the way up: its last step here ends it
+      const bool last = coarsest && s + 1 == steps;
+      g.update(src, s == 0 ? sol : nullptr, last ? corr : nullptr, last ? sol : nullptr);
+    }
+    if (!coarsest) {
+      g.residual_source(4, nullptr, src);
+      restrict_pmg(g, *grids[n + 1], sourcesBT[n + 1].as<double>());
     }
   }
   for (int n = nLevels - 1; n > lvl; n--) {
     FRSolver &g = *grids[n];
     Elements &e = *g.eles;
+    double *sol = solutionsBT[n].as<double>(), *corr = correctionsBT[n].as<double>();
+    const unsigned steps = input->mg_steps[n];
     if (n != nLevels - 1)
-      for (unsigned s = 0; s < input->mg_steps[n]; s++) g.update(sourcesBT[n].as<double>());
-    // correction = U - stored solution (the CPU path's form; the GPU path of the
-    // reference leaves the difference in U as well, which nothing reads afterwards)
-    status_or_throw(zfr_axpby((long)e.state_size(), 1.0, e.U_spts_d.as<double>(), -1.0,
-                              solutionsBT[n].as<double>(), correctionsBT[n].as<double>(), g.stream));
-    if (n > lvl + 1) prolong_err(g, correctionsBT[n].as<double>(), *grids[n - 1]);
+      for (unsigned s = 0; s < steps; s++)
+        g.update(sourcesBT[n].as<double>(), nullptr, s + 1 == steps ? corr : nullptr, s + 1 == steps ? sol : nullptr);
+    if (steps == 0)
+      status_or_throw(zfr_axpby((long)e.state_size(), 1.0, e.U_spts_d.as<double>(), -1.0, sol, corr, g.stream));
+    if (n > lvl + 1) prolong_err(g, corr, *grids[n - 1]);
   }
   if (lvl != nLevels - 1) prolong_err(*grids[lvl + 1], correctionsBT[lvl + 1].as<double>(), solver);
 }

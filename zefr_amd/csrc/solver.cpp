@@ -371,12 +371,14 @@ bool FRSolver::onesided() const {
 //            general LDG beta: gradient and its traces;
 //   finish = one-sided LDG: face correction of divF;
 //            otherwise: common flux + full divergence.
+const double *FRSolver::cur_state() const { return state_in ? state_in : eles->U_spts_d.as<double>(); }
+
 void FRSolver::compute_residual_start(unsigned, int color) {
   need_device(*this);
   if (color > -1) raise("multi-colour (MCGS) residuals are not supported");
   const auto d = eles->desc(*input, 0);
   // the one-sided and the Euler kernels extrapolate the partner traces they need themselves
-  if (input->viscous && !onesided()) status_or_throw(zfr_extrapolate(&d, eles->U_spts_d.as<double>(), stream));
+  if (input->viscous && !onesided()) status_or_throw(zfr_extrapolate(&d, cur_state(), stream));
 }
 
 void FRSolver::compute_residual_mid(unsigned stage, int) {
@@ -387,10 +389,10 @@ void FRSolver::compute_residual_mid(unsigned stage, int) {
   if (onesided()) {
     double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
     // with trace_in (RK stages 1..), the previous stage's update has written the traces
-    if (trace_in) status_or_throw(zfr_gradient_flux_traced(&d, eles->U_spts_d.as<double>(), divF, stream));
-    else status_or_throw(zfr_gradient_flux(&d, eles->U_spts_d.as<double>(), divF, stream));
+    if (trace_in) status_or_throw(zfr_gradient_flux_traced(&d, cur_state(), divF, stream));
+    else status_or_throw(zfr_gradient_flux(&d, cur_state(), divF, stream));
   } else {
-    status_or_throw(zfr_gradient(&d, eles->U_spts_d.as<double>(), nullptr, stream));
+    status_or_throw(zfr_gradient(&d, cur_state(), nullptr, stream));
   }
 }
 
@@ -400,7 +402,7 @@ void FRSolver::compute_residual_finish(unsigned stage, int) {
   if (trace_in) d.Utr = trace_in;
   double *divF = eles->divF_d.as<double>() + (size_t)stage * eles->state_size();
   if (onesided()) status_or_throw(zfr_face_correction(&d, divF, stream));
-  else status_or_throw(zfr_divergence(&d, eles->U_spts_d.as<double>(), divF, stream));
+  else status_or_throw(zfr_divergence(&d, cur_state(), divF, stream));
 }
 
 void FRSolver::compute_residual(unsigned stage, int color) {
@@ -469,19 +471,23 @@ void FRSolver::rk_final_update() {
 
 // compute_residual_finish with the RK update of this stage done by the same kernel; with
 // trace_out, that kernel also writes the new state's trace there (stage updates only).
-void FRSolver::finish_with_rk(int stage, double *trace_out) {
+// split: the step keeps its starting state in U_spts_d until the final combination and
+// its stage states in U_ini_d (see step_RK_eager).
+void FRSolver::finish_with_rk(int stage, double *trace_out, bool split) {
   auto d = eles->desc(*input, 2);
   if (trace_in) d.Utr = trace_in;
   const int ns = (int)input->nStages;
   const size_t st = eles->state_size();
+  double *const A = eles->U_spts_d.as<double>(), *const I = eles->U_ini_d.as<double>();
   zfr_rk_args_t r{};
-  r.U = eles->U_spts_d.as<double>();
-  r.U_ini = (ns > 1) ? eles->U_ini_d.as<double>() : eles->U_spts_d.as<double>();
+  r.U = (split && stage < ns - 1) ? I : A;
+  r.U_ini = (split || ns == 1) ? A : I;
   r.divF0 = eles->divF_d.as<double>();
   r.stage = stage;
   r.nStages = ns;
   r.dt = input->dt;
   r.source = rk_source;
+  if (split && stage == 0) r.U_ini_out = rk_save;
   if (stage < ns - 1) {
     r.mode = 1;
     r.alpha = kRkAlpha[stage];
@@ -489,50 +495,71 @@ void FRSolver::finish_with_rk(int stage, double *trace_out) {
   } else {
     r.mode = 2;
     for (int s = 0; s < ns; s++) r.beta[s] = (ns > 1) ? kRkBeta[s] : 1.0;
+    if (split) r.U_ini_out = I;  // the starting state, where the unfused step leaves it
+    r.corr_out = rk_corr;
+    r.corr_base = rk_corr_base;
   }
   double *divF = eles->divF_d.as<double>() + (size_t)stage * st;
   if (onesided()) status_or_throw(zfr_face_correction_rk(&d, divF, &r, stream));
-  else status_or_throw(zfr_divergence_rk(&d, eles->U_spts_d.as<double>(), divF, &r, stream));
+  else status_or_throw(zfr_divergence_rk(&d, cur_state(), divF, &r, stream));
 }
 
-// With fuse_rk, each stage update also writes the next stage's trace, so only stage 0
-// forms it from U (the extrapolation kernel, or on the one-sided path the gradient_flux
-// kernel itself).  On the one-sided path the writing kernel (face_correction) reads no
-// trace and writes Utr_d in place, and stages 1.. read it (gradient_flux_traced).  On
-// the Euler and general-LDG paths the writing kernel (divergence) reads the partners'
-// traces, so the stages alternate Utr_d and Utr2_d; there the fused divergence reads the
-// trace even for Euler, whose standalone residual extrapolates it itself, because its
-// update overwrites U while other workgroups may still read it.
+// One RK step.  Unfused: U_ini = U, then per stage a residual and an update kernel.
+// Fused with several stages (split): the starting state stays in U_spts_d, stage 0's
+// residual is formed from it and its update writes the stage state into U_ini_d; stages
+// 1.. update U_ini_d in place; the final combination writes U_spts_d and stores the
+// starting state into U_ini_d (so U_ini_d ends as the unfused step leaves it, and no copy
+// precedes the step).  Each stage update also writes the next stage's trace, so only
+// stage 0 forms it from the state (the extrapolation kernel, or on the one-sided path the
+// gradient_flux kernel itself).  On the one-sided path the writing kernel
+// (face_correction) reads no trace and writes Utr_d in place, and stages 1.. read it
+// (gradient_flux_traced).  On the Euler and general-LDG paths the writing kernel
+// (divergence) reads the partners' traces, so the stages alternate Utr_d and Utr2_d.  An
+// Euler divergence whose epilogue writes the state it reads (stages 1.., and the final
+// combination, which writes U_ini_d) reads the partners' traces instead of their state;
+// stage 0 of a split step writes only U_ini_d and extrapolates them itself.
 void FRSolver::step_RK_eager() {
   const int ns = (int)input->nStages;
-  if (ns > 1)
+  const bool split = fuse_rk && ns > 1;
+  if (ns > 1 && !split)
     hip_or_throw(hipMemcpyAsync(eles->U_ini_d.p, eles->U_spts_d.p, eles->U_spts_d.bytes, hipMemcpyDeviceToDevice,
                                 stream), "hipMemcpyAsync");
+  if (rk_save && !split)
+    hip_or_throw(hipMemcpyAsync(rk_save, eles->U_spts_d.p, eles->U_spts_d.bytes, hipMemcpyDeviceToDevice, stream),
+                 "hipMemcpyAsync");
   double *const tr[2] = {eles->Utr_d.as<double>(),
                          onesided() ? eles->Utr_d.as<double>() : eles->Utr2_d.as<double>()};
   const bool chain = fuse_rk && ns > 1 && tr[1] != nullptr;
   for (int stage = 0; stage < ns; stage++) {
     if (fuse_rk) {
       trace_in = (chain && stage > 0) ? tr[stage % 2] : nullptr;
+      state_in = (split && stage > 0) ? eles->U_ini_d.as<double>() : nullptr;
       try {
-        // the fused divergence (Euler, general LDG) reads a trace: form it at stage 0
-        if ((!chain || stage == 0) && !onesided()) {
+        // the fused divergence reads a trace (viscous, or Euler writing the state it
+        // reads): form it at stage 0, or at every stage without the chain
+        const bool reads_tr = input->viscous || !split || stage > 0;
+        if ((!chain || stage == 0) && !onesided() && reads_tr) {
           const auto d0 = eles->desc(*input, 0);
-          status_or_throw(zfr_extrapolate(&d0, eles->U_spts_d.as<double>(), stream));
+          status_or_throw(zfr_extrapolate(&d0, cur_state(), stream));
         }
         compute_residual_mid(stage);
-        finish_with_rk(stage, (chain && stage < ns - 1) ? tr[(stage + 1) % 2] : nullptr);
+        finish_with_rk(stage, (chain && stage < ns - 1) ? tr[(stage + 1) % 2] : nullptr, split);
       } catch (...) {
         trace_in = nullptr;
+        state_in = nullptr;
         throw;
       }
       trace_in = nullptr;
+      state_in = nullptr;
     } else {
       compute_residual(stage);
       if (stage < ns - 1) rk_stage_update(stage);
       else rk_final_update();
     }
   }
+  if (rk_corr && !fuse_rk)
+    status_or_throw(zfr_axpby((long)eles->state_size(), 1.0, eles->U_spts_d.as<double>(), -1.0, rk_corr_base,
+                              rk_corr, stream));
 }
 
 void FRSolver::drop_graph() {
@@ -557,8 +584,9 @@ void FRSolver::step_RK() {
     step_RK_eager();
     return;
   }
+  const std::array<const void *, 3> extra = {rk_save, rk_corr, rk_corr_base};
   if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile_key() || graph_fused != fuse_rk ||
-      graph_source != rk_source) {
+      graph_source != rk_source || graph_extra != extra) {
     drop_graph();
     hip_or_throw(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
@@ -575,6 +603,7 @@ void FRSolver::step_RK() {
     graph_tile = eles->tile_key();
     graph_fused = fuse_rk;
     graph_source = rk_source;
+    graph_extra = extra;
   }
   hip_or_throw(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
 }
@@ -586,20 +615,53 @@ void FRSolver::restart_solution() {
   else raise("restart_type must be 2 (PyFR .pyfrs files)");
 }
 
-void FRSolver::update() { update(nullptr); }
+void FRSolver::update() { update(nullptr, nullptr, nullptr, nullptr); }
 
-void FRSolver::update(const double *source) {
+void FRSolver::update(const double *source) { update(source, nullptr, nullptr, nullptr); }
+
+void FRSolver::update(const double *source, double *save, double *corr, const double *corr_base) {
+  if (corr && !corr_base) raise("update: corr needs corr_base");
   prev_time = flow_time;
   rk_source = source;
+  rk_save = save;
+  rk_corr = corr;
+  rk_corr_base = corr_base;
+  auto clear = [&] {
+    rk_source = nullptr;
+    rk_save = nullptr;
+    rk_corr = nullptr;
+    rk_corr_base = nullptr;
+  };
   try {
     step_RK();
   } catch (...) {
-    rk_source = nullptr;
+    clear();
     throw;
   }
-  rk_source = nullptr;
+  clear();
   flow_time = prev_time + input->dt;
   current_iter++;
+}
+
+void FRSolver::residual_source(int mode, double *source_out, const double *source) {
+  if (mode != 3 && mode != 4) raise("residual_source: mode must be 3 or 4");
+  const long n = (long)eles->state_size();
+  double *divF = eles->divF_d.as<double>();
+  if (!fuse_rk) {
+    compute_residual(0);
+    if (mode == 3) status_or_throw(zfr_axpby(n, 1.0, source_out, -1.0, divF, source_out, stream));
+    else status_or_throw(zfr_axpby(n, 1.0, divF, 1.0, source, divF, stream));
+    return;
+  }
+  compute_residual_start(0);
+  compute_residual_mid(0);
+  auto d = eles->desc(*input, 2);
+  zfr_rk_args_t r{};
+  r.mode = mode;
+  r.source = source;
+  r.source_out = source_out;
+  if (onesided()) status_or_throw(zfr_face_correction_rk(&d, divF, &r, stream));
+  else status_or_throw(zfr_divergence_rk(&d, cur_state(), divF, &r, stream));
 }
 
 // step_RK_stage (src/solver.cpp:3458-3690)
