@@ -196,12 +196,43 @@ template <int D, int N>
 __device__ __forceinline__ void stage_U(const double *__restrict__ U, const KArgs &a, double *lops, double *lU,
                                         long e0, long neb, int Ustride) {
   using S = Shape<D, N>;
+  constexpr int VS = S::V * S::nS;
+  // every load of the first batch (OB table values and V state values per thread: the
+  // whole block when it has >= 64 threads and EB * nS of them) is issued before the
+  // first LDS store, so the staging costs one memory round trip, not one per value
+  constexpr int OB = (S::NOPS + 63) / 64, B = S::V;
   const int tid = threadIdx.x, NT = blockDim.x;
-  for (int i = tid; i < S::NOPS; i += NT) lops[i] = a.ops[i];
-  const double *Ub = U + e0 * S::V * S::nS;
-  for (long i = tid; i < neb * S::V * S::nS; i += NT) {
-    const long el = i / (S::V * S::nS), r = i - el * (S::V * S::nS);
-    lU[el * Ustride + r] = Ub[i];
+  const double *Ub = U + e0 * VS;
+  const long n = neb * VS;
+  double ro[OB], ru[B];
+#pragma unroll
+  for (int k = 0; k < OB; k++) {
+    const int i = tid + k * NT;
+    ro[k] = (i < S::NOPS) ? a.ops[i] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < B; k++) {
+    const long i = tid + (long)k * NT;
+    ru[k] = (i < n) ? Ub[i] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < OB; k++) {
+    const int i = tid + k * NT;
+    if (i < S::NOPS) lops[i] = ro[k];
+  }
+#pragma unroll
+  for (int k = 0; k < B; k++) {
+    const long i = tid + (long)k * NT;
+    if (i < n) {
+      const long el = i / VS;
+      lU[el * Ustride + (i - el * VS)] = ru[k];
+    }
+  }
+  // the rest, for blocks smaller than that
+  for (int i = tid + OB * NT; i < S::NOPS; i += NT) lops[i] = a.ops[i];
+  for (long i = tid + (long)B * NT; i < n; i += NT) {
+    const long el = i / VS;
+    lU[el * Ustride + (i - el * VS)] = Ub[i];
   }
 }
 
@@ -234,6 +265,12 @@ __device__ __forceinline__ void partner_trace(const double *__restrict__ U, cons
   }
 }
 
+// Whether a fused epilogue writes the array the residual reads its state from.
+inline bool rk_writes_input(const KArgs::Rk &r, const double *U) {
+  if (r.mode != 1 && r.mode != 2) return false;
+  return r.U == U || r.Uini_out == U || r.corr == U;
+}
+
 // RK update of one spt straight after its residual, in the last kernel of the residual
 // (the stage update of src/solver.cpp:2897-2941 and the final combination of
 // :2997-3062, with the same arithmetic as k_rk_stage / k_rk_final).  acc holds this
@@ -242,16 +279,30 @@ __device__ __forceinline__ void partner_trace(const double *__restrict__ U, cons
 // Side outputs: a copy of the starting state (Uini_out), and after a final combination
 // the change against a stored state (corr: p-multigrid's coarse correction,
 // src/multigrid.cpp:236-256).  Mode 3 subtracts the residual from a p-multigrid source
-// term in place (src/multigrid.cpp:474-518).
-// Whether a fused epilogue writes the array the residual reads its state from.
-inline bool rk_writes_input(const KArgs::Rk &r, const double *U) {
-  if (r.mode != 1 && r.mode != 2) return false;
-  return r.U == U || r.Uini_out == U || r.corr == U;
+// term in place (src/multigrid.cpp:474-518).  pre: the spt's starting state, source term
+// and detJ loaded ahead by rk_prefetch (null: loaded here).
+template <int D>
+struct RkPre {
+  double u0[D + 2], src[D + 2], detJ;
+};
+
+template <int D>
+__device__ __forceinline__ void rk_prefetch(const KArgs &a, long e, int t, int nS, RkPre<D> &p) {
+  constexpr int V = D + 2, nM = D * D + 1;
+  const bool upd = a.rk.mode == 1 || a.rk.mode == 2;
+  const bool src = a.rk.src && (upd || a.rk.mode == 4);
+  p.detJ = upd ? a.metric[(e * nM + D * D) * nS + t] : 1.0;
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    const long i = (e * V + v) * nS + t;
+    p.u0[v] = upd ? a.rk.Uini[i] : 0.0;
+    p.src[v] = src ? a.rk.src[i] : 0.0;
+  }
 }
 
 template <int D>
 __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc,
-                                            double *unew = nullptr) {
+                                            double *unew = nullptr, const RkPre<D> *pre = nullptr) {
   constexpr int V = D + 2, nM = D * D + 1;
   if (a.rk.mode == 0 || a.rk.mode == 4) return;
   if (a.rk.mode == 3) {
@@ -259,14 +310,15 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
     for (int v = 0; v < V; v++) a.rk.srcw[(e * V + v) * nS + t] -= acc[v];
     return;
   }
-  const double detJ = a.metric[(e * nM + D * D) * nS + t];
+  const double detJ = pre ? pre->detJ : a.metric[(e * nM + D * D) * nS + t];
 #pragma unroll
   for (int v = 0; v < V; v++) {
     const long i = (e * V + v) * nS + t;
-    const double u0 = a.rk.Uini[i];
+    const double u0 = pre ? pre->u0[v] : a.rk.Uini[i];
+    const double sv = a.rk.src ? (pre ? pre->src[v] : a.rk.src[i]) : 0.0;
     if (a.rk.Uini_out) a.rk.Uini_out[i] = u0;
     if (a.rk.mode == 1) {
-      const double r = a.rk.src ? acc[v] + a.rk.src[i] : acc[v];
+      const double r = a.rk.src ? acc[v] + sv : acc[v];
       const double u = u0 - a.rk.alpha * a.rk.dt / detJ * r;
       a.rk.U[i] = u;
       if (unew) unew[v] = u;
@@ -274,7 +326,7 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
       double u = u0;
       for (int s = 0; s < a.rk.nStages; s++) {
         double r = (s == a.rk.stage) ? acc[v] : a.rk.divF0[s * a.rk.total + i];
-        if (a.rk.src) r += a.rk.src[i];
+        if (a.rk.src) r += sv;
         u -= a.rk.beta[s] * a.rk.dt / detJ * r;
       }
       a.rk.U[i] = u;
@@ -567,11 +619,42 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
   double *lG = lC + EB * V * nF;  // gradient (viscous), then the transformed flux
   const long e0 = bid * EB;
   const long neb = min((long)EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  __syncthreads();
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   const bool valid = el < neb;
+  // Euler: everything this thread reads from global memory besides the staged block (the
+  // partner states and face geometry of its flux points, its metric, the RK epilogue's
+  // inputs) is loaded here, before the staging, so that the loads overlap one another
+  // instead of each costing a memory round trip between the phases
+  constexpr int PF = VISC ? 1 : S::FPT;
+  double pUs[PF][V], pn[PF][D], pdA[PF], pij[D][D], pdet = 1.0;
+  int psl[PF];
+  RkPre<D> pre;
+  if (!VISC && valid) {
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+      const int f = t + k * nS;
+      psl[k] = 0;
+      if (f >= nF) continue;
+      long ep; int sl, fp;
+      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      psl[k] = sl;
+      if constexpr (SELF) {
+        partner_trace<D, N>(U, a.ops, ep, fp, pUs[k]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) pUs[k][v] = a.Utr[(ep * V + v) * nF + fp];
+      }
+      const double *fg = a.fgeo + e * (D + 1) * nF + f;
+#pragma unroll
+      for (int d = 0; d < D; d++) pn[k][d] = fg[d * nF];
+      pdA[k] = fg[D * nF];
+    }
+    load_metric<D>(a, e, t, nS, pij, pdet, false);
+    rk_prefetch<D>(a, e, t, nS, pre);
+  }
+  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  __syncthreads();
   const double *u = lU + el * V * nS;
   double *c = lC + el * V * nF;
   double *g_l = lG + el * D * V * nS;
@@ -591,7 +674,28 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
   }
 
   // common flux at this thread's flux points -> lC (Ucomm has been consumed)
-  if (valid) {
+  if (!VISC && valid) {
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+      const int f = t + k * nS;
+      if (f >= nF) continue;
+      int dim, side, base, str;
+      fpt_info<D, N>(f, dim, side, base, str);
+      const double *lE = tab_lE<N>(lops, side);
+      double Uo[V], UL[V], UR[V], G1[1], G2[1], Fo[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) Uo[v] = line_extrap<N>(lE, u + v * nS, base, str);
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        UL[v] = psl[k] ? pUs[k][v] : Uo[v];
+        UR[v] = psl[k] ? Uo[v] : pUs[k][v];
+      }
+      common_flux<D, false, BM, FC>(UL, UR, G1, G2, pn[k], pdA[k], a, Fo);
+#pragma unroll
+      for (int v = 0; v < V; v++) c[v * nF + f] = Fo[v];
+    }
+  }
+  if (VISC && valid) {
 #pragma unroll 1
     for (int k = 0; k < S::FPT; k++) {
       const int f = t + k * nS;
@@ -663,7 +767,14 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
   // each thread reads and overwrites only its own spt column of lG
   if (valid) {
     double ij[D][D], detJ;
-    load_metric<D>(a, e, t, nS, ij, detJ, false);
+    if (VISC) {
+      load_metric<D>(a, e, t, nS, ij, detJ, false);
+    } else {
+#pragma unroll
+      for (int d1 = 0; d1 < D; d1++)
+#pragma unroll
+        for (int d2 = 0; d2 < D; d2++) ij[d1][d2] = pij[d1][d2];
+    }
     double Us[V];
 #pragma unroll
     for (int v = 0; v < V; v++) Us[v] = u[v * nS + t];
@@ -721,11 +832,11 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
     }
     if (a.rk.mode == 4) {
 #pragma unroll
-      for (int v = 0; v < V; v++) acc[v] += a.rk.src[(e * V + v) * nS + t];
+      for (int v = 0; v < V; v++) acc[v] += VISC ? a.rk.src[(e * V + v) * nS + t] : pre.src[v];
     }
 #pragma unroll
     for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
-    rk_epilogue<D>(a, e, t, nS, acc, unew);
+    rk_epilogue<D>(a, e, t, nS, acc, unew, VISC ? nullptr : &pre);
   }
   // the new state's full trace for the next stage, into a buffer other than the Utr this
   // launch reads; the element's U block is no longer read
