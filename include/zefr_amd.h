@@ -369,6 +369,20 @@ typedef struct zfr_rk_args {
    * source_out -= divF (source_out holding the restricted residual); mode 4 writes
    * divF += source.  U and U_ini are not read or written in these modes. */
   double *source_out;
+  /* p-multigrid transfers fused into the same kernel (PMGrid::restrict_pmg / prolong_err,
+   * src/multigrid.cpp:335-433), with zfr_pmg_transfer's arithmetic.  Restriction (modes 0
+   * and 4; mode 0 only with it): the element's U and its divF (after the source term)
+   * restricted with restrict_A [restrict_n][N] (N = d->order + 1, 1 <= restrict_n <= N)
+   * into restrict_U and restrict_divF ([E][V][restrict_n^D]); zfr_face_correction_rk
+   * reads U from restrict_src, zfr_divergence_rk restricts the U it is given.
+   * Prolongation (mode 2 with corr_out): prolong_out ([E][V][prolong_n^D], prolong_n >= N)
+   * += prolong_alpha * (prolong_A [prolong_n][N] applied to corr_out). */
+  const double *restrict_A, *restrict_src;
+  double *restrict_U, *restrict_divF;
+  int restrict_n, prolong_n;
+  const double *prolong_A;
+  double prolong_alpha;
+  double *prolong_out;
 } zfr_rk_args_t;
 int zfr_face_correction_rk(const zfr_ele_desc_t *d, double *divF, const zfr_rk_args_t *rk, void *stream);
 int zfr_divergence_rk(const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *rk,

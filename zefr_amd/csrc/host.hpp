@@ -198,6 +198,21 @@ class FRSolver {
   // copy and the difference are written by the step's own kernels; otherwise by a copy
   // and an axpby around the step.
   void update(const double *source, double *save, double *corr, const double *corr_base);
+  // p-multigrid's prolongation of the correction written by a step (prolong_err,
+  // src/multigrid.cpp:394-433): out += alpha * (A [n][order+1] per direction) corr
+  struct Prolong {
+    const double *A = nullptr;
+    int n = 0;
+    double alpha = 0.0;
+    double *out = nullptr;
+  };
+  // With fuse_rk: the step's final combination also prolongs the correction (corr and
+  // corr_base needed); the caller prolongs it otherwise.
+  void update(const double *source, double *save, double *corr, const double *corr_base, const Prolong &pro);
+  // With fuse_rk: the stage-0 residual (plus source when not null) and, in the same last
+  // kernel, U and that residual restricted with R [n_out][order+1] per direction into Uc
+  // and Rc (restrict_pmg, src/multigrid.cpp:335-392).
+  void residual_restrict(const double *source, const double *R, int n_out, double *Uc, double *Rc);
   // The stage-0 residual into divF(0) with a p-multigrid epilogue: mode 3 subtracts it
   // from source_out (source_out -= divF), mode 4 adds the source term to it
   // (divF += source).  With fuse_rk the last residual kernel does it; otherwise an axpby.
@@ -227,11 +242,14 @@ class FRSolver {
   std::array<int, 4> graph_tile = {-1, -1, -1, -1};
   bool graph_fused = false;
   const double *graph_source = nullptr;
-  std::array<const void *, 3> graph_extra = {nullptr, nullptr, nullptr};
+  std::array<const void *, 5> graph_extra = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double graph_pro_alpha = 0.0;
+  int graph_pro_n = 0;
   const double *rk_source = nullptr;  // source term of the step in progress
   double *rk_save = nullptr;          // store the step's starting state here
   double *rk_corr = nullptr;          // after the step: rk_corr = U - rk_corr_base
   const double *rk_corr_base = nullptr;
+  Prolong rk_prolong;                  // with rk_corr: prolong the correction (fused steps)
 };
 
 // p-multigrid (include/multigrid.hpp:17-44, src/multigrid.cpp:18-518) on quads/hexes:

@@ -87,11 +87,13 @@ __global__ __launch_bounds__(kMaxThreads) void k_pmg_transfer(const double *__re
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
       const int q = k % inner, o = (k / inner) % no, lo = k / (inner * no);
       const double *s = src + lo * ni * inner + q;
+      // explicit fmas: the p-multigrid epilogues of the residual kernels (tp_out,
+      // pmg_epilogue) form the same numbers
       double acc = 0.0;
-      for (int i = 0; i < ni; i++) acc += lA[o * ni + i] * s[i * inner];
+      for (int i = 0; i < ni; i++) acc = __builtin_fma(lA[o * ni + i], s[i * inner], acc);
       if (d == D - 1) {  // outer == 1: k runs over [nl][no^D] in output order
         double *g = out + l0 * szo + k;
-        *g = (beta == 0.0) ? alpha * acc : alpha * acc + beta * *g;
+        *g = (beta == 0.0) ? alpha * acc : __builtin_fma(alpha, acc, beta * *g);
       } else {
         dst[k] = acc;
       }
@@ -334,8 +336,26 @@ namespace {
 int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, const zfr_rk_args_t *r,
              hipStream_t st) {
   if (!r) return zfr::fail("RK fusion: missing buffer");
-  if (r->mode < 1 || r->mode > 4)
-    return zfr::fail("RK fusion: mode must be 1 (stage), 2 (final), 3 (source -= divF) or 4 (divF += source)");
+  const bool restr = r->restrict_U || r->restrict_divF || r->restrict_A;
+  const bool prol = r->prolong_out || r->prolong_A;
+  if (r->mode < 0 || r->mode > 4 || (r->mode == 0 && !restr))
+    return zfr::fail("RK fusion: mode must be 1 (stage), 2 (final), 3 (source -= divF), 4 (divF += source) "
+                     "or 0 (residual) with a restriction");
+  if (restr) {
+    if (r->mode != 0 && r->mode != 4) return zfr::fail("RK fusion: the restriction goes with modes 0 and 4");
+    if (!r->restrict_U || !r->restrict_divF || !r->restrict_A)
+      return zfr::fail("RK fusion: restriction needs restrict_A, restrict_U and restrict_divF");
+    if (r->restrict_n < 1 || r->restrict_n > d->order + 1)
+      return zfr::fail("RK fusion: restrict_n must be in 1..order+1");
+    if (op == zk::Op::FaceCorr && !r->restrict_src)
+      return zfr::fail("RK fusion: the face correction restricts U from restrict_src");
+  }
+  if (prol) {
+    if (r->mode != 2 || !r->corr_out) return zfr::fail("RK fusion: the prolongation goes with mode 2 and corr_out");
+    if (!r->prolong_out || !r->prolong_A) return zfr::fail("RK fusion: prolongation needs prolong_A and prolong_out");
+    if (r->prolong_n < d->order + 1 || r->prolong_n > 6)
+      return zfr::fail("RK fusion: prolong_n must be in order+1..6");
+  }
   const bool update = r->mode == 1 || r->mode == 2;
   if (update && (!r->U || !r->U_ini || (r->mode == 2 && r->nStages > 1 && !r->divF0)))
     return zfr::fail("RK fusion: missing buffer");
@@ -363,6 +383,19 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
   rk.corr = r->corr_out;
   rk.corr_base = r->corr_base;
   rk.srcw = r->source_out;
+  if (restr) {
+    rk.pmgR = r->restrict_A;
+    rk.pmgRn = r->restrict_n;
+    rk.pmgUc = r->restrict_U;
+    rk.pmgRc = r->restrict_divF;
+    rk.pmgU = r->restrict_src;
+  }
+  if (prol) {
+    rk.pmgP = r->prolong_A;
+    rk.pmgPn = r->prolong_n;
+    rk.pmgAlpha = r->prolong_alpha;
+    rk.pmgUf = r->prolong_out;
+  }
   // the divergence reads the trace when viscous, or when its epilogue writes the state it
   // reads (Euler then reads the partners' traces instead of their U)
   const bool reads_tr = op == zk::Op::Div && (d->viscous || zk::rk_writes_input(rk, U));

@@ -107,6 +107,17 @@ struct KArgs {
     double *corr;             // mode 2: corr = new U - corr_base, or null
     const double *corr_base;
     double *srcw;             // mode 3
+    // p-multigrid transfers of each element fused after its residual or update
+    // (PMGrid::restrict_pmg, prolong_err): restriction (modes 0, 4) of U and of the
+    // residual with the table pmgR [pmgRn][N] into pmgUc, pmgRc ([E][V][pmgRn^D]), the U
+    // read from pmgU when the kernel holds no copy of it; prolongation (mode 2 with corr)
+    // pmgUf += pmgAlpha * P corr with pmgP [pmgPn][N]
+    const double *pmgR, *pmgU;
+    double *pmgUc, *pmgRc;
+    int pmgRn, pmgPn;
+    const double *pmgP;
+    double pmgAlpha;
+    double *pmgUf;
   } rk;
 };
 
@@ -330,6 +341,7 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
         u -= a.rk.beta[s] * a.rk.dt / detJ * r;
       }
       a.rk.U[i] = u;
+      if (unew) unew[v] = u;
       if (a.rk.corr) a.rk.corr[i] = u - a.rk.corr_base[i];
     }
   }
@@ -364,6 +376,85 @@ __device__ __forceinline__ void rk_trace(const KArgs &a, const double *lops, dou
     const double *lE = tab_lE<N>(lops, side);
 #pragma unroll
     for (int v = 0; v < V; v++) a.rk.trace[(e * V + v) * nF + f] = line_extrap<N>(lE, w + v * nS, base, str);
+  }
+}
+
+// Output o (x fastest) of a tensor-product table A [no][N] (one 1D table per direction)
+// applied to one [N^D] block, with k_pmg_transfer's roundings: the sums along x, then y,
+// then z, each from 0.0 over increasing input index.  Every output recomputes the partial
+// sums of its lines, which are the same numbers the separate passes form.
+template <int D, int N>
+__device__ __forceinline__ double tp_out(const double *blk, const double *__restrict__ A, int no, int o) {
+  const int o1 = o % no, o2 = (o / no) % no, o3 = (D == 3) ? o / (no * no) : 0;
+  const double *A1 = A + o1 * N, *A2 = A + o2 * N, *A3 = A + o3 * N;
+  double s3 = 0.0;
+#pragma unroll 1
+  for (int i3 = 0; i3 < ((D == 3) ? N : 1); i3++) {
+    double s2 = 0.0;
+#pragma unroll 1
+    for (int i2 = 0; i2 < N; i2++) {
+      double s1 = 0.0;
+#pragma unroll
+      for (int i1 = 0; i1 < N; i1++) s1 = __builtin_fma(A1[i1], blk[(i3 * N + i2) * N + i1], s1);
+      s2 = __builtin_fma(A2[i2], s1, s2);
+    }
+    s3 = (D == 3) ? __builtin_fma(A3[i3], s2, s3) : s2;
+  }
+  return s3;
+}
+
+// The p-multigrid transfers fused at the end of a residual kernel (every thread of the
+// workgroup calls this; it holds barriers).  res: this thread's residual (after the source
+// term); unew: its new state (mode 2); ublk: the element's U block in LDS, or null (then U
+// is read from a.rk.pmgU); w: a free per-element LDS block of at least V nS doubles.
+template <int D, int N>
+__device__ __forceinline__ void pmg_epilogue(const KArgs &a, const double *ublk, double *w, long e, int t,
+                                             bool valid, const double *res, const double *unew) {
+  using S = Shape<D, N>;
+  constexpr int nS = S::nS, V = S::V;
+  const KArgs::Rk &r = a.rk;
+  auto apply = [&](const double *blk, const double *A, int no, double *out, double alpha, bool add) {
+    int noD = no * no;
+    if (D == 3) noD *= no;
+    if (!valid) return;
+#pragma unroll 1
+    for (int o = t; o < noD; o += nS) {
+#pragma unroll 1
+      for (int v = 0; v < V; v++) {
+        const double x = tp_out<D, N>(blk + v * nS, A, no, o);
+        double *g = out + (e * V + v) * noD + o;
+        *g = add ? __builtin_fma(alpha, x, 1.0 * *g) : x;
+      }
+    }
+  };
+  if (r.pmgRc && (r.mode == 0 || r.mode == 4)) {
+    __syncthreads();  // w's earlier contents are consumed
+    if (valid) {
+#pragma unroll
+      for (int v = 0; v < V; v++) w[v * nS + t] = res[v];
+    }
+    __syncthreads();
+    apply(w, r.pmgR, r.pmgRn, r.pmgRc, 1.0, false);
+    if (ublk) {
+      apply(ublk, r.pmgR, r.pmgRn, r.pmgUc, 1.0, false);
+    } else {
+      __syncthreads();
+      if (valid) {
+#pragma unroll
+        for (int v = 0; v < V; v++) w[v * nS + t] = r.pmgU[(e * V + v) * nS + t];
+      }
+      __syncthreads();
+      apply(w, r.pmgR, r.pmgRn, r.pmgUc, 1.0, false);
+    }
+  }
+  if (r.pmgUf && r.mode == 2 && r.corr) {
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int v = 0; v < V; v++) w[v * nS + t] = unew[v] - r.corr_base[(e * V + v) * nS + t];
+    }
+    __syncthreads();
+    apply(w, r.pmgP, r.pmgPn, r.pmgUf, r.pmgAlpha, true);
   }
 }
 
@@ -798,7 +889,7 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
       for (int v = 0; v < V; v++) g_l[(d * V + v) * nS + t] = tF[d][v];
   }
   __syncthreads();
-  double unew[V];
+  double unew[V], res[V];
   if (valid) {
     // divergence: oppDiv tF + oppDiv_fpts Fcomm (src/elements.cpp:1449-1501)
     const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
@@ -836,11 +927,15 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
     }
 #pragma unroll
     for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+#pragma unroll
+    for (int v = 0; v < V; v++) res[v] = acc[v];
     rk_epilogue<D>(a, e, t, nS, acc, unew, VISC ? nullptr : &pre);
   }
   // the new state's full trace for the next stage, into a buffer other than the Utr this
   // launch reads; the element's U block is no longer read
   if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, lU + el * V * nS, e, t, valid, unew, -1);
+  // p-multigrid transfers (the U block is still the staged state; lG is free)
+  if (a.rk.pmgRc || a.rk.pmgUf) pmg_epilogue<D, N>(a, lU + el * V * nS, g_l, e, t, valid, res, unew);
 }
 
 template <int D, int N, bool VISC, int BM, int FC, bool SELF = !VISC>
@@ -1151,7 +1246,7 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
     }
   }
   __syncthreads();
-  double unew[V];
+  double unew[V], res[V];
   if (valid) {
     const int ia = t % N, ib = (t / N) % N, ic = (D == 3) ? t / (N * N) : 0;
     double acc[V];
@@ -1174,10 +1269,14 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
     }
 #pragma unroll
     for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+#pragma unroll
+    for (int v = 0; v < V; v++) res[v] = acc[v];
     rk_epilogue<D>(a, e, t, nS, acc, unew);
   }
   // the new state's partner-owned trace for the next stage
   if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, c, e, t, valid, unew, owner);
+  // p-multigrid transfers (the common fluxes in c are consumed; U comes from pmgU)
+  if (a.rk.pmgRc || a.rk.pmgUf) pmg_epilogue<D, N>(a, nullptr, c, e, t, valid, res, unew);
 }
 
 template <int D, int N, int BM>

@@ -168,47 +168,75 @@ void PMGrid::compute_source_term(FRSolver &grid, double *source) { grid.residual
 // Each coarse level's stored solution is written by its first smoothing step (the state
 // it starts from), its correction (U - stored solution, the CPU path's form; the GPU
 // path of the reference leaves the difference in U as well, which nothing reads
-// afterwards) by its last one, and the residual restricted to the next level carries the
-// source term added by the residual's last kernel.
+// afterwards) by its last one.  With fused steps the residual restricted to the next
+// level (with the source term added) and the restricted U are written by the residual's
+// last kernel, and the correction is prolonged into the next finer level by the last
+// step's final combination; otherwise separate launches do the same arithmetic.
 void PMGrid::v_cycle(FRSolver &solver, int lvl) {
-  if (lvl != nLevels - 1) {
-    solver.compute_residual(0);
-    restrict_pmg(solver, *grids[lvl + 1], sourcesBT[lvl + 1].as<double>());
-  }
+  auto level_at = [&](int n) -> FRSolver & { return n == lvl ? solver : *grids[n]; };
+  // residual (+ source) of level n, restricted with U into level n + 1
+  auto restrict_from = [&](int n, const double *src) {
+    FRSolver &g = level_at(n), &gc = *grids[n + 1];
+    double *Rc = sourcesBT[n + 1].as<double>();
+    if (g.fuse_rk) {
+      g.residual_restrict(src, oppRes_d[n].as<double>(), gc.order + 1, gc.eles->U_spts_d.as<double>(), Rc);
+    } else {
+      if (src) g.residual_source(4, nullptr, src);
+      else g.compute_residual(0);
+      restrict_pmg(g, gc, Rc);
+    }
+  };
+  auto prolong = [&](int n) -> FRSolver::Prolong {
+    FRSolver &gf = level_at(n - 1);
+    FRSolver::Prolong p;
+    p.A = oppPro_d[n].as<double>();
+    p.n = gf.order + 1;
+    p.alpha = input->rel_fac;
+    p.out = gf.eles->U_spts_d.as<double>();
+    return p;
+  };
+  // steps smoothing steps of level n under its source term: the first stores the solution
+  // (store), the last writes the correction and, fused, prolongs it (finish)
+  auto smooth = [&](int n, unsigned steps, bool store, bool finish) {
+    FRSolver &g = *grids[n];
+    double *src = sourcesBT[n].as<double>(), *sol = solutionsBT[n].as<double>(),
+           *corr = correctionsBT[n].as<double>();
+    for (unsigned s = 0; s < steps; s++) {
+      const bool last = finish && s + 1 == steps;
+      g.update(src, (store && s == 0) ? sol : nullptr, last ? corr : nullptr, last ? sol : nullptr,
+               (last && g.fuse_rk) ? prolong(n) : FRSolver::Prolong{});
+    }
+  };
+  // the correction of a level without smoothing steps, and the unfused prolongation
+  auto finish_level = [&](int n, bool stepped) {
+    FRSolver &g = *grids[n];
+    double *sol = solutionsBT[n].as<double>(), *corr = correctionsBT[n].as<double>();
+    if (!stepped)
+      status_or_throw(zfr_axpby((long)g.eles->state_size(), 1.0, g.eles->U_spts_d.as<double>(), -1.0, sol, corr,
+                                g.stream));
+    if (!stepped || !g.fuse_rk) prolong_err(g, corr, level_at(n - 1));
+  };
+
+  if (lvl != nLevels - 1) restrict_from(lvl, nullptr);
   for (int n = lvl + 1; n < nLevels; n++) {
     FRSolver &g = *grids[n];
     Elements &e = *g.eles;
-    const size_t bytes = e.state_size() * sizeof(double);
-    double *src = sourcesBT[n].as<double>(), *sol = solutionsBT[n].as<double>(),
-           *corr = correctionsBT[n].as<double>();
-    compute_source_term(g, src);
+    compute_source_term(g, sourcesBT[n].as<double>());
     const unsigned steps = input->mg_steps[n];
-    if (steps == 0)
-      hip_or_throw(hipMemcpyAsync(sol, e.U_spts_d.p, bytes, hipMemcpyDeviceToDevice, g.stream), "hipMemcpyAsync");
     const bool coarsest = n == nLevels - 1;
-    for (unsigned s = 0; s < steps; s++) {
-      // the coarsest level is not smoothed again on the way up: its last step here ends it
-      const bool last = coarsest && s + 1 == steps;
-      g.update(src, s == 0 ? sol : nullptr, last ? corr : nullptr, last ? sol : nullptr);
-    }
-    if (!coarsest) {
-      g.residual_source(4, nullptr, src);
-      restrict_pmg(g, *grids[n + 1], sourcesBT[n + 1].as<double>());
-    }
-  }
-  for (int n = nLevels - 1; n > lvl; n--) {
-    FRSolver &g = *grids[n];
-    Elements &e = *g.eles;
-    double *sol = solutionsBT[n].as<double>(), *corr = correctionsBT[n].as<double>();
-    const unsigned steps = input->mg_steps[n];
-    if (n != nLevels - 1)
-      for (unsigned s = 0; s < steps; s++)
-        g.update(sourcesBT[n].as<double>(), nullptr, s + 1 == steps ? corr : nullptr, s + 1 == steps ? sol : nullptr);
     if (steps == 0)
-      status_or_throw(zfr_axpby((long)e.state_size(), 1.0, e.U_spts_d.as<double>(), -1.0, sol, corr, g.stream));
-    if (n > lvl + 1) prolong_err(g, corr, *grids[n - 1]);
+      hip_or_throw(hipMemcpyAsync(solutionsBT[n].p, e.U_spts_d.p, e.state_size() * sizeof(double),
+                                  hipMemcpyDeviceToDevice, g.stream), "hipMemcpyAsync");
+    // the coarsest level is not smoothed again on the way up: its last step here ends it
+    smooth(n, steps, true, coarsest);
+    if (coarsest) finish_level(n, steps > 0);
+    else restrict_from(n, sourcesBT[n].as<double>());
   }
-  if (lvl != nLevels - 1) prolong_err(*grids[lvl + 1], correctionsBT[lvl + 1].as<double>(), solver);
+  for (int n = nLevels - 2; n > lvl; n--) {
+    const unsigned steps = input->mg_steps[n];
+    smooth(n, steps, false, true);
+    finish_level(n, steps > 0);
+  }
 }
 
 PMGrid::~PMGrid() { drop_graph(); }

@@ -498,6 +498,12 @@ void FRSolver::finish_with_rk(int stage, double *trace_out, bool split) {
     if (split) r.U_ini_out = I;  // the starting state, where the unfused step leaves it
     r.corr_out = rk_corr;
     r.corr_base = rk_corr_base;
+    if (rk_corr && rk_prolong.out) {
+      r.prolong_A = rk_prolong.A;
+      r.prolong_n = rk_prolong.n;
+      r.prolong_alpha = rk_prolong.alpha;
+      r.prolong_out = rk_prolong.out;
+    }
   }
   double *divF = eles->divF_d.as<double>() + (size_t)stage * st;
   if (onesided()) status_or_throw(zfr_face_correction_rk(&d, divF, &r, stream));
@@ -584,9 +590,10 @@ void FRSolver::step_RK() {
     step_RK_eager();
     return;
   }
-  const std::array<const void *, 3> extra = {rk_save, rk_corr, rk_corr_base};
+  const std::array<const void *, 5> extra = {rk_save, rk_corr, rk_corr_base, rk_prolong.A, rk_prolong.out};
   if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile_key() || graph_fused != fuse_rk ||
-      graph_source != rk_source || graph_extra != extra) {
+      graph_source != rk_source || graph_extra != extra || graph_pro_alpha != rk_prolong.alpha ||
+      graph_pro_n != rk_prolong.n) {
     drop_graph();
     hip_or_throw(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
@@ -604,6 +611,8 @@ void FRSolver::step_RK() {
     graph_fused = fuse_rk;
     graph_source = rk_source;
     graph_extra = extra;
+    graph_pro_alpha = rk_prolong.alpha;
+    graph_pro_n = rk_prolong.n;
   }
   hip_or_throw(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
 }
@@ -620,17 +629,25 @@ void FRSolver::update() { update(nullptr, nullptr, nullptr, nullptr); }
 void FRSolver::update(const double *source) { update(source, nullptr, nullptr, nullptr); }
 
 void FRSolver::update(const double *source, double *save, double *corr, const double *corr_base) {
+  update(source, save, corr, corr_base, Prolong{});
+}
+
+void FRSolver::update(const double *source, double *save, double *corr, const double *corr_base,
+                      const Prolong &pro) {
   if (corr && !corr_base) raise("update: corr needs corr_base");
+  if (pro.out && (!fuse_rk || !corr)) raise("update: the prolongation is fused into a fused step with corr");
   prev_time = flow_time;
   rk_source = source;
   rk_save = save;
   rk_corr = corr;
   rk_corr_base = corr_base;
+  rk_prolong = pro;
   auto clear = [&] {
     rk_source = nullptr;
     rk_save = nullptr;
     rk_corr = nullptr;
     rk_corr_base = nullptr;
+    rk_prolong = Prolong{};
   };
   try {
     step_RK();
@@ -641,6 +658,24 @@ void FRSolver::update(const double *source, double *save, double *corr, const do
   clear();
   flow_time = prev_time + input->dt;
   current_iter++;
+}
+
+void FRSolver::residual_restrict(const double *source, const double *R, int n_out, double *Uc, double *Rc) {
+  if (!fuse_rk) raise("residual_restrict: the restriction is fused into the residual's last kernel (fuse_rk)");
+  compute_residual_start(0);
+  compute_residual_mid(0);
+  auto d = eles->desc(*input, 2);
+  zfr_rk_args_t r{};
+  r.mode = source ? 4 : 0;
+  r.source = source;
+  r.restrict_A = R;
+  r.restrict_n = n_out;
+  r.restrict_U = Uc;
+  r.restrict_divF = Rc;
+  r.restrict_src = cur_state();
+  double *divF = eles->divF_d.as<double>();
+  if (onesided()) status_or_throw(zfr_face_correction_rk(&d, divF, &r, stream));
+  else status_or_throw(zfr_divergence_rk(&d, cur_state(), divF, &r, stream));
 }
 
 void FRSolver::residual_source(int mode, double *source_out, const double *source) {
