@@ -382,21 +382,33 @@ __device__ __forceinline__ void rk_trace(const KArgs &a, const double *lops, dou
 // Output o (x fastest) of a tensor-product table A [no][N] (one 1D table per direction)
 // applied to one [N^D] block, with k_pmg_transfer's roundings: the sums along x, then y,
 // then z, each from 0.0 over increasing input index.  Every output recomputes the partial
-// sums of its lines, which are the same numbers the separate passes form.
+// sums of its lines, which are the same numbers the separate passes form.  blk and A are
+// in LDS.
 template <int D, int N>
-__device__ __forceinline__ double tp_out(const double *blk, const double *__restrict__ A, int no, int o) {
+__device__ __forceinline__ double tp_out(const double *blk, const double *A, int no, int o) {
   const int o1 = o % no, o2 = (o / no) % no, o3 = (D == 3) ? o / (no * no) : 0;
   const double *A1 = A + o1 * N, *A2 = A + o2 * N, *A3 = A + o3 * N;
+  double a1[N];
+#pragma unroll
+  for (int i1 = 0; i1 < N; i1++) a1[i1] = A1[i1];
   double s3 = 0.0;
 #pragma unroll 1
   for (int i3 = 0; i3 < ((D == 3) ? N : 1); i3++) {
+    const double *b = blk + i3 * N * N;
     double s2 = 0.0;
-#pragma unroll 1
-    for (int i2 = 0; i2 < N; i2++) {
+    auto line = [&](int i2) {
       double s1 = 0.0;
 #pragma unroll
-      for (int i1 = 0; i1 < N; i1++) s1 = __builtin_fma(A1[i1], blk[(i3 * N + i2) * N + i1], s1);
+      for (int i1 = 0; i1 < N; i1++) s1 = __builtin_fma(a1[i1], b[i2 * N + i1], s1);
       s2 = __builtin_fma(A2[i2], s1, s2);
+    };
+    if constexpr (D == 3) {
+      // kept rolled: the hex kernels that carry this epilogue hold no registers to spare
+#pragma unroll 1
+      for (int i2 = 0; i2 < N; i2++) line(i2);
+    } else {
+#pragma unroll
+      for (int i2 = 0; i2 < N; i2++) line(i2);
     }
     s3 = (D == 3) ? __builtin_fma(A3[i3], s2, s3) : s2;
   }
@@ -406,55 +418,56 @@ __device__ __forceinline__ double tp_out(const double *blk, const double *__rest
 // The p-multigrid transfers fused at the end of a residual kernel (every thread of the
 // workgroup calls this; it holds barriers).  res: this thread's residual (after the source
 // term); unew: its new state (mode 2); ublk: the element's U block in LDS, or null (then U
-// is read from a.rk.pmgU); w: a free per-element LDS block of at least V nS doubles.
+// is read from a.rk.pmgU); w: a free per-element LDS block of at least V nS doubles; lops:
+// the block's operator tables, no longer needed, which take the transfer table (at most
+// 6 N doubles, within the tables' N^2 + 4 N, padded).
 template <int D, int N>
-__device__ __forceinline__ void pmg_epilogue(const KArgs &a, const double *ublk, double *w, long e, int t,
-                                             bool valid, const double *res, const double *unew) {
+__device__ __forceinline__ void pmg_epilogue(const KArgs &a, double *lops, const double *ublk, double *w, long e,
+                                             int t, bool valid, const double *res, const double *unew) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, V = S::V;
   const KArgs::Rk &r = a.rk;
-  auto apply = [&](const double *blk, const double *A, int no, double *out, double alpha, bool add) {
+  // each (variable, output point) pair of the element is one work item
+  auto apply = [&](const double *blk, int no, double *out, double alpha, bool add) {
     int noD = no * no;
     if (D == 3) noD *= no;
     if (!valid) return;
 #pragma unroll 1
-    for (int o = t; o < noD; o += nS) {
-#pragma unroll 1
-      for (int v = 0; v < V; v++) {
-        const double x = tp_out<D, N>(blk + v * nS, A, no, o);
-        double *g = out + (e * V + v) * noD + o;
-        *g = add ? __builtin_fma(alpha, x, 1.0 * *g) : x;
-      }
+    for (int k = t; k < V * noD; k += nS) {
+      const int v = k / noD, o = k - v * noD;
+      const double x = tp_out<D, N>(blk + v * nS, lops, no, o);
+      double *g = out + (e * V + v) * noD + o;
+      *g = add ? __builtin_fma(alpha, x, 1.0 * *g) : x;
     }
   };
-  if (r.pmgRc && (r.mode == 0 || r.mode == 4)) {
-    __syncthreads();  // w's earlier contents are consumed
-    if (valid) {
+  const bool restr = r.pmgRc && (r.mode == 0 || r.mode == 4);
+  const bool prol = r.pmgUf && r.mode == 2 && r.corr;
+  if (!restr && !prol) return;
+  __syncthreads();  // lops and w are consumed
+  const double *A = restr ? r.pmgR : r.pmgP;
+  const int nA = (restr ? r.pmgRn : r.pmgPn) * N;
+  for (int i = threadIdx.x; i < nA; i += blockDim.x) lops[i] = A[i];
+  if (valid) {
 #pragma unroll
-      for (int v = 0; v < V; v++) w[v * nS + t] = res[v];
-    }
-    __syncthreads();
-    apply(w, r.pmgR, r.pmgRn, r.pmgRc, 1.0, false);
-    if (ublk) {
-      apply(ublk, r.pmgR, r.pmgRn, r.pmgUc, 1.0, false);
-    } else {
-      __syncthreads();
-      if (valid) {
-#pragma unroll
-        for (int v = 0; v < V; v++) w[v * nS + t] = r.pmgU[(e * V + v) * nS + t];
-      }
-      __syncthreads();
-      apply(w, r.pmgR, r.pmgRn, r.pmgUc, 1.0, false);
-    }
+    for (int v = 0; v < V; v++)
+      w[v * nS + t] = restr ? res[v] : unew[v] - r.corr_base[(e * V + v) * nS + t];
   }
-  if (r.pmgUf && r.mode == 2 && r.corr) {
+  __syncthreads();
+  if (prol) {
+    apply(w, r.pmgPn, r.pmgUf, r.pmgAlpha, true);
+    return;
+  }
+  apply(w, r.pmgRn, r.pmgRc, 1.0, false);
+  if (ublk) {
+    apply(ublk, r.pmgRn, r.pmgUc, 1.0, false);
+  } else {
     __syncthreads();
     if (valid) {
 #pragma unroll
-      for (int v = 0; v < V; v++) w[v * nS + t] = unew[v] - r.corr_base[(e * V + v) * nS + t];
+      for (int v = 0; v < V; v++) w[v * nS + t] = r.pmgU[(e * V + v) * nS + t];
     }
     __syncthreads();
-    apply(w, r.pmgP, r.pmgPn, r.pmgUf, r.pmgAlpha, true);
+    apply(w, r.pmgRn, r.pmgUc, 1.0, false);
   }
 }
 
@@ -935,7 +948,7 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
   // launch reads; the element's U block is no longer read
   if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, lU + el * V * nS, e, t, valid, unew, -1);
   // p-multigrid transfers (the U block is still the staged state; lG is free)
-  if (a.rk.pmgRc || a.rk.pmgUf) pmg_epilogue<D, N>(a, lU + el * V * nS, g_l, e, t, valid, res, unew);
+  if (a.rk.pmgRc || a.rk.pmgUf) pmg_epilogue<D, N>(a, lops, lU + el * V * nS, g_l, e, t, valid, res, unew);
 }
 
 template <int D, int N, bool VISC, int BM, int FC, bool SELF = !VISC>
@@ -1276,7 +1289,7 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
   // the new state's partner-owned trace for the next stage
   if (a.rk.mode == 1 && a.rk.trace) rk_trace<D, N>(a, lops, c, e, t, valid, unew, owner);
   // p-multigrid transfers (the common fluxes in c are consumed; U comes from pmgU)
-  if (a.rk.pmgRc || a.rk.pmgUf) pmg_epilogue<D, N>(a, nullptr, c, e, t, valid, res, unew);
+  if (a.rk.pmgRc || a.rk.pmgUf) pmg_epilogue<D, N>(a, lops, nullptr, c, e, t, valid, res, unew);
 }
 
 template <int D, int N, int BM>
