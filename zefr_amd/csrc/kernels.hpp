@@ -978,11 +978,13 @@ __global__ __launch_bounds__(kMaxThreads) void k_divergence_multi(const MultiCas
 // LDS allows per CU, and both face phases fit it (Rusanov 110 VGPRs; Roe at the cap) once
 // each gradient line's sum is pinned before the next line is loaded (the value-tied asm
 // fences in the face phase: without them the scheduler issued all the face phase's LDS
-// loads before their FMAs, 168 VGPRs for Roe, 196 for quads at p=4).  Other shapes keep
-// the default allocation (quads at p=4: 90 VGPRs, five waves).
+// loads before their FMAs, 168 VGPRs for Roe, 196 for quads at p=4).  Quads at p=4 with
+// Rusanov fit six waves (78-80 VGPRs, no spills; the default allocation took 90 VGPRs,
+// five waves); seven spill.  Other shapes keep the default allocation (at six waves the
+// quad kernels at p = 1..3 spill).
 template <int D, int N, int FC>
 struct OneSidedWaves {
-  static constexpr int value = (D == 3 && N == 4) ? 4 : 1;
+  static constexpr int value = (D == 3 && N == 4) ? 4 : ((D == 2 && N == 5 && FC == 0) ? 6 : 1);
 };
 
 // SELF: extrapolate the partner traces this element needs from the partner's U (no trace
