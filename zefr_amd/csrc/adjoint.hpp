@@ -95,7 +95,7 @@ struct AdjLds {
 
 template <int D, int N, bool VISC, int BM, int FC>
 __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, const double *__restrict__ w,
-                                               const double *__restrict__ Wtr, double *__restrict__ Xtr,
+                                               double *__restrict__ Wtr, double *__restrict__ Xtr,
                                                double *__restrict__ vout, const KArgs &a, long bid) {
   using S = Shape<D, N>;
   using L = AdjLds<D, N, VISC>;
@@ -218,7 +218,9 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
   //    accumulation order is fixed)
   const unsigned mask = valid ? lmask[el] : 0u;
   const int cnt = __popc(mask) * S::nFF;
-  double uob[S::FPT][V];  // viscous: the owned points' trace cotangents, stored once the gradient is read
+  // viscous: the owned points' trace cotangents go to LDS once the gradient is read; until
+  // then each waits in its own point's Wtr entry, which this thread has just read and no
+  // other element reads (the partner reads Wtr only at the points it owns)
   int fown[S::FPT];
 #pragma unroll
   for (int kk = 0; kk < S::FPT; kk++) fown[kk] = -1;
@@ -284,7 +286,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
       if (VISC) {
         fown[kk] = f;
 #pragma unroll
-        for (int v = 0; v < V; v++) uob[kk][v] = Uob[v];
+        for (int v = 0; v < V; v++) Wtr[(e * V + v) * nF + f] = Uob[v];
       } else {
 #pragma unroll
         for (int v = 0; v < V; v++) to[v * nF + f] = Uob[v];
@@ -315,10 +317,14 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
     for (int kk = 0; kk < S::FPT; kk++) {
       if (fown[kk] < 0) continue;
 #pragma unroll
-      for (int v = 0; v < V; v++) to[v * nF + fown[kk]] = uob[kk][v];
+      for (int v = 0; v < V; v++) to[v * nF + fown[kk]] = Wtr[(e * V + v) * nF + fown[kk]];
     }
     // 5. reference-space gradient cotangent: grb[d2][v] = sum_d1 adj[d2][d1] gb[v][d1] / detJ
+    //    (the metric read again here: held across the face adjoint it takes registers the
+    //    Roe reverse sweep needs)
     if (valid) {
+      asm volatile("" ::: "memory");
+      load_metric<D>(a, e, t, nS, ij, detJ, true);
       double gv[V][D];
 #pragma unroll
       for (int j = 0; j < D; j++)
@@ -381,7 +387,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
 
 template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
-void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
+void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, double *__restrict__ Wtr,
                 double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
   adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, xcd_block(blockIdx.x, gridDim.x));
 }
@@ -390,7 +396,7 @@ void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, cons
 // workgroup; one wave for hex p=3): its barriers need no cross-wave synchronisation.
 template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-void k_adj_main_w64(const double *__restrict__ U, const double *__restrict__ w, const double *__restrict__ Wtr,
+void k_adj_main_w64(const double *__restrict__ U, const double *__restrict__ w, double *__restrict__ Wtr,
                     double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
   adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, xcd_block(blockIdx.x, gridDim.x));
 }
