@@ -203,6 +203,61 @@ __device__ __forceinline__ bool need_left_grad(double beta) { return beta != -0.
 __device__ __forceinline__ bool need_right_grad(double beta) { return beta != 0.5; }
 
 // Stage EB consecutive elements' U blocks and the 1D tables into LDS.
+// stage_U in two halves, so that a kernel can issue other loads between them: the first
+// batch of loads into registers, then the LDS stores and the rest.
+template <int D, int N>
+struct StageRegs {
+  double ro[(Shape<D, N>::NOPS + 63) / 64], ru[Shape<D, N>::V];
+};
+
+template <int D, int N>
+__device__ __forceinline__ void stage_U_load(const double *__restrict__ U, const KArgs &a, long e0, long neb,
+                                             StageRegs<D, N> &r) {
+  using S = Shape<D, N>;
+  constexpr int VS = S::V * S::nS, OB = (S::NOPS + 63) / 64, B = S::V;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const double *Ub = U + e0 * VS;
+  const long n = neb * VS;
+#pragma unroll
+  for (int k = 0; k < OB; k++) {
+    const int i = tid + k * NT;
+    r.ro[k] = (i < S::NOPS) ? a.ops[i] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < B; k++) {
+    const long i = tid + (long)k * NT;
+    r.ru[k] = (i < n) ? Ub[i] : 0.0;
+  }
+}
+
+template <int D, int N>
+__device__ __forceinline__ void stage_U_store(const double *__restrict__ U, const KArgs &a, double *lops,
+                                              double *lU, long e0, long neb, int Ustride, const StageRegs<D, N> &r) {
+  using S = Shape<D, N>;
+  constexpr int VS = S::V * S::nS, OB = (S::NOPS + 63) / 64, B = S::V;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const double *Ub = U + e0 * VS;
+  const long n = neb * VS;
+#pragma unroll
+  for (int k = 0; k < OB; k++) {
+    const int i = tid + k * NT;
+    if (i < S::NOPS) lops[i] = r.ro[k];
+  }
+#pragma unroll
+  for (int k = 0; k < B; k++) {
+    const long i = tid + (long)k * NT;
+    if (i < n) {
+      const long el = i / VS;
+      lU[el * Ustride + (i - el * VS)] = r.ru[k];
+    }
+  }
+  for (int i = tid + OB * NT; i < S::NOPS; i += NT) lops[i] = a.ops[i];
+  for (long i = tid + (long)B * NT; i < n; i += NT) {
+    const long el = i / VS;
+    lU[el * Ustride + (i - el * VS)] = Ub[i];
+  }
+}
+
 template <int D, int N>
 __device__ __forceinline__ void stage_U(const double *__restrict__ U, const KArgs &a, double *lops, double *lU,
                                         long e0, long neb, int Ustride) {
@@ -280,6 +335,40 @@ __device__ __forceinline__ void partner_trace(const double *__restrict__ U, cons
 inline bool rk_writes_input(const KArgs::Rk &r, const double *U) {
   if (r.mode != 1 && r.mode != 2) return false;
   return r.U == U || r.Uini_out == U || r.corr == U;
+}
+
+// partner_trace in two halves: the partner's N-point solution line through fp (V N values
+// into registers, with the table row), then the extrapolation, the same sums.
+template <int D, int N>
+struct PartnerLine {
+  double x[Shape<D, N>::V][N], l[N];
+};
+
+template <int D, int N>
+__device__ __forceinline__ void partner_line_load(const double *__restrict__ U, const double *ops, long ep, int fp,
+                                                  PartnerLine<D, N> &p) {
+  using S = Shape<D, N>;
+  int dim, side, base, str;
+  fpt_info<D, N>(fp, dim, side, base, str);
+  const double *lE = tab_lE<N>(ops, side);
+  const double *Up = U + ep * S::V * S::nS + base;
+#pragma unroll
+  for (int m = 0; m < N; m++) p.l[m] = lE[m];
+#pragma unroll
+  for (int v = 0; v < S::V; v++)
+#pragma unroll
+    for (int m = 0; m < N; m++) p.x[v][m] = Up[v * S::nS + m * str];
+}
+
+template <int D, int N>
+__device__ __forceinline__ void partner_line_reduce(const PartnerLine<D, N> &p, double *out) {
+#pragma unroll
+  for (int v = 0; v < Shape<D, N>::V; v++) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < N; m++) s += p.l[m] * p.x[v][m];
+    out[v] = s;
+  }
 }
 
 // RK update of one spt straight after its residual, in the last kernel of the residual
@@ -1016,20 +1105,35 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
   // partner's trace: with SELF extrapolated here from the partner's solution line
   // (no trace launch runs before this kernel), else read from the partner's Utr.  Issued
   // before the staging barrier so the loads overlap it.
+  // The staging loads and all neighbour entries are issued first, then every partner load,
+  // and only then are values consumed: the prologue costs two dependent memory round trips
+  // rather than two per flux point plus the staging's.
   int slot[S::FPT];
   double cp[S::FPT][V];
   constexpr bool self = SELF;
+  StageRegs<D, N> sr;
+  stage_U_load<D, N>(U, a, e0, neb, sr);
+  unsigned nb[S::FPT];
+#pragma unroll
+  for (int k = 0; k < S::FPT; k++) {
+    const int f = t + k * nS;
+    nb[k] = (valid && f < nF) ? a.nbr[e * nF + f] : 0u;
+  }
+  PartnerLine<D, N> pl[self ? S::FPT : 1];
+  bool own[S::FPT];
 #pragma unroll
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
     slot[k] = -1;
+    own[k] = false;
     if (valid && f < nF) {
       long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      unpack_nbr(nb[k], ep, sl, fp);
       slot[k] = sl;
-      if (sl == owner) {
-        if (self) {
-          partner_trace<D, N>(U, a.ops, ep, fp, cp[k]);
+      own[k] = sl == owner;
+      if (own[k]) {
+        if constexpr (self) {
+          partner_line_load<D, N>(U, a.ops, ep, fp, pl[k]);
         } else {
 #pragma unroll
           for (int v = 0; v < V; v++) cp[k][v] = a.Utr[(ep * V + v) * nF + fp];
@@ -1037,7 +1141,12 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       }
     }
   }
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
+  if constexpr (self) {
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++)
+      if (own[k]) partner_line_reduce<D, N>(pl[k], cp[k]);
+  }
+  stage_U_store<D, N>(U, a, lops, lU, e0, neb, V * nS, sr);
   if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
   __syncthreads();
   const double *u = lU + el * V * nS;
