@@ -1040,8 +1040,17 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
   if (a.rk.pmgRc || a.rk.pmgUf) pmg_epilogue<D, N>(a, lops, lU + el * V * nS, g_l, e, t, valid, res, unew);
 }
 
+// Occupancy target of the divergence kernel: the general-beta hex p=3 Rusanov kernel fits
+// four waves per SIMD (it took 130 VGPRs, three waves, once it carried the p-multigrid
+// transfer epilogue); other shapes keep the default allocation.
+template <int D, int N, bool VISC, int FC>
+struct DivWaves {
+  static constexpr int value = (D == 3 && N == 4 && VISC && FC == 0) ? 4 : 1;
+};
+
 template <int D, int N, bool VISC, int BM, int FC, bool SELF = !VISC>
-__global__ __launch_bounds__(kMaxThreads) void k_divergence(const double *U, double *__restrict__ divF, KArgs a) {
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(DivWaves<D, N, VISC, FC>::value)))
+void k_divergence(const double *U, double *__restrict__ divF, KArgs a) {
   static_assert(!(VISC && SELF), "the viscous divergence reads the gradient traces, hence the traces too");
   divergence_block<D, N, VISC, BM, FC, SELF>(U, divF, a, xcd_block(blockIdx.x, gridDim.x));
 }
