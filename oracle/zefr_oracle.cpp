@@ -105,7 +105,20 @@ std::vector<double> gl_points_d(unsigned n) {
     case 6:
       return {-0.932469514203152, -0.661209386466264, -0.238619186083197,
               0.238619186083197,  0.661209386466264,  0.932469514203152};
-    default: fail("oracle: Gauss-Legendre points only for 1..6 points");
+    case 7:
+      return {-0.949107912342758, -0.741531185599394, -0.405845151377397, 0.0,
+              0.405845151377397,  0.741531185599394,  0.949107912342758};
+    case 8:
+      return {-0.960289856497536, -0.796666477413627, -0.525532409916329, -0.183434642495650,
+              0.183434642495650,  0.525532409916329,  0.796666477413627,  0.960289856497536};
+    case 9:
+      return {-0.968160239507626, -0.836031107326636, -0.613371432700590, -0.324253423403809, 0.0,
+              0.324253423403809,  0.613371432700590,  0.836031107326636,  0.968160239507626};
+    case 10:
+      return {-0.973906528517172, -0.865063366688985, -0.679409568299024, -0.433895394129247,
+              -0.148874338981631, 0.148874338981631,  0.433895394129247,  0.679409568299024,
+              0.865063366688985,  0.973906528517172};
+    default: fail("oracle: Gauss-Legendre points only for 1..10 points");
   }
 }
 
@@ -126,7 +139,19 @@ std::vector<double> gl_weights_d(unsigned n) {
     case 6:
       return {0.171324492379171, 0.360761573048139, 0.467913934572691,
               0.467913934572691, 0.360761573048139, 0.171324492379171};
-    default: fail("oracle: Gauss-Legendre weights only for 1..6 points");
+    case 7:
+      return {0.129484966168870, 0.279705391489277, 0.381830050505119, 0.417959183673469,
+              0.381830050505119, 0.279705391489277, 0.129484966168870};
+    case 8:
+      return {0.101228536290377, 0.222381034453374, 0.313706645877887, 0.362683783378362,
+              0.362683783378362, 0.313706645877887, 0.222381034453374, 0.101228536290377};
+    case 9:
+      return {0.081274388361575, 0.180648160694857, 0.260610696402936, 0.312347077040003, 0.330239355001260,
+              0.312347077040003, 0.260610696402936, 0.180648160694857, 0.081274388361575};
+    case 10:
+      return {0.066671344308688, 0.149451349150581, 0.219086362515982, 0.269266719309996, 0.295524224714753,
+              0.295524224714753, 0.269266719309996, 0.219086362515982, 0.149451349150581, 0.066671344308688};
+    default: fail("oracle: Gauss-Legendre weights only for 1..10 points");
   }
 }
 
@@ -1343,7 +1368,7 @@ void *zo_create(const char *meshfile, int nDims, int order, const zo_phys *p) {
   Oracle *o = new Oracle();
   int rc = guarded([&] {
     if (nDims != 2 && nDims != 3) fail("oracle: nDims must be 2 or 3");
-    if (order < 0 || order > 5) fail("oracle: order must be in 0..5");
+    if (order < 0 || order > 9) fail("oracle: order must be in 0..9");
     if (p->nStages != 1 && p->nStages != 4) fail("oracle: nStages must be 1 or 4");
     if (p->fconv != 0 && p->fconv != 2) fail("oracle: fconv must be 0 (Rusanov) or 2 (Roe)");
     o->nDims = nDims; o->order = order;

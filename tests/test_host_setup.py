@@ -57,6 +57,8 @@ CASES = [
     dict(D=2, n=(5, 3), lo=(0.0, 0.0), hi=(1.0, 2.0), order=2, perturb=0.04),
     dict(D=2, n=(6, 6), lo=(0.0, 0.0), hi=(TWO_PI, TWO_PI), order=4, perturb=0.0),
     dict(D=2, n=(3, 3), lo=(0.0, 0.0), hi=(1.0, 1.0), order=0, perturb=0.0),
+    dict(D=2, n=(3, 4), lo=(0.0, 0.0), hi=(1.0, 1.5), order=7, perturb=0.03),
+    dict(D=2, n=(3, 3), lo=(0.0, 0.0), hi=(TWO_PI, TWO_PI), order=9, perturb=0.0),
     dict(D=3, n=(3, 3, 3), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=3, perturb=0.0),
     dict(D=3, n=(2, 3, 4), lo=(0.0, -1.0, 0.5), hi=(1.0, 1.0, 2.0), order=2, perturb=0.03),
     dict(D=3, n=(2, 2, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=4, perturb=0.02),
@@ -182,6 +184,25 @@ def test_triangle_mesh_is_rejected_with_the_reason():
     z.read_input(path)
     with pytest.raises(RuntimeError, match="triangular elements.*volume divergence"):
         z.setup_host()
+
+
+@pytest.mark.parametrize("D,order,ok", [(2, 9, True), (2, 10, False), (3, 5, True), (3, 6, False)])
+def test_order_limits(D, order, ok):
+    """Quads run to order 9 (the reference's Gauss-Legendre tables end at 10 points); hexes
+    to order 5, with a message naming the limit."""
+    from zefr_amd import zefr
+    n = (2,) * D
+    mesh = make_mesh(f"ordlim{D}.msh", n, (0.0,) * D, (1.0,) * D)
+    lines = [l.format(mesh=mesh) for l in BASE if not l.startswith(("nDims", "order"))]
+    path = _deck_lines(tmp(f"ordlim{D}_{order}.deck"), lines + [f"nDims {D}", f"order {order}"])
+    z = zefr.Zefr()
+    z.read_input(path)
+    if ok:
+        z.setup_host()
+        assert z.dims()["nSpts"] == (order + 1) ** D
+    else:
+        with pytest.raises(RuntimeError, match="order must be in 0.." + ("9 for quads" if D == 2 else "5 for hexes")):
+            z.setup_host()
 
 
 def test_deck_rejects_missing_required_key():

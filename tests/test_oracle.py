@@ -86,16 +86,30 @@ def test_gimmik_fixture_shapes():
 
 
 # ---------------------------------------------------------------- closed forms
-@pytest.mark.parametrize("n", range(1, 7))
+# The reference's 10-point table (src/points.cpp:86-96) lists +-0.433895394129247 where the
+# Gauss-Legendre node is +-0.433395394129247 (one digit changed; its weight entry is the
+# correct one for the true node).  The restatement keeps the reference's value, since
+# the point set defines the p = 9 discretisation being reproduced; the check below pins
+# every other entry and that this one differs by exactly that digit.
+REF_GL10_NODE = 0.433895394129247
+
+
+@pytest.mark.parametrize("n", range(1, 11))
 def test_gauss_legendre_rule(n):
     p, w = gl_points(n)
     xr, wr = np.polynomial.legendre.leggauss(n)
     assert np.all(np.diff(p) > 0)
+    if n == 10:
+        moved = [3, 6]
+        np.testing.assert_array_equal(p[moved], [-REF_GL10_NODE, REF_GL10_NODE])
+        np.testing.assert_allclose(np.abs(p[moved]) - np.abs(xr[moved]), 5e-4, rtol=0, atol=2e-15)
+        keep = [i for i in range(n) if i not in moved]
+        p, xr = p[keep], xr[keep]
     np.testing.assert_allclose(p, xr, rtol=0, atol=2e-15)
     np.testing.assert_allclose(w, wr, rtol=0, atol=2e-15)
 
 
-@pytest.mark.parametrize("n", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 8, 10])
 def test_lagrange_interpolates_polynomials_exactly(n):
     grid, _ = gl_points(n)
     grid = np.concatenate([[-1.0], grid, [1.0]])  # the DFR grid [-1, GL, 1]
@@ -142,7 +156,7 @@ def _tensor_poly(deg, D, seed):
     return f, g
 
 
-@pytest.mark.parametrize("D,p", [(2, 1), (2, 3), (2, 5), (3, 2), (3, 3), (3, 4)])
+@pytest.mark.parametrize("D,p", [(2, 1), (2, 3), (2, 5), (2, 7), (2, 9), (3, 2), (3, 3), (3, 4)])
 def test_operators_polynomially_exact(D, p):
     o = _hex(f"ex{p}", order=p) if D == 3 else _quad(f"exq{p}", order=p)
     nS, nF = o.nSpts, o.nFpts

@@ -19,8 +19,22 @@
 
 namespace zfr {
 
-// src/points.cpp:9-98 (closed forms up to 5 points; the 6-point table is the
-// reference's 15-digit one, needed to reproduce its p = 5 operators).
+// A symmetric rule of n values from its half at the -1 end, listed inwards (for odd n the
+// last entry is the middle value): magnitudes, with 'sign' on the -1 side (points -1,
+// weights +1).
+static std::vector<double> mirror(const std::vector<double> &half, unsigned n, double sign) {
+  std::vector<double> r(n);
+  for (unsigned i = 0; i < n / 2; i++) {
+    r[i] = sign * half[i];
+    r[n - 1 - i] = half[i];
+  }
+  if (n % 2) r[n / 2] = half[n / 2];
+  return r;
+}
+
+// src/points.cpp:9-98 (closed forms up to 5 points; for 6..10 points the tables are the
+// reference's 15-digit ones, needed to reproduce its operators at p = 5..9: the 7-point
+// outer node ends in ...758, one unit below the correctly rounded value).
 std::vector<double> gauss_legendre_points(unsigned n) {
   switch (n) {
     case 1: return {0.0};
@@ -36,10 +50,15 @@ std::vector<double> gauss_legendre_points(unsigned n) {
       const double i = (1. / 3.) * std::sqrt(5. - 2. * std::sqrt(10. / 7.));
       return {-o, -i, 0.0, i, o};
     }
-    case 6:
-      return {-0.932469514203152, -0.661209386466264, -0.238619186083197,
-              0.238619186083197,  0.661209386466264,  0.932469514203152};
-    default: raise("Gauss-Legendre points are available for 1..6 points (order 0..5)");
+    case 6: return mirror({0.932469514203152, 0.661209386466264, 0.238619186083197}, n, -1.0);
+    case 7: return mirror({0.949107912342758, 0.741531185599394, 0.405845151377397, 0.0}, n, -1.0);
+    case 8: return mirror({0.960289856497536, 0.796666477413627, 0.525532409916329, 0.183434642495650}, n, -1.0);
+    case 9:
+      return mirror({0.968160239507626, 0.836031107326636, 0.613371432700590, 0.324253423403809, 0.0}, n, -1.0);
+    case 10:
+      return mirror({0.973906528517172, 0.865063366688985, 0.679409568299024, 0.433895394129247,
+                        0.148874338981631}, n, -1.0);
+    default: raise("Gauss-Legendre points are available for 1..10 points (order 0..9)");
   }
 }
 
@@ -57,10 +76,16 @@ std::vector<double> gauss_legendre_weights(unsigned n) {
       const double a = 1. / 900. * (322 - 13. * std::sqrt(70.)), b = 1. / 900. * (322 + 13. * std::sqrt(70.));
       return {a, b, 128. / 225., b, a};
     }
-    case 6:
-      return {0.171324492379171, 0.360761573048139, 0.467913934572691,
-              0.467913934572691, 0.360761573048139, 0.171324492379171};
-    default: raise("Gauss-Legendre weights are available for 1..6 points");
+    case 6: return mirror({0.171324492379171, 0.360761573048139, 0.467913934572691}, n, 1.0);
+    case 7: return mirror({0.129484966168870, 0.279705391489277, 0.381830050505119, 0.417959183673469}, n, 1.0);
+    case 8: return mirror({0.101228536290377, 0.222381034453374, 0.313706645877887, 0.362683783378362}, n, 1.0);
+    case 9:
+      return mirror({0.081274388361575, 0.180648160694857, 0.260610696402936, 0.312347077040003,
+                        0.330239355001260}, n, 1.0);
+    case 10:
+      return mirror({0.066671344308688, 0.149451349150581, 0.219086362515982, 0.269266719309996,
+                        0.295524224714753}, n, 1.0);
+    default: raise("Gauss-Legendre weights are available for 1..10 points");
   }
 }
 
@@ -91,7 +116,10 @@ double lagrange_d1(const std::vector<double> &g, unsigned mode, double x) {
 
 void RefElement::build(int nd, int p) {
   if (nd != 2 && nd != 3) raise("nDims must be 2 or 3");
-  if (p < 0 || p > 5) raise("order must be in 0..5");
+  // quads to order 9 (the reference's Gauss-Legendre tables end at 10 points); hexes to
+  // order 5: the kernels run one thread per solution point in workgroups of at most 256
+  // threads, and the partner map holds a flux point index in 8 bits
+  if (p < 0 || p > max_order(nd)) raise("order must be in 0.." + std::to_string(max_order(nd)) + (nd == 2 ? " for quads" : " for hexes"));
   nDims = nd; order = p; N = p + 1;
   loc1d = gauss_legendre_points(N);
   wts1d = gauss_legendre_weights(N);

@@ -333,7 +333,7 @@ void FRSolver::restart_pyfr(const std::string &restart_case, unsigned restart_it
     const int n1 = (int)std::lround(std::pow((double)nR, 1.0 / D));
     int cube = 1;
     for (int d = 0; d < D; d++) cube *= n1;
-    if (cube != nR || n1 < 1 || n1 > 6) raise("restart file: point count does not match a quad/hex order 0..5");
+    if (cube != nR || n1 < 1 || n1 > 10) raise("restart file: point count does not match a quad/hex order 0..9");
     const auto xr = gauss_legendre_points((unsigned)n1);
     const auto xs = gauss_legendre_points((unsigned)(order + 1));
     const int N = order + 1;

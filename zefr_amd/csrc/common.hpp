@@ -38,6 +38,13 @@ inline void hip_or_throw(hipError_t err, const char *what) {
   if (err != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(err));
 }
 
+// Doubles of the operator-table region a residual kernel stages for elements with N
+// points per direction (lE, Dss, Dsf: N^2 + 4 N, padded to 8).  A fused p-multigrid
+// prolongation reuses that region for its [n_fine][N] table, so it is fused only where
+// n_fine N fits.
+inline int ops_region_doubles(int N) { return (N * N + 4 * N + 7) & ~7; }
+inline bool pmg_prolong_fits(int N, int n_fine) { return n_fine >= N && n_fine <= 10 && n_fine * N <= ops_region_doubles(N); }
+
 // Convert a kernel-layer status into an exception carrying its message.
 inline void status_or_throw(int rc) {
   if (rc != 0) throw Error(last_error());

@@ -60,14 +60,15 @@ __global__ __launch_bounds__(kMaxThreads) void k_rk_final(double *U, const doubl
 // along x, then y, then z, with points x fastest.  A workgroup takes L consecutive lines
 // (one contiguous chunk of in and of out), staged in LDS and ping-ponged between passes.
 constexpr int kPmgBuf = 2048;  // doubles per LDS buffer: L * max(n_in, n_out)^D <= kPmgBuf
-constexpr int kPmgLds = 64 + 2 * kPmgBuf;
+constexpr int kPmgTab = 128;  // doubles for the [n_out][n_in] table (n <= 10)
+constexpr int kPmgLds = kPmgTab + 2 * kPmgBuf;
 
 template <int D>
 __global__ __launch_bounds__(kMaxThreads) void k_pmg_transfer(const double *__restrict__ in, double *__restrict__ out,
                                                               const double *__restrict__ A, long lines, int ni,
                                                               int no, int L, double alpha, double beta) {
   extern __shared__ double lds[];  // kPmgLds doubles
-  double *lA = lds, *buf = lds + 64;
+  double *lA = lds, *buf = lds + kPmgTab;
   const long l0 = (long)blockIdx.x * L;
   const int nl = (int)min((long)L, lines - l0);
   int szi = 1, szo = 1;
@@ -201,6 +202,10 @@ int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipSt
       case 4: return launch<2, 4>(op, d, U, out, st, j);
       case 5: return launch<2, 5>(op, d, U, out, st, j);
       case 6: return launch<2, 6>(op, d, U, out, st, j);
+      case 7: return launch<2, 7>(op, d, U, out, st, j);
+      case 8: return launch<2, 8>(op, d, U, out, st, j);
+      case 9: return launch<2, 9>(op, d, U, out, st, j);
+      case 10: return launch<2, 10>(op, d, U, out, st, j);
     }
   } else if (d->nDims == 3) {
     switch (N) {
@@ -212,7 +217,7 @@ int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipSt
       case 6: return launch<3, 6>(op, d, U, out, st, j);
     }
   }
-  return zfr::fail("kernel launch: supported shapes are quads and hexes with order 0..5");
+  return zfr::fail("kernel launch: supported shapes are quads of order 0..9 and hexes of order 0..5");
 }
 
 template <int D>
@@ -228,6 +233,14 @@ void dispatch_multi_n(const MultiGroup &g, hipStream_t st) {
     case 4: launch_multi<D, 4>(g, st); break;
     case 5: launch_multi<D, 5>(g, st); break;
     case 6: launch_multi<D, 6>(g, st); break;
+  }
+  if constexpr (D == 2) {
+    switch (g.N) {
+      case 7: launch_multi<2, 7>(g, st); break;
+      case 8: launch_multi<2, 8>(g, st); break;
+      case 9: launch_multi<2, 9>(g, st); break;
+      case 10: launch_multi<2, 10>(g, st); break;
+    }
   }
 }
 
@@ -353,8 +366,10 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
   if (prol) {
     if (r->mode != 2 || !r->corr_out) return zfr::fail("RK fusion: the prolongation goes with mode 2 and corr_out");
     if (!r->prolong_out || !r->prolong_A) return zfr::fail("RK fusion: prolongation needs prolong_A and prolong_out");
-    if (r->prolong_n < d->order + 1 || r->prolong_n > 6)
-      return zfr::fail("RK fusion: prolong_n must be in order+1..6");
+    if (!zfr::pmg_prolong_fits(d->order + 1, r->prolong_n))
+      return zfr::fail("RK fusion: prolong_n must be in order+1..10 with a [prolong_n][order+1] table that fits "
+                       "the kernel's operator-table region (prolong_n (order+1) <= (order+1)^2 + 4 (order+1), "
+                       "padded to 8); otherwise prolong with zfr_pmg_transfer");
   }
   const bool update = r->mode == 1 || r->mode == 2;
   if (update && (!r->U || !r->U_ini || (r->mode == 2 && r->nStages > 1 && !r->divF0)))
@@ -475,8 +490,8 @@ int zfr_rk_final(const zfr_ele_desc_t *d, double *U, const double *U_ini, const 
 int zfr_pmg_transfer(int nDims, int nVars, long nEles, int n_in, int n_out, const double *A, double alpha,
                      const double *in, double beta, double *out, void *stream) {
   if (nDims != 2 && nDims != 3) return zfr::fail("zfr_pmg_transfer: nDims must be 2 or 3");
-  if (n_in < 1 || n_in > 6 || n_out < 1 || n_out > 6)
-    return zfr::fail("zfr_pmg_transfer: points per direction must be in 1..6 (orders 0..5)");
+  if (n_in < 1 || n_in > 10 || n_out < 1 || n_out > 10)
+    return zfr::fail("zfr_pmg_transfer: points per direction must be in 1..10 (orders 0..9)");
   if (nVars < 1 || nEles < 0) return zfr::fail("zfr_pmg_transfer: bad sizes");
   if (!A || !in || !out) return zfr::fail("zfr_pmg_transfer: missing buffer");
   const long lines = nEles * nVars;
@@ -659,8 +674,8 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
   for (int i = 0; i < n; i++) {
     const zfr_ele_desc_t *d = &descs[i];
     const int D = d->nDims, N = d->order + 1;
-    if ((D != 2 && D != 3) || N < 1 || N > 6)
-      return zfr::fail("zfr_multi_create: supported shapes are quads and hexes with order 0..5");
+    if ((D != 2 && D != 3) || N < 1 || N > (D == 2 ? 10 : 6))
+      return zfr::fail("zfr_multi_create: supported shapes are quads of order 0..9 and hexes of order 0..5");
     long nS = 1, nFF = 1;
     for (int k = 0; k < D; k++) nS *= N;
     for (int k = 0; k < D - 1; k++) nFF *= N;
@@ -675,13 +690,16 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
       if (!d->Ftr) return zfr::fail("zfr_multi_create: the one-sided path needs Ftr");
       // one launch per phase for every order and element type (N = 0, D = 0: mixed launch;
       // narrowed to one nDims below when the group holds only one)
-      phase[1][{Op::OneSided, 0, 0, 1, bm, fc}].push_back(i);  // extrapolates the partner traces itself
-      phase[2][{Op::FaceCorr, 0, 0, 1, bm, 0}].push_back(i);
+      // (orders above the mixed launches' range: one launch per shape)
+      const bool mixed = N <= zk::kMixedMaxN;
+      phase[1][{Op::OneSided, mixed ? 0 : D, mixed ? 0 : N, 1, bm, fc}].push_back(i);  // extrapolates the partner traces itself
+      phase[2][{Op::FaceCorr, mixed ? 0 : D, mixed ? 0 : N, 1, bm, 0}].push_back(i);
     } else {
       if (visc && !d->Gtr) return zfr::fail("zfr_multi_create: the general LDG path needs Gtr");
       if (visc) phase[0][{Op::Extrap, D, N, 0, 0, 0}].push_back(i);  // Euler: extrapolated in the divergence
       if (visc) phase[1][{Op::Grad, D, N, 1, 0, 0}].push_back(i);
-      phase[2][{Op::Div, visc ? D : 0, visc ? N : 0, visc, bm, fc}].push_back(i);
+      const bool own = visc || N > zk::kMixedMaxN;
+      phase[2][{Op::Div, own ? D : 0, own ? N : 0, visc, bm, fc}].push_back(i);
     }
   }
   auto m = new zfr_multi_s();
@@ -700,11 +718,11 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
       }
       // one tile per order, chosen for that order's total element count in the group, as
       // for one solver; the launch takes the largest LDS of its orders
-      std::map<int, int> eb_of;  // tile per shape, keyed nDims * 8 + N
+      std::map<int, int> eb_of;  // tile per shape, keyed nDims * 16 + N
       g.lds = 0;
       g.EB = 0;
       for (int i : ids) {
-        const int N = descs[i].order + 1, cD = descs[i].nDims, shape = cD * 8 + N;
+        const int N = descs[i].order + 1, cD = descs[i].nDims, shape = cD * 16 + N;
         if (eb_of.count(shape)) continue;
         const size_t per_ele = (cD == 2) ? zk::lds_per_element_n<2>(N, k.op, k.visc != 0)
                                          : zk::lds_per_element_n<3>(N, k.op, k.visc != 0);
@@ -731,7 +749,7 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
       long blocks = 0;
       for (size_t q = 0; q < ids.size(); q++) {
         const int i = ids[q];
-        const int N = descs[i].order + 1, eb = eb_of[descs[i].nDims * 8 + N];
+        const int N = descs[i].order + 1, eb = eb_of[descs[i].nDims * 16 + N];
         hc[q].a = zk::make_args(&descs[i], eb);
         hc[q].a.trace_sel = (k.op == Op::ExtrapPartner) ? 1 : 0;
         hc[q].U = U[i];

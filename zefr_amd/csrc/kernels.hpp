@@ -508,8 +508,8 @@ __device__ __forceinline__ double tp_out(const double *blk, const double *A, int
 // workgroup calls this; it holds barriers).  res: this thread's residual (after the source
 // term); unew: its new state (mode 2); ublk: the element's U block in LDS, or null (then U
 // is read from a.rk.pmgU); w: a free per-element LDS block of at least V nS doubles; lops:
-// the block's operator tables, no longer needed, which take the transfer table (at most
-// 6 N doubles, within the tables' N^2 + 4 N, padded).
+// the block's operator tables, no longer needed, which take the transfer table (the host
+// fuses a transfer only where its table fits their N^2 + 4 N doubles, padded).
 template <int D, int N>
 __device__ __forceinline__ void pmg_epilogue(const KArgs &a, double *lops, const double *ublk, double *w, long e,
                                              int t, bool valid, const double *res, const double *unew) {
@@ -1522,6 +1522,14 @@ size_t lds_per_element_n(int N, Op op, bool viscous) {
     case 5: return lds_per_element<D, 5>(op, viscous);
     case 6: return lds_per_element<D, 6>(op, viscous);
   }
+  if constexpr (D == 2) {
+    switch (N) {
+      case 7: return lds_per_element<2, 7>(op, viscous);
+      case 8: return lds_per_element<2, 8>(op, viscous);
+      case 9: return lds_per_element<2, 9>(op, viscous);
+      case 10: return lds_per_element<2, 10>(op, viscous);
+    }
+  }
   return 0;
 }
 
@@ -1535,5 +1543,8 @@ template <int D>
 void launch_mixed(const MultiGroup &g, hipStream_t st);
 // The ops a mixed-order launch covers (the Euler divergence and the one-sided LDG pair).
 inline bool mixed_op(Op op, int visc) { return (op == Op::Div && !visc) || op == Op::OneSided || op == Op::FaceCorr; }
+// Orders a mixed-order launch compiles in (points per direction 1..6); quads of higher
+// order get launches of their own shape.
+constexpr int kMixedMaxN = 6;
 
 }  // namespace zk

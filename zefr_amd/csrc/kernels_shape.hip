@@ -1,6 +1,7 @@
 // kernels_shape.hip -- the kernel launches of one element shape: compiled once per
-// (nDims, points per direction) with -DZK_D=<2|3> -DZK_N=<1..6>, so the shapes build in
-// parallel and a change to one shape's code rebuilds one object.
+// (nDims, points per direction) with -DZK_D=<2|3> -DZK_N=<1..10 for quads, 1..6 for
+// hexes>, so the shapes build in parallel and a change to one shape's code rebuilds one
+// object.
 #include "kernels.hpp"
 
 #if !defined(ZK_D) || !defined(ZK_N)

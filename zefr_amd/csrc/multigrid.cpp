@@ -49,7 +49,7 @@ std::vector<double> matmul(const std::vector<double> &A, const std::vector<doubl
 // ((I[r<-r+1] I[r+1<-r+2]) I[r+2<-r+3]) ... I[p-1<-p], the association of
 // src/quads.cpp:416-457.  The composition is not the direct interpolation from p to r.
 std::vector<double> pmg_restriction_1d(int p, int r) {
-  if (r < 0 || r >= p || p > 5) raise("p-multigrid restriction needs 0 <= coarse order < fine order <= 5");
+  if (r < 0 || r >= p || p > 9) raise("p-multigrid restriction needs 0 <= coarse order < fine order <= 9");
   std::vector<double> R = interp_1d(r + 2, r + 1);
   for (int P = r + 1; P < p; P++) R = matmul(R, interp_1d(P + 2, P + 1), r + 1, P + 1, P + 2);
   return R;  // [r+1][p+1]
@@ -59,7 +59,7 @@ std::vector<double> pmg_restriction_1d(int p, int r) {
 // (src/quads.cpp:373-414).  Interpolation reproduces polynomials, so this equals the direct
 // interpolation from p to q up to rounding.
 std::vector<double> pmg_prolongation_1d(int p, int q) {
-  if (p < 0 || q <= p || q > 5) raise("p-multigrid prolongation needs 0 <= coarse order < fine order <= 5");
+  if (p < 0 || q <= p || q > 9) raise("p-multigrid prolongation needs 0 <= coarse order < fine order <= 9");
   std::vector<double> Pm = interp_1d(q, q + 1);
   for (int P = q - 1; P > p; P--) Pm = matmul(Pm, interp_1d(P, P + 1), q + 1, P + 1, P);
   return Pm;  // [q+1][p+1]
@@ -186,6 +186,11 @@ void PMGrid::v_cycle(FRSolver &solver, int lvl) {
       restrict_pmg(g, gc, Rc);
     }
   };
+  // the correction of level n is prolonged by its last step's final combination when the
+  // step is fused and the table fits the kernel's operator-table region
+  auto fused_prolong = [&](int n) {
+    return grids[n]->fuse_rk && zfr::pmg_prolong_fits(grids[n]->order + 1, level_at(n - 1).order + 1);
+  };
   auto prolong = [&](int n) -> FRSolver::Prolong {
     FRSolver &gf = level_at(n - 1);
     FRSolver::Prolong p;
@@ -204,7 +209,7 @@ void PMGrid::v_cycle(FRSolver &solver, int lvl) {
     for (unsigned s = 0; s < steps; s++) {
       const bool last = finish && s + 1 == steps;
       g.update(src, (store && s == 0) ? sol : nullptr, last ? corr : nullptr, last ? sol : nullptr,
-               (last && g.fuse_rk) ? prolong(n) : FRSolver::Prolong{});
+               (last && fused_prolong(n)) ? prolong(n) : FRSolver::Prolong{});
     }
   };
   // the correction of a level without smoothing steps, and the unfused prolongation
@@ -214,7 +219,7 @@ void PMGrid::v_cycle(FRSolver &solver, int lvl) {
     if (!stepped)
       status_or_throw(zfr_axpby((long)g.eles->state_size(), 1.0, g.eles->U_spts_d.as<double>(), -1.0, sol, corr,
                                 g.stream));
-    if (!stepped || !g.fuse_rk) prolong_err(g, corr, level_at(n - 1));
+    if (!stepped || !fused_prolong(n)) prolong_err(g, corr, level_at(n - 1));
   };
 
   if (lvl != nLevels - 1) restrict_from(lvl, nullptr);
