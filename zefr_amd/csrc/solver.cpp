@@ -753,16 +753,21 @@ void FRSolver::report_residuals(double *res) {
 
 // Time the residual with each feasible tile and keep the fastest.
 // Workgroup tiles by measurement: every candidate (1, 2, 4, ... elements per workgroup up
-// to 256 threads, and the kernels' default) is timed per launch with time_kernels, twice
-// in alternation; each of the residual's three launches keeps its own fastest tile.
+// to 256 threads, and the largest tile of 1..4 full waves) is timed per launch with
+// time_kernels, twice in alternation; each of the residual's three launches keeps its own
+// fastest tile.
 // Returns the tile of the middle (dominant) launch.
 int FRSolver::autotune(int reps) {
   need_device(*this);
   Elements &el = *eles;
   std::vector<int> cands;
   for (int eb = 1; eb * el.nSpts <= 256; eb *= 2) cands.push_back(eb);
-  const int def = std::max(1, 256 / el.nSpts);
-  if (std::find(cands.begin(), cands.end(), def) == cands.end()) cands.push_back(def);
+  // and the largest tile that fits 1, 2, 3, 4 waves (e.g. 2, 5, 7, 10 elements of 25
+  // points), which leave the fewest lanes idle
+  for (int w = 1; w <= 4; w++) {
+    const int eb = 64 * w / el.nSpts;
+    if (eb >= 1 && std::find(cands.begin(), cands.end(), eb) == cands.end()) cands.push_back(eb);
+  }
   el.tile_phase[0] = el.tile_phase[1] = el.tile_phase[2] = 0;
   double best[3] = {1e300, 1e300, 1e300};
   int best_eb[3] = {el.tile, el.tile, el.tile};
