@@ -362,6 +362,17 @@ output_prefix pmg
         z.do_step()
     z.synchronize()
     cyc_graph_ms = (time.perf_counter() - t) / cycles * 1e3
+    # the same cycle as one persistent (cooperative) launch
+    z.set_step_mode(True, False)
+    z.set_pmg_persistent(True)
+    z.do_step()
+    z.synchronize()
+    t = time.perf_counter()
+    for _ in range(cycles):
+        z.do_step()
+    z.synchronize()
+    cyc_persist_ms = (time.perf_counter() - t) / cycles * 1e3
+    z.set_pmg_persistent(False)
     # one fine RK44 step on the same mesh (a solver without multigrid)
     z2 = zefr.Zefr()
     z2.read_input(deck)
@@ -398,6 +409,7 @@ output_prefix pmg
     return {"workload": "p-multigrid V-cycle, 128^2 periodic quads on [0,1]^2, p=4, Euler, levels 4-3-2-1-0, "
                         "one RK44 step per level",
             "vcycle_ms": round(cyc_ms, 4), "vcycle_graph_ms": round(cyc_graph_ms, 4),
+            "vcycle_persistent_ms": round(cyc_persist_ms, 4),
             "fine_rk44_step_ms": round(step_ms, 4),
             "vcycle_over_fine_step": round(cyc_ms / step_ms, 3),
             "restrict_kernel": {"name": "k_pmg_transfer<2>", "avg_ms": round(kms, 5), "bytes": kbytes,

@@ -178,6 +178,14 @@ int zefr_get_tiles(zefr_t *z, int *tiles);
  * level, and use_graph captures a whole V-cycle (all levels, transfers and updates) as one
  * graph instead.  Results are identical either way. */
 int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph);
+/* p-multigrid V-cycles as one persistent launch (new; no reference counterpart; built on
+ * zfr_persist_*): with on != 0 and p_multi set, do_step records the V-cycle of
+ * PMGrid::cycle (src/multigrid.cpp:263-333) once and replays it as one cooperative launch
+ * (recorded again when dt, rel_fac, a tile or the fusion mode change).  For inviscid runs
+ * with fused steps, at least one smoothing step on every coarse level and every
+ * correction's prolongation fused into its level's last step; otherwise do_step fails
+ * with the reason.  Results are identical to the eager and graph cycles. */
+int zefr_set_pmg_persistent(zefr_t *z, int on);
 /* Run `reps` residuals (stage 0) with HIP events around each kernel on the solver's
  * stream; ms_out[0..2] = average duration of the extrapolate / gradient / divergence
  * launch (gradient is 0 for inviscid runs).  No counterpart in the reference: this is the
@@ -415,6 +423,29 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
 int zfr_multi_residual(zfr_multi_t *plan, void *stream);
 int zfr_multi_launches(const zfr_multi_t *plan);
 void zfr_multi_destroy(zfr_multi_t *plan);
+/* Persistent plans (new; no reference counterpart; they replace the launch sequence of
+ * PMGrid::v_cycle, src/multigrid.cpp:105-261, for inviscid runs): a recorded sequence of
+ * Euler divergence launches (residuals, fused RK stages, source terms and the fused
+ * p-multigrid transfers) replayed by ONE cooperative launch.  Its workgroups, all
+ * resident, walk each recorded launch's element blocks and meet at a grid-wide barrier
+ * before the next one, so results equal the recorded launches' bit for bit.
+ * zfr_persist_record_begin starts recording on the calling thread: zfr_divergence and
+ * zfr_divergence_rk with inviscid descriptors (order 0..5, one nDims) are then appended
+ * instead of launched, and every other launching entry point fails, until
+ * zfr_persist_record_end, which builds the plan (hipMalloc and a blocking copy of the
+ * phase table; plan NULL discards the recording).  The plan keeps the recorded pointers
+ * and scalars: replay it only while those buffers live and hold the same meaning.
+ * zfr_persist_info returns the number of recorded launches and the workgroups of the
+ * cooperative launch (by default one per CU, at most the largest recorded launch's
+ * workgroups); zfr_persist_set_workgroups sets it, up to the workgroups that can be
+ * resident at once. */
+typedef struct zfr_persist_s zfr_persist_t;
+int zfr_persist_record_begin(void);
+int zfr_persist_record_end(zfr_persist_t **plan);
+int zfr_persist_launch(const zfr_persist_t *plan, void *stream);
+int zfr_persist_info(const zfr_persist_t *plan, int *phases, int *workgroups);
+int zfr_persist_set_workgroups(zfr_persist_t *plan, int n);
+void zfr_persist_destroy(zfr_persist_t *plan);
 /* Plain device copy dst[i] = src[i], i < n (bandwidth calibration for the benchmark; no
  * counterpart in the reference). */
 int zfr_stream_copy(double *dst, const double *src, long n, void *stream);
@@ -423,7 +454,7 @@ int zfr_stream_copy(double *dst, const double *src, long n, void *stream);
  * dense [nSpts_out][nSpts_in] operator oppRes / oppPro (src/quads.cpp:363-465,
  * src/hexas.cpp:586-704).  That operator is the tensor product of a 1D table, so this takes
  * the 1D table A [n_out][n_in] (row-major, device memory; n = order + 1 points per
- * direction, 1..6) and computes, per element and variable,
+ * direction, 1..10) and computes, per element and variable,
  * out = alpha * (A x A [x A]) in + beta * out, in [nEles][nVars][n_in^nDims] ->
  * [nEles][nVars][n_out^nDims].  beta = 0 does not read out. */
 int zfr_pmg_transfer(int nDims, int nVars, long nEles, int n_in, int n_out, const double *A, double alpha,

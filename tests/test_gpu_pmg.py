@@ -9,6 +9,7 @@
   the oracle (1e-12 norm-wise per variable), and every coarse level's source term, stored
   solution, correction and state.
 * The RK smoothing under a source term: fused and separate update kernels bit-identical.
+* The V-cycle as one persistent (cooperative) launch: bit-identical to the eager cycle.
 """
 
 from __future__ import annotations
@@ -230,3 +231,49 @@ def test_vcycle_graph_matches_eager():
             for which in ("U", "source", "correction"):
                 assert np.array_equal(zg.pmg_state(lv, which), ze.pmg_state(lv, which)), (it, lv, which)
         assert zg.get_input().iter == ze.get_input().iter
+
+
+PERSIST_CASES = [
+    _pmg_case("pmg_persist_quad", (12, 10), (0.0, 0.0), (1.0, 1.0), 4, False, "noisy:7", [4, 3, 2, 1, 0],
+              [1, 1, 1, 1, 2], dt=2e-4),
+    _pmg_case("pmg_persist_hex_roe", (3, 3, 3), (0.0,) * 3, (1.0,) * 3, 3, False, "noisy:3", [3, 1, 0], [2, 1, 2],
+              dt=2e-4, rel_fac=0.9),
+]
+PERSIST_CASES[1]["extra"]["fconv_type"] = "Roe"
+
+
+@pytest.mark.parametrize("case", PERSIST_CASES, ids=[c["name"] for c in PERSIST_CASES])
+def test_persistent_vcycle_matches_eager(case):
+    """The V-cycle recorded once and replayed as one cooperative launch (every level's
+    residuals, RK stages, source terms and fused transfers, a grid-wide barrier between
+    them) gives the eager cycle's results bit for bit on every level, cycle after cycle,
+    and advances the iteration counters as the eager cycle does; it also matches the
+    oracle."""
+    print("[pmg]", case["name"], flush=True)
+    zp, o, p = _paired_pmg(case)
+    ze, _, _ = paired(dict(case, name=case["name"] + "_eager"))
+    zp.set_pmg_persistent(True)
+    for it in range(3):
+        zp.do_step()
+        ze.do_step()
+        p.cycle(1)
+        assert np.array_equal(zp.state("U"), ze.state("U")), it
+        for lv in range(1, len(case["levels"])):
+            for which in ("U", "source", "solution", "correction"):
+                assert np.array_equal(zp.pmg_state(lv, which), ze.pmg_state(lv, which)), (it, lv, which)
+        assert zp.get_input().iter == ze.get_input().iter
+        _check_cycle(zp, o, p, case, it)
+
+
+def test_persistent_vcycle_refuses_what_it_cannot_hold():
+    """A viscous V-cycle has launches other than the Euler divergence: do_step fails with
+    the reason before anything runs, and the eager cycle still works afterwards."""
+    case = _pmg_case("pmg_persist_ns", (4, 4), (0.0, 0.0), (TWO_PI, TWO_PI), 3, True, "smooth", [3, 1, 0],
+                     [1, 1, 1])
+    z, _, _ = paired(case)
+    z.set_pmg_persistent(True)
+    with pytest.raises(RuntimeError, match="inviscid runs only"):
+        z.do_step()
+    z.set_pmg_persistent(False)
+    z.do_step()
+    assert np.isfinite(z.state("U")).all()

@@ -115,6 +115,7 @@ SIGNATURES = {
     "zefr_set_tiles": (_I, [_P, _IP]),
     "zefr_get_tiles": (_I, [_P, _IP]),
     "zefr_set_step_mode": (_I, [_P, _I, _I]),
+    "zefr_set_pmg_persistent": (_I, [_P, _I]),
     "zefr_time_residual_kernels": (_I, [_P, _I, _DP]),
     "zefr_residual_at": (_I, [_P, _P, _P, _P]),
     "zefr_residual_vjp": (_I, [_P, _P, _P, _P, _P]),
@@ -155,6 +156,12 @@ SIGNATURES = {
     "zfr_multi_residual": (_I, [_P, _P]),
     "zfr_multi_launches": (_I, [_P]),
     "zfr_multi_destroy": (None, [_P]),
+    "zfr_persist_record_begin": (_I, []),
+    "zfr_persist_record_end": (_I, [C.POINTER(_P)]),
+    "zfr_persist_launch": (_I, [_P, _P]),
+    "zfr_persist_info": (_I, [_P, _IP, _IP]),
+    "zfr_persist_set_workgroups": (_I, [_P, _I]),
+    "zfr_persist_destroy": (None, [_P]),
     "zfr_rk_stage_source": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, C.c_double, C.c_double, _P]),
     "zfr_rk_final_source": (_I, [C.POINTER(EleDesc), _P, _P, _P, _P, _DP, _I, C.c_double, _P]),
     "zfr_pmg_transfer": (_I, [_I, _I, C.c_long, _I, _I, _P, C.c_double, _P, C.c_double, _P, _P]),

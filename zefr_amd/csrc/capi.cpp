@@ -418,6 +418,13 @@ int zefr_set_step_mode(zefr_t *z, int fuse_rk, int use_graph) {
   });
 }
 
+int zefr_set_pmg_persistent(zefr_t *z, int on) {
+  return zfr::guard([&] {
+    if (!z->pmg) zfr::raise("set_pmg_persistent: p-multigrid is not set up (p_multi 0)");
+    z->pmg->set_persistent(on != 0);
+  });
+}
+
 // ---------------------------------------------------------------- batches
 // Residuals of independent solvers: a multi-case plan over their element blocks (one
 // launch per kernel, shape and flux branch; zfr_multi_create), captured once as a HIP

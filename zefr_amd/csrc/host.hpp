@@ -282,9 +282,23 @@ class PMGrid {
   // V-cycles replayed from one captured HIP graph (every level, transfer and update of
   // the cycle); recaptured when dt, rel_fac, a tile or the fusion mode change
   bool use_graph = false;
+  // V-cycles replayed as one persistent launch (zfr_persist_*); recorded like the graph
+  void set_persistent(bool on);
+  bool persistent() const { return use_persistent; }
   ~PMGrid();
 
  private:
+  bool use_persistent = false;
+  zfr_persist_t *persist = nullptr;
+  std::vector<double> persist_key;
+  void check_persistent(FRSolver &solver);
+  // run_v between begin() and end() (a stream capture or a plan recording: nothing runs),
+  // abort() instead of end() if it throws; each level's iteration counter and time are
+  // left as they were, and the updates of each level in one cycle are returned
+  std::vector<int> record_cycle(FRSolver &solver, const std::function<void()> &begin,
+                                const std::function<void()> &abort, const std::function<void()> &end);
+  void advance_counters(FRSolver &solver, const std::vector<int> &iters);
+  std::vector<int> persist_iters;
   FRSolver &level(FRSolver &fine, int n) { return n == 0 ? fine : *grids[n]; }
   void run_v(FRSolver &solver);
   void drop_graph();

@@ -1547,4 +1547,33 @@ inline bool mixed_op(Op op, int visc) { return (op == Op::Div && !visc) || op ==
 // order get launches of their own shape.
 constexpr int kMixedMaxN = 6;
 
+// ---------------------------------------------------------------- persistent plans
+// One phase of a persistent plan: an Euler divergence launch (a residual, or a fused RK
+// stage / source-term / restriction epilogue around one), recorded instead of launched.
+// k_persist runs every phase of a plan in one cooperative launch: its resident workgroups
+// walk the phase's element blocks (block b as the single launch's workgroup b) and meet
+// at a grid-wide barrier before the next phase.
+struct PersistPhase {
+  KArgs a;
+  const double *U;
+  double *out;
+  long nblk;     // element blocks of the phase (the single launch's grid)
+  int N, D, fc, self;  // points per direction, nDims, Roe, partner traces extrapolated from U
+};
+// While set on a thread, the Euler divergence launches of that thread are appended here
+// and every other launch entry point fails (a plan holds nothing else).
+struct PersistRecorder {
+  std::vector<PersistPhase> phases;
+  size_t lds = 0;  // the largest LDS of the phases
+  int D = 0;
+};
+extern thread_local PersistRecorder *g_persist;
+// 0, or a failure status naming the launch that cannot be recorded.
+int refuse_while_recording(const char *what);
+// Defined and instantiated in kernels_mixed.hip (D = 2, 3).
+template <int D>
+hipError_t launch_persist(const PersistPhase *phases, int nph, unsigned grid, size_t lds, hipStream_t st);
+template <int D>
+hipError_t persist_blocks_per_cu(size_t lds, int *n);
+
 }  // namespace zk

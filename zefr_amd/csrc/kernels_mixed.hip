@@ -7,6 +7,8 @@
 // its EB * nS idle.  The launch takes the largest LDS any of its cases needs; the register
 // budget is the largest of the shapes compiled in, which is why single-nDims groups keep
 // their own kernel.
+#include <hip/hip_cooperative_groups.h>
+
 #include "kernels.hpp"
 
 #if !defined(ZK_D)
@@ -57,6 +59,45 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction_mixed(const Mul
   const long b = (long)blockIdx.x - starts[c];
   ZK_SWITCH_N((face_correction_block<D, N, BM>(m.out, m.a, b)))
 }
+#if ZK_D != 0
+// A persistent plan (zfr_persist_*): every phase's element blocks walked by the resident
+// workgroups, one grid-wide barrier between phases (cooperative launch: all workgroups are
+// resident, so every barrier is reached by all of them).  Each block runs the Euler
+// divergence of the single launch's workgroup b with the phase's own tile; threads
+// beyond its EB * nS idle.
+// Each variant is a function of its own (not inlined): the registers of one variant are
+// allocated on their own, so the kernel needs the largest variant's, not those of all of
+// them inlined into one body (256 VGPRs and 76 AGPRs for the quad kernel).
+template <int D, int N, int FC, bool SELF>
+__device__ __attribute__((noinline)) void persist_div(const PersistPhase *m, long bid) {
+  divergence_block<D, N, false, 0, FC, SELF>(m->U, m->out, m->a, bid);
+}
+
+template <int D, int N>
+__device__ __forceinline__ void persist_block(const PersistPhase &m, long bid) {
+  if (m.fc) {
+    if (m.self) persist_div<D, N, 1, true>(&m, bid);
+    else persist_div<D, N, 1, false>(&m, bid);
+  } else {
+    if (m.self) persist_div<D, N, 0, true>(&m, bid);
+    else persist_div<D, N, 0, false>(&m, bid);
+  }
+}
+
+template <int DIMS>
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(3)))
+void k_persist(const PersistPhase *__restrict__ ph, int nph) {
+  for (int i = 0; i < nph; i++) {
+    const PersistPhase &m = ph[i];
+    for (long b = blockIdx.x; b < m.nblk; b += gridDim.x) {
+      const long bid = xcd_block(b, m.nblk);
+      ZK_SWITCH_N((persist_block<D, N>(m, bid)))
+      __syncthreads();  // the block's LDS is reused by the next one
+    }
+    if (i + 1 < nph) cooperative_groups::this_grid().sync();
+  }
+}
+#endif
 #undef ZK_SWITCH_N
 #undef ZK_SWITCH_N_OF
 
@@ -87,5 +128,22 @@ void launch_mixed(const MultiGroup &g, hipStream_t st) {
 }
 
 template void launch_mixed<ZK_D>(const MultiGroup &, hipStream_t);
+
+#if ZK_D != 0
+template <int D>
+hipError_t launch_persist(const PersistPhase *phases, int nph, unsigned grid, size_t lds, hipStream_t st) {
+  void *args[] = {(void *)&phases, (void *)&nph};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_persist<D>), dim3(grid), dim3(kMaxThreads),
+                                    args, (unsigned)lds, st);
+}
+
+template <int D>
+hipError_t persist_blocks_per_cu(size_t lds, int *n) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_persist<D>, kMaxThreads, lds);
+}
+
+template hipError_t launch_persist<ZK_D>(const PersistPhase *, int, unsigned, size_t, hipStream_t);
+template hipError_t persist_blocks_per_cu<ZK_D>(size_t, int *);
+#endif
 
 }  // namespace zk

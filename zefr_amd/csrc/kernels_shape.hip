@@ -71,6 +71,18 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   }
   const dim3 grid((unsigned)((d->nEles + EB - 1) / EB)), block((unsigned)(EB * S::nS));
   const size_t lds = sizeof(double) * (S::NOPS_PAD + EB * per_ele);
+  if (g_persist) {  // recording a persistent plan: append the phase instead of launching
+    if (op != Op::Div || d->viscous)
+      return zfr::fail("persistent plan: only Euler divergence launches (residuals and fused RK stages of an "
+                       "inviscid run) can be recorded");
+    if (N > kMixedMaxN) return zfr::fail("persistent plan: the persistent kernel covers orders 0..5");
+    if (g_persist->D != 0 && g_persist->D != D) return zfr::fail("persistent plan: one nDims per plan");
+    g_persist->phases.push_back({a, U, out, (long)grid.x, N, D, d->fconv_type == 2 ? 1 : 0,
+                                 rk_writes_input(a.rk, U) ? 0 : 1});
+    g_persist->D = D;
+    if (lds > g_persist->lds) g_persist->lds = lds;
+    return 0;
+  }
   switch (op) {
     case Op::Extrap:
     case Op::ExtrapPartner:

@@ -244,13 +244,23 @@ void PMGrid::v_cycle(FRSolver &solver, int lvl) {
   }
 }
 
-PMGrid::~PMGrid() { drop_graph(); }
+PMGrid::~PMGrid() {
+  drop_graph();
+  if (persist) zfr_persist_destroy(persist);
+}
 
 void PMGrid::drop_graph() {
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
   graph_exec = nullptr;
   graph = nullptr;
+}
+
+void PMGrid::set_persistent(bool on) {
+  use_persistent = on;
+  if (persist) zfr_persist_destroy(persist);
+  persist = nullptr;
+  persist_key.clear();
 }
 
 std::vector<double> PMGrid::graph_key(const FRSolver &solver) const {
@@ -262,6 +272,70 @@ std::vector<double> PMGrid::graph_key(const FRSolver &solver) const {
   }
   for (int n = 0; n < nLevels; n++) k.push_back((double)input->mg_steps[n]);
   return k;
+}
+
+std::vector<int> PMGrid::record_cycle(FRSolver &solver, const std::function<void()> &begin,
+                                      const std::function<void()> &abort, const std::function<void()> &end) {
+  std::vector<FRSolver *> lv = {&solver};
+  for (int n = 1; n < nLevels; n++) lv.push_back(grids[n].get());
+  std::vector<bool> saved;
+  std::vector<int> before;
+  for (auto *g : lv) {
+    saved.push_back(g->use_graph);
+    g->use_graph = false;  // the levels' own step graphs are not nested into the recording
+    before.push_back(g->current_iter);
+  }
+  auto restore = [&] {
+    for (size_t i = 0; i < lv.size(); i++) lv[i]->use_graph = saved[i];
+  };
+  begin();
+  try {
+    run_v(solver);
+  } catch (...) {
+    abort();
+    restore();
+    throw;
+  }
+  try {
+    end();
+  } catch (...) {
+    restore();
+    throw;
+  }
+  std::vector<int> iters;
+  for (size_t i = 0; i < lv.size(); i++) {
+    lv[i]->use_graph = saved[i];
+    iters.push_back(lv[i]->current_iter - before[i]);
+    lv[i]->current_iter = before[i];  // the recording ran nothing; each replay advances them
+    lv[i]->flow_time -= iters.back() * input->dt;
+  }
+  return iters;
+}
+
+void PMGrid::advance_counters(FRSolver &solver, const std::vector<int> &iters) {
+  for (int n = 0; n < nLevels; n++) {
+    FRSolver &g = level(solver, n);
+    g.prev_time = g.flow_time;
+    g.current_iter += iters[n];
+    g.flow_time += iters[n] * input->dt;
+  }
+}
+
+// What a V-cycle must be to run as one persistent launch: only Euler divergence launches
+// (inviscid levels, fused steps, no level without smoothing steps, every prolongation
+// fused).  Anything else would need a launch the plan cannot hold.
+void PMGrid::check_persistent(FRSolver &solver) {
+  if (input->viscous)
+    raise("persistent V-cycle: inviscid runs only (a viscous residual is two or three kernels of other kinds)");
+  if (input->nStages != 4) raise("persistent V-cycle: RK44 steps only");
+  for (int n = 0; n < nLevels; n++) {
+    FRSolver &g = level(solver, n);
+    if (!g.fuse_rk) raise("persistent V-cycle: needs fused steps (set_step_mode fuse_rk)");
+    if (n > 0 && input->mg_steps[n] == 0) raise("persistent V-cycle: every coarse level needs a smoothing step");
+    if (n > 0 && !zfr::pmg_prolong_fits(g.order + 1, level(solver, n - 1).order + 1))
+      raise("persistent V-cycle: the prolongation from order " + std::to_string(g.order) + " to " +
+            std::to_string(level(solver, n - 1).order) + " is not fused into the step (its table is too large)");
+  }
 }
 
 // One V-cycle: mg_steps[0] fine steps, then v_cycle from the fine level.
@@ -277,47 +351,44 @@ void PMGrid::run_v(FRSolver &solver) {
 void PMGrid::cycle(FRSolver &solver) {
   if (!solver.device_ready) raise("solver is not set up on the device (call setup_solver)");
   const std::string type = input->mg_cycle;
+  if (type == "V" && use_persistent) {
+    const auto k = graph_key(solver);
+    if (!persist || k != persist_key) {
+      check_persistent(solver);
+      if (persist) zfr_persist_destroy(persist);
+      persist = nullptr;
+      zfr_persist_t *p = nullptr;
+      persist_iters = record_cycle(
+          solver, [&] { status_or_throw(zfr_persist_record_begin()); },
+          [&] { (void)zfr_persist_record_end(nullptr); }, [&] { status_or_throw(zfr_persist_record_end(&p)); });
+      persist = p;
+      persist_key = k;
+    }
+    status_or_throw(zfr_persist_launch(persist, solver.stream));
+    advance_counters(solver, persist_iters);
+    return;
+  }
   if (type == "V" && use_graph) {
     const auto k = graph_key(solver);
     if (!graph_exec || k != key) {
       drop_graph();
-      std::vector<FRSolver *> lv = {&solver};
-      for (int n = 1; n < nLevels; n++) lv.push_back(grids[n].get());
-      std::vector<bool> saved;
-      std::vector<int> before;
-      for (auto *g : lv) {
-        saved.push_back(g->use_graph);
-        g->use_graph = false;  // the levels' own step graphs are not nested into this one
-        before.push_back(g->current_iter);
-      }
-      hip_or_throw(hipStreamBeginCapture(solver.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-      try {
-        run_v(solver);
-      } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(solver.stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        for (size_t i = 0; i < lv.size(); i++) lv[i]->use_graph = saved[i];
-        throw;
-      }
-      hip_or_throw(hipStreamEndCapture(solver.stream, &graph), "hipStreamEndCapture");
-      iters_per_cycle.clear();
-      for (size_t i = 0; i < lv.size(); i++) {
-        lv[i]->use_graph = saved[i];
-        iters_per_cycle.push_back(lv[i]->current_iter - before[i]);
-        lv[i]->current_iter = before[i];  // the capture ran nothing; the replay below does
-        lv[i]->flow_time -= iters_per_cycle.back() * input->dt;
-      }
+      iters_per_cycle = record_cycle(
+          solver,
+          [&] {
+            hip_or_throw(hipStreamBeginCapture(solver.stream, hipStreamCaptureModeThreadLocal),
+                         "hipStreamBeginCapture");
+          },
+          [&] {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(solver.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+          },
+          [&] { hip_or_throw(hipStreamEndCapture(solver.stream, &graph), "hipStreamEndCapture"); });
       hip_or_throw(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
       key = k;
     }
     hip_or_throw(hipGraphLaunch(graph_exec, solver.stream), "hipGraphLaunch");
-    for (int n = 0; n < nLevels; n++) {
-      FRSolver &g = level(solver, n);
-      g.prev_time = g.flow_time;
-      g.current_iter += iters_per_cycle[n];
-      g.flow_time += iters_per_cycle[n] * input->dt;
-    }
+    advance_counters(solver, iters_per_cycle);
     return;
   }
   if (type == "V") {

@@ -231,6 +231,10 @@ class Zefr:
         """RK update fused into each residual's last kernel; whole step as a HIP graph."""
         check(lib().zefr_set_step_mode(self._h, int(fuse_rk), int(use_graph)))
 
+    def set_pmg_persistent(self, on=True):
+        """p-multigrid V-cycles as one persistent (cooperative) launch; inviscid runs."""
+        check(lib().zefr_set_pmg_persistent(self._h, int(on)))
+
     # -- functional residual and its vector-Jacobian product on device tensors
     def state_tensor(self):
         """A device tensor copy of the current state in the device layout."""
