@@ -1242,7 +1242,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
         if (owner == 0) ldg_common_add<D, BM>(Uo, Up, G, G, a, Fo);
         else ldg_common_add<D, BM>(Up, Uo, G, G, a, Fo);
 #pragma unroll
-        for (int v = 0; v < V; v++) a.Ftr[(e * V + v) * nF + f] = Fo[v];
+        for (int v = 0; v < V; v++) __builtin_nontemporal_store(Fo[v], &a.Ftr[(e * V + v) * nF + f]);
         continue;
       }
       double G[V];
@@ -1267,7 +1267,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       if (owner == 0) common_flux<D, true, BM, FC>(Uo, Up, G, G, n, 1.0, a, Fo);
       else common_flux<D, true, BM, FC>(Up, Uo, G, G, n, 1.0, a, Fo);
 #pragma unroll
-      for (int v = 0; v < V; v++) a.Ftr[(e * V + v) * nF + f] = Fo[v];
+      for (int v = 0; v < V; v++) __builtin_nontemporal_store(Fo[v], &a.Ftr[(e * V + v) * nF + f]);
     }
   }
   __syncthreads();  // the gradient field is consumed
@@ -1316,8 +1316,11 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       acc[v] = s;
     }
   }
+  // streaming stores for the outputs (divF, Ftr): the next kernel reads them, this one
+  // does not, and they should not displace the partner lines other workgroups are about to
+  // read from L2 (-0.8% per residual, measured A/B)
 #pragma unroll
-  for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+  for (int v = 0; v < V; v++) __builtin_nontemporal_store(acc[v], &divF[(e * V + v) * nS + t]);
 }
 
 template <int D, int N, int BM, int FC, bool SELF>
@@ -1401,7 +1404,7 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
       for (int v = 0; v < V; v++) acc[v] += a.rk.src[(e * V + v) * nS + t];
     }
 #pragma unroll
-    for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
+    for (int v = 0; v < V; v++) __builtin_nontemporal_store(acc[v], &divF[(e * V + v) * nS + t]);
 #pragma unroll
     for (int v = 0; v < V; v++) res[v] = acc[v];
     rk_epilogue<D>(a, e, t, nS, acc, unew);
