@@ -111,12 +111,42 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
   unsigned *lmask = reinterpret_cast<unsigned *>(lTo + EB * L::TO);  // owned faces per element
   const long e0 = bid * EB;
   const long neb = min((long)EB, (long)a.E - e0);
-  stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
-  if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
-  __syncthreads();
   const int tid = threadIdx.x, el = tid / nS, t = tid - el * nS;
   const long e = e0 + el;
   const bool valid = el < neb;
+  // the staging loads, this thread's neighbour entries and (viscous) the partner's solution
+  // line at each point this element owns are all issued before any is consumed: two
+  // dependent memory round trips instead of one per flux point after the staging's
+  StageRegs<D, N> sr;
+  stage_U_load<D, N>(U, a, e0, neb, sr);
+  unsigned nb[S::FPT];
+#pragma unroll
+  for (int k = 0; k < S::FPT; k++) {
+    const int f = t + k * nS;
+    nb[k] = (valid && f < nF) ? a.nbr[e * nF + f] : 0u;
+  }
+  PartnerLine<D, N> pl[VISC ? S::FPT : 1];
+  double cp[VISC ? S::FPT : 1][V];
+  bool own[S::FPT];
+#pragma unroll
+  for (int k = 0; k < S::FPT; k++) {
+    const int f = t + k * nS;
+    own[k] = false;
+    if (valid && f < nF) {
+      long ep; int sl, fp;
+      unpack_nbr(nb[k], ep, sl, fp);
+      own[k] = sl == owner;
+      if (VISC && own[k]) partner_line_load<D, N>(U, a.ops, ep, fp, pl[VISC ? k : 0]);
+    }
+  }
+  if (VISC) {
+#pragma unroll
+    for (int k = 0; k < S::FPT; k++)
+      if (own[k]) partner_line_reduce<D, N>(pl[VISC ? k : 0], cp[VISC ? k : 0]);
+  }
+  stage_U_store<D, N>(U, a, lops, lU, e0, neb, V * nS, sr);
+  if (threadIdx.x < EB) lmask[threadIdx.x] = 0u;
+  __syncthreads();
   const double *u = lU + el * V * nS;
   const double *we = w + e * V * nS;
   double *c = lR + el * L::R, *g_l = lR + el * L::R;
@@ -129,18 +159,14 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
     for (int k = 0; k < S::FPT; k++) {
       const int f = t + k * nS;
       if (f >= nF) continue;
-      long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
-      if (sl == owner && f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
+      if (own[k] && f % S::nFF == 0) atomicOr(&lmask[el], 1u << (f / S::nFF));
       if (VISC) {
-        if (sl == owner) {
+        if (own[k]) {
           // the partner's trace, from its U; parked in Utr at this element's own point
-          double pt[V];
-          partner_trace<D, N>(U, a.ops, ep, fp, pt);
 #pragma unroll
           for (int v = 0; v < V; v++) {
-            c[v * nF + f] = pt[v];
-            a.Utr[(e * V + v) * nF + f] = pt[v];
+            c[v * nF + f] = cp[VISC ? k : 0][v];
+            a.Utr[(e * V + v) * nF + f] = cp[VISC ? k : 0][v];
           }
         } else {
           int dim, side, base, str;
