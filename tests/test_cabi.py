@@ -143,3 +143,37 @@ def test_kernel_entries_reject_null_arguments():
     for name, args, what in null_arg:
         assert getattr(L, name)(*args) != 0, name
         assert f"{what} is NULL" in _lib.last_error(), (name, _lib.last_error())
+
+
+def test_persistent_plan_recording_rules():
+    """zfr_persist_record_begin / _end: one recording per thread, an empty recording is an
+    error, a NULL plan discards, and while a thread records, launches other than the Euler
+    divergence fail with a message before any HIP call (so this runs without a GPU)."""
+    L = _lib.lib()
+    assert L.zfr_persist_record_end(None) != 0 and "not recording" in _lib.last_error()
+    assert L.zfr_persist_record_begin() == 0
+    assert L.zfr_persist_record_begin() != 0 and "already recording" in _lib.last_error()
+    p = C.c_void_p()
+    assert L.zfr_persist_record_end(C.byref(p)) != 0 and "no launch was recorded" in _lib.last_error()
+    assert not p.value
+    buf = C.c_void_p(0x1000)  # never dereferenced
+    d = _lib.EleDesc(nDims=3, order=3, nVars=5, nEles=8, nSpts=64, nFpts=96, viscous=1, ldg_b=0.5,
+                     gamma=1.4, prandtl=0.72, mu=1e-3, fix_vis=1, ldg_tau=0.1,
+                     ops1d=buf, metric=buf, fgeo=buf, nbr=buf, Utr=buf, Ftr=buf, Gtr=buf)
+    assert L.zfr_persist_record_begin() == 0
+    try:
+        assert L.zfr_gradient_flux(C.byref(d), buf, buf, None) != 0
+        assert "cannot be recorded" in _lib.last_error()
+        assert L.zfr_divergence(C.byref(d), buf, buf, None) != 0  # viscous: not an Euler divergence
+        assert "only Euler divergence launches" in _lib.last_error()
+        assert L.zfr_axpby(4, 1.0, buf, 1.0, buf, buf, None) != 0
+        assert "zfr_axpby cannot be recorded" in _lib.last_error()
+        assert L.zfr_pmg_transfer(3, 5, 8, 4, 3, buf, 1.0, buf, 0.0, buf, None) != 0
+        assert "zfr_pmg_transfer cannot be recorded" in _lib.last_error()
+        d.viscous = 0
+        d.order = 6  # a quad-only order: no hex kernel for it
+        d.nSpts, d.nFpts = 343, 294
+        assert L.zfr_divergence(C.byref(d), buf, buf, None) != 0
+    finally:
+        assert L.zfr_persist_record_end(None) == 0  # discarded
+    assert L.zfr_persist_record_end(None) != 0  # no longer recording
