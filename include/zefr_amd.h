@@ -267,7 +267,7 @@ typedef struct zfr_ele_desc {
   const double *ops1d;             /* 1D tables: lE[2][N], Dss[N][N], Dsf[N][2]          */
   const double *metric;            /* [E][D*D+1][nS]: adjugate inv_jaco(d1,d2), detJ     */
   const double *fgeo;              /* [E][D+1][nF]: unit normal of the gfpt, signed dA   */
-  const unsigned int *nbr;         /* [E][nF]: partner (ele << 9 | slot << 8 | fpt)      */
+  const unsigned int *nbr;         /* [E][nF]: partner (ele << (B+1) | slot << B | fpt), B = 8 if nF <= 256, else 9 */
   double *Utr;                     /* [E][V][nF] solution trace (scratch)                */
   double *Gtr;                     /* [E][D][V][nF] gradient trace (scratch, viscous)    */
   double *Ftr;                     /* [E][V][nF] common normal flux at the flux points

@@ -63,6 +63,8 @@ CASES = [
     dict(D=3, n=(2, 3, 4), lo=(0.0, -1.0, 0.5), hi=(1.0, 1.0, 2.0), order=2, perturb=0.03),
     dict(D=3, n=(2, 2, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=4, perturb=0.02),
     dict(D=3, n=(3, 2, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=1, perturb=0.0),
+    dict(D=3, n=(2, 2, 3), lo=(0.0,) * 3, hi=(1.0,) * 3, order=6, perturb=0.02),
+    dict(D=3, n=(2, 2, 2), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=7, perturb=0.0),
 ]
 
 
@@ -186,10 +188,10 @@ def test_triangle_mesh_is_rejected_with_the_reason():
         z.setup_host()
 
 
-@pytest.mark.parametrize("D,order,ok", [(2, 9, True), (2, 10, False), (3, 5, True), (3, 6, False)])
+@pytest.mark.parametrize("D,order,ok", [(2, 9, True), (2, 10, False), (3, 7, True), (3, 8, False)])
 def test_order_limits(D, order, ok):
     """Quads run to order 9 (the reference's Gauss-Legendre tables end at 10 points); hexes
-    to order 5, with a message naming the limit."""
+    to order 7, with a message naming the limit."""
     from zefr_amd import zefr
     n = (2,) * D
     mesh = make_mesh(f"ordlim{D}.msh", n, (0.0,) * D, (1.0,) * D)
@@ -201,7 +203,7 @@ def test_order_limits(D, order, ok):
         z.setup_host()
         assert z.dims()["nSpts"] == (order + 1) ** D
     else:
-        with pytest.raises(RuntimeError, match="order must be in 0.." + ("9 for quads" if D == 2 else "5 for hexes")):
+        with pytest.raises(RuntimeError, match="order must be in 0.." + ("9 for quads" if D == 2 else "7 for hexes")):
             z.setup_host()
 
 

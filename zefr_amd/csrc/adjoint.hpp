@@ -47,7 +47,7 @@ __device__ __forceinline__ double line_Dsf(const double *lops, int side, const d
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_adj_prep(const double *__restrict__ w, double *__restrict__ Wtr,
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_adj_prep(const double *__restrict__ w, double *__restrict__ Wtr,
                                                           KArgs a) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
@@ -134,7 +134,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
     own[k] = false;
     if (valid && f < nF) {
       long ep; int sl, fp;
-      unpack_nbr(nb[k], ep, sl, fp);
+      unpack_nbr<D, N>(nb[k], ep, sl, fp);
       own[k] = sl == owner;
       if (VISC && own[k]) partner_line_load<D, N>(U, a.ops, ep, fp, pl[VISC ? k : 0]);
     }
@@ -270,7 +270,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
       double Uo[V], Uob[V], Gb[V];
       {
         long ep; int sl, fp;
-        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+        unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
         double Up[V], Fb[V], Upb[V];
         if (VISC) {  // parked in phase 1
 #pragma unroll
@@ -386,7 +386,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
         int dim, side, base, str;
         fpt_info<D, N>(f, dim, side, base, str);
         long ep; int sl, fp;
-        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+        unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
 #pragma unroll
         for (int v = 0; v < V; v++) {
           const double cb = line_Dsf<N>(lops, side, gb + (dim * V + v) * nS, base, str);
@@ -412,7 +412,7 @@ __device__ __forceinline__ void adj_main_block(const double *__restrict__ U, con
 }
 
 template <int D, int N, bool VISC, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) __attribute__((amdgpu_waves_per_eu(2)))
 void k_adj_main(const double *__restrict__ U, const double *__restrict__ w, double *__restrict__ Wtr,
                 double *__restrict__ Xtr, double *__restrict__ vout, KArgs a) {
   adj_main_block<D, N, VISC, BM, FC>(U, w, Wtr, Xtr, vout, a, xcd_block(blockIdx.x, gridDim.x));
@@ -429,7 +429,7 @@ void k_adj_main_w64(const double *__restrict__ U, const double *__restrict__ w, 
 
 // v += E^T (the owner's cotangent of this element's trace) at the points the partner owns.
 template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_adj_gather(const double *__restrict__ Xtr, double *__restrict__ vout,
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_adj_gather(const double *__restrict__ Xtr, double *__restrict__ vout,
                                                             KArgs a) {
   using S = Shape<D, N>;
   constexpr int nS = S::nS, nF = S::nF, V = S::V;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_gather(const double *__rest
       const int f = t + k * nS;
       if (f >= nF) continue;
       long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
 #pragma unroll
       for (int v = 0; v < V; v++)
         T[v * nF + f] = (BM != kAdjAllPoints && sl == owner) ? 0.0 : Xtr[(ep * V + v) * nF + fp];
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_adj_face(const double *__restri
   const long e = i / nF;
   const int f = (int)(i - e * nF);
   long ep; int sl, fp;
-  unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+  unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
   if (sl != 0) return;  // each pair is reversed once, by its slot-0 (left) side
   const double *fg = a.fgeo + e * (D + 1) * nF + f;
   double n[D];
@@ -608,7 +608,7 @@ __device__ __forceinline__ void adj_main_general_block(const double *__restrict_
       fpt_info<D, N>(f, dim, side, base, str);
       const double *lE = tab_lE<N>(lops, side);
       long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
       double pt[V];
       partner_trace<D, N>(U, a.ops, ep, fp, pt);
 #pragma unroll
@@ -732,7 +732,7 @@ __device__ __forceinline__ void adj_main_general_block(const double *__restrict_
       if (f >= nF) continue;
       int dim, side, base, str;
       fpt_info<D, N>(f, dim, side, base, str);
-      const int sl = (int)((a.nbr[e * nF + f] >> 8) & 1u);
+      const int sl = nbr_slot<D, N>(a.nbr[e * nF + f]);
       const double ao = sl ? (0.5 + beta) : (0.5 - beta), ap = sl ? (0.5 - beta) : (0.5 + beta);
 #pragma unroll
       for (int v = 0; v < V; v++) {
@@ -758,7 +758,7 @@ __device__ __forceinline__ void adj_main_general_block(const double *__restrict_
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) __attribute__((amdgpu_waves_per_eu(2)))
 void k_adj_main_gen(const double *__restrict__ U, const double *__restrict__ w, double *__restrict__ Xtr,
                     double *__restrict__ vout, KArgs a) {
   adj_main_general_block<D, N>(U, w, Xtr, vout, a, xcd_block(blockIdx.x, gridDim.x));

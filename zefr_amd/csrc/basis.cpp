@@ -117,8 +117,8 @@ double lagrange_d1(const std::vector<double> &g, unsigned mode, double x) {
 void RefElement::build(int nd, int p) {
   if (nd != 2 && nd != 3) raise("nDims must be 2 or 3");
   // quads to order 9 (the reference's Gauss-Legendre tables end at 10 points); hexes to
-  // order 5: the kernels run one thread per solution point in workgroups of at most 256
-  // threads, and the partner map holds a flux point index in 8 bits
+  // order 7: the kernels run one thread per solution point, and at order 8 the viscous
+  // adjoint's LDS per element (199 KiB) exceeds what a workgroup can allocate
   if (p < 0 || p > max_order(nd)) raise("order must be in 0.." + std::to_string(max_order(nd)) + (nd == 2 ? " for quads" : " for hexes"));
   nDims = nd; order = p; N = p + 1;
   loc1d = gauss_legendre_points(N);
@@ -133,7 +133,7 @@ void RefElement::build(int nd, int p) {
     nSpts = N * N * N; nFptsPerFace = N * N; nFaces = 6; nNodes = 8;
   }
   nFpts = nFaces * nFptsPerFace;
-  if (nFpts > 256) raise("flux points per element must fit the 8-bit partner index");
+  if (nFpts > 512) raise("flux points per element must fit the 9-bit partner index");
   loc_spts.assign((size_t)nSpts * D, 0.0); idx_spts.assign((size_t)nSpts * D, 0);
   weights_spts.assign(nSpts, 0.0);
   // solution points: x index fastest (src/quads.cpp:72-84, src/hexas.cpp:84-101)

@@ -40,8 +40,8 @@ double lagrange_d1(const std::vector<double> &grid, unsigned mode, double x);
 // Reference element of a tensor-product quad/hex of order p: point sets in the
 // reference's orderings (src/quads.cpp:45-190, src/hexas.cpp:51-292), the 1D DFR tables
 // the kernels contract with, and linear shape functions.
-// Highest supported order per nDims: quads 9, hexes 5.
-inline int max_order(int nDims) { return nDims == 2 ? 9 : 5; }
+// Highest supported order per nDims: quads 9, hexes 7.
+inline int max_order(int nDims) { return nDims == 2 ? 9 : 7; }
 
 struct RefElement {
   int nDims = 0, order = 0, N = 0, nSpts = 0, nFpts = 0, nFptsPerFace = 0, nFaces = 0, nNodes = 0;
@@ -136,7 +136,7 @@ class Faces {
  public:
   int nGfpts = 0;
   std::vector<double> fgeo;   // [E][D+1][nF]
-  std::vector<unsigned> nbr;  // [E][nF]: partner ele << 9 | own slot << 8 | partner fpt
+  std::vector<unsigned> nbr;  // [E][nF]: partner ele << (B + 1) | own slot << B | partner fpt (B: nbr_fpt_bits)
   void setup(const Elements &e, const Connectivity &c);
   // reference layouts: norm [nDims][nGfpts], dA [2][nGfpts]
   std::vector<double> export_array(const std::string &name, const Elements &e, const Connectivity &c) const;

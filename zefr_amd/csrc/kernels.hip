@@ -226,9 +226,11 @@ int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipSt
       case 4: return launch<3, 4>(op, d, U, out, st, j);
       case 5: return launch<3, 5>(op, d, U, out, st, j);
       case 6: return launch<3, 6>(op, d, U, out, st, j);
+      case 7: return launch<3, 7>(op, d, U, out, st, j);
+      case 8: return launch<3, 8>(op, d, U, out, st, j);
     }
   }
-  return zfr::fail("kernel launch: supported shapes are quads of order 0..9 and hexes of order 0..5");
+  return zfr::fail("kernel launch: supported shapes are quads of order 0..9 and hexes of order 0..7");
 }
 
 template <int D>
@@ -245,10 +247,12 @@ void dispatch_multi_n(const MultiGroup &g, hipStream_t st) {
     case 5: launch_multi<D, 5>(g, st); break;
     case 6: launch_multi<D, 6>(g, st); break;
   }
+  switch (g.N) {
+    case 7: launch_multi<D, 7>(g, st); break;
+    case 8: launch_multi<D, 8>(g, st); break;
+  }
   if constexpr (D == 2) {
     switch (g.N) {
-      case 7: launch_multi<2, 7>(g, st); break;
-      case 8: launch_multi<2, 8>(g, st); break;
       case 9: launch_multi<2, 9>(g, st); break;
       case 10: launch_multi<2, 10>(g, st); break;
     }
@@ -690,8 +694,8 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
   for (int i = 0; i < n; i++) {
     const zfr_ele_desc_t *d = &descs[i];
     const int D = d->nDims, N = d->order + 1;
-    if ((D != 2 && D != 3) || N < 1 || N > (D == 2 ? 10 : 6))
-      return zfr::fail("zfr_multi_create: supported shapes are quads of order 0..9 and hexes of order 0..5");
+    if ((D != 2 && D != 3) || N < 1 || N > (D == 2 ? 10 : 8))
+      return zfr::fail("zfr_multi_create: supported shapes are quads of order 0..9 and hexes of order 0..7");
     long nS = 1, nFF = 1;
     for (int k = 0; k < D; k++) nS *= N;
     for (int k = 0; k < D - 1; k++) nFF *= N;
@@ -753,7 +757,7 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
         const int eb = zk::eles_per_block(&agg, defEB, nops_pad, per_ele);
         eb_of[shape] = eb;
         const size_t lds = sizeof(double) * (nops_pad + eb * per_ele);
-        if (lds > zk::kMaxLds) {
+        if (lds > zk::max_lds_n(cD, N)) {
           delete m;
           return zfr::fail("zfr_multi_create: one element needs more LDS than a workgroup may use");
         }

@@ -59,6 +59,9 @@ struct Shape {
   static constexpr int NOPS = 2 * N + N * N + 2 * N;
   static constexpr int NOPS_PAD = (NOPS + 7) & ~7;
   static constexpr int defaultEB() { return (kMaxThreads / nS) > 0 ? kMaxThreads / nS : 1; }
+  // workgroup size bound of the per-shape kernels: 256 threads, or one element's points
+  // rounded up to whole waves where an element has more (hexes of order 6 and 7)
+  static constexpr int kThreads = (nS > kMaxThreads) ? ((nS + 63) / 64) * 64 : kMaxThreads;
 };
 
 // The hardware dispatches workgroup b of a grid to XCD b % 8 (each XCD has its own L2).
@@ -192,10 +195,24 @@ __device__ __forceinline__ int fpt_end(int a, int b, int c, int dim, int side) {
 // reverse-mode counterparts live in physics.hpp (shared with a host build).
 using namespace zphys;
 
+// Partner map entries: ele << (B + 1) | slot << B | fpt, with B = 8 flux-point bits for
+// elements of at most 256 flux points and 9 above (hexes of order 6 and 7).
+template <int D, int N>
+struct NbrBits {
+  static constexpr int B = (Shape<D, N>::nF <= 256) ? 8 : 9;
+};
+
+template <int D, int N>
 __device__ __forceinline__ void unpack_nbr(unsigned int p, long &ep, int &slot, int &fp) {
-  ep = (long)(p >> 9);
-  slot = (int)((p >> 8) & 1u);
-  fp = (int)(p & 255u);
+  constexpr int B = NbrBits<D, N>::B;
+  ep = (long)(p >> (B + 1));
+  slot = (int)((p >> B) & 1u);
+  fp = (int)(p & ((1u << B) - 1u));
+}
+
+template <int D, int N>
+__device__ __forceinline__ int nbr_slot(unsigned int p) {
+  return (int)((p >> NbrBits<D, N>::B) & 1u);
 }
 
 // Which sides' gradients enter the LDG flux (src/faces.cpp:1520-1545).
@@ -459,7 +476,7 @@ __device__ __forceinline__ void rk_trace(const KArgs &a, const double *lops, dou
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
     if (f >= nF) continue;
-    if (owner >= 0 && (int)((a.nbr[e * nF + f] >> 8) & 1u) == owner) continue;
+    if (owner >= 0 && nbr_slot<D, N>(a.nbr[e * nF + f]) == owner) continue;
     int dim, side, base, str;
     fpt_info<D, N>(f, dim, side, base, str);
     const double *lE = tab_lE<N>(lops, side);
@@ -596,7 +613,7 @@ __device__ __forceinline__ void extrapolate_block(const double *__restrict__ U, 
 #pragma unroll
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
-    slot[k] = (a.trace_sel && el < neb && f < nF) ? (int)((a.nbr[e * nF + f] >> 8) & 1u) : -1;
+    slot[k] = (a.trace_sel && el < neb && f < nF) ? nbr_slot<D, N>(a.nbr[e * nF + f]) : -1;
   }
   stage_U<D, N>(U, a, lops, lU, e0, neb, V * nS);
   __syncthreads();
@@ -618,7 +635,7 @@ __device__ __forceinline__ void extrapolate_block(const double *__restrict__ U, 
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_extrapolate(const double *__restrict__ U, KArgs a) {
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_extrapolate(const double *__restrict__ U, KArgs a) {
   extrapolate_block<D, N>(U, a, xcd_block(blockIdx.x, gridDim.x));
 }
 
@@ -629,7 +646,7 @@ __global__ __launch_bounds__(64) void k_extrapolate_w64(const double *__restrict
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_extrapolate_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_extrapolate_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
   extrapolate_block<D, N>(cs[c].U, cs[c].a, (long)blockIdx.x - starts[c]);
 }
@@ -650,7 +667,7 @@ __device__ __forceinline__ void common_solution(const KArgs &a, const double *lo
     fpt_info<D, N>(f, dim, side, base, str);
     const double *lE = tab_lE<N>(lops, side);
     long ep; int sl, fp;
-    unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+    unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
 #pragma unroll
     for (int v = 0; v < V; v++) {
       const double own = line_extrap<N>(lE, u + v * nS, base, str);
@@ -764,7 +781,7 @@ __device__ __forceinline__ void gradient_block(const double *__restrict__ U, dou
   for (int k = 0; k < S::FPT; k++) {
     const int f = t + k * nS;
     if (f >= nF) continue;
-    const int sl = (int)((a.nbr[e * nF + f] >> 8) & 1u);
+    const int sl = nbr_slot<D, N>(a.nbr[e * nF + f]);
     // the partner needs this side's gradient iff this side is L and dUL is used, or R and dUR is used
     const bool write = sl ? need_right_grad(beta) : need_left_grad(beta);
     if (!write) continue;
@@ -780,13 +797,13 @@ __device__ __forceinline__ void gradient_block(const double *__restrict__ U, dou
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_gradient(const double *__restrict__ U, double *__restrict__ dU_out,
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_gradient(const double *__restrict__ U, double *__restrict__ dU_out,
                                                           KArgs a) {
   gradient_block<D, N>(U, dU_out, a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <int D, int N>
-__global__ __launch_bounds__(kMaxThreads) void k_gradient_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_gradient_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
   gradient_block<D, N>(cs[c].U, nullptr, cs[c].a, (long)blockIdx.x - starts[c]);
 }
@@ -830,7 +847,7 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
       psl[k] = 0;
       if (f >= nF) continue;
       long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
       psl[k] = sl;
       if constexpr (SELF) {
         partner_trace<D, N>(U, a.ops, ep, fp, pUs[k]);
@@ -897,7 +914,7 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
       fpt_info<D, N>(f, dim, side, base, str);
       const double *lE = tab_lE<N>(lops, side);
       long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
       const double *fg = a.fgeo + e * (D + 1) * nF + f;
       double n[D];
 #pragma unroll
@@ -1049,14 +1066,14 @@ struct DivWaves {
 };
 
 template <int D, int N, bool VISC, int BM, int FC, bool SELF = !VISC>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(DivWaves<D, N, VISC, FC>::value)))
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) __attribute__((amdgpu_waves_per_eu(DivWaves<D, N, VISC, FC>::value)))
 void k_divergence(const double *U, double *__restrict__ divF, KArgs a) {
   static_assert(!(VISC && SELF), "the viscous divergence reads the gradient traces, hence the traces too");
   divergence_block<D, N, VISC, BM, FC, SELF>(U, divF, a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <int D, int N, bool VISC, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) void k_divergence_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_divergence_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
   divergence_block<D, N, VISC, BM, FC, !VISC>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
 }
@@ -1137,7 +1154,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
     own[k] = false;
     if (valid && f < nF) {
       long ep; int sl, fp;
-      unpack_nbr(nb[k], ep, sl, fp);
+      unpack_nbr<D, N>(nb[k], ep, sl, fp);
       slot[k] = sl;
       own[k] = sl == owner;
       if (own[k]) {
@@ -1223,7 +1240,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
         // gradient's lines are gathered (formed in the Rusanov order it needs ~350 registers)
         {
           long ep; int sl, fp;
-          unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+          unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
           const long ip = self ? (e * V * nF + f) : (ep * V * nF + fp);
 #pragma unroll
           for (int v = 0; v < V; v++) Up[v] = a.Utr[ip + v * nF];
@@ -1258,7 +1275,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
       asm volatile("" ::: "memory");  // load the partner's state only now: fewer live registers
       {
         long ep; int sl, fp;
-        unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+        unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
         const long ip = self ? (e * V * nF + f) : (ep * V * nF + fp);  // cached since phase 1
 #pragma unroll
         for (int v = 0; v < V; v++) Up[v] = a.Utr[ip + v * nF];
@@ -1324,7 +1341,7 @@ __device__ __forceinline__ void visc_onesided_block(const double *__restrict__ U
 }
 
 template <int D, int N, int BM, int FC, bool SELF>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
 void k_visc_onesided(const double *__restrict__ U, double *__restrict__ divF, KArgs a) {
   visc_onesided_block<D, N, BM, FC, SELF>(U, divF, a, xcd_block(blockIdx.x, gridDim.x));
 }
@@ -1339,7 +1356,7 @@ void k_visc_onesided_w64(const double *__restrict__ U, double *__restrict__ divF
 }
 
 template <int D, int N, int BM, int FC>
-__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) __attribute__((amdgpu_waves_per_eu(OneSidedWaves<D, N, FC>::value)))
 void k_visc_onesided_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
   visc_onesided_block<D, N, BM, FC, true>(cs[c].U, cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
@@ -1374,7 +1391,7 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
       const int f = t + k * nS;
       if (f >= nF) continue;
       long ep; int sl, fp;
-      unpack_nbr(a.nbr[e * nF + f], ep, sl, fp);
+      unpack_nbr<D, N>(a.nbr[e * nF + f], ep, sl, fp);
       const double dAs = a.fgeo[(e * (D + 1) + D) * nF + f];
       const double *src = (sl == owner) ? a.Ftr + e * V * nF + f : a.Ftr + ep * V * nF + fp;
 #pragma unroll
@@ -1416,12 +1433,12 @@ __device__ __forceinline__ void face_correction_block(double *__restrict__ divF,
 }
 
 template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_face_correction(double *__restrict__ divF, KArgs a) {
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_face_correction(double *__restrict__ divF, KArgs a) {
   face_correction_block<D, N, BM>(divF, a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <int D, int N, int BM>
-__global__ __launch_bounds__(kMaxThreads) void k_face_correction_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
+__global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_face_correction_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
   face_correction_block<D, N, BM>(cs[c].out, cs[c].a, (long)blockIdx.x - starts[c]);
 }
@@ -1430,8 +1447,18 @@ __global__ __launch_bounds__(kMaxThreads) void k_face_correction_multi(const Mul
 #include "adjoint.hpp"
 
 // ---------------------------------------------------------------- dispatch
-// LDS budget per workgroup; kept at 64 KiB so several workgroups share a CU.
+// LDS budget per workgroup; kept at 64 KiB so several workgroups share a CU.  Shapes whose
+// element alone needs more than 256 threads (hexes of order 6 and 7) may take up to the
+// 160 KiB a gfx950 workgroup can allocate (one element per workgroup).
 constexpr size_t kMaxLds = 64 * 1024;
+constexpr size_t kMaxLdsBig = 160 * 1024;
+template <int D, int N>
+constexpr size_t max_lds() { return Shape<D, N>::nS > kMaxThreads ? kMaxLdsBig : kMaxLds; }
+inline size_t max_lds_n(int D, int N) {
+  int nS = N * N;
+  if (D == 3) nS *= N;
+  return nS > kMaxThreads ? kMaxLdsBig : kMaxLds;
+}
 
 // Elements per workgroup: the requested tile, limited by the thread count and the LDS
 // the kernel needs per element (per_ele doubles).
@@ -1524,11 +1551,11 @@ size_t lds_per_element_n(int N, Op op, bool viscous) {
     case 4: return lds_per_element<D, 4>(op, viscous);
     case 5: return lds_per_element<D, 5>(op, viscous);
     case 6: return lds_per_element<D, 6>(op, viscous);
+    case 7: return lds_per_element<D, 7>(op, viscous);
+    case 8: return lds_per_element<D, 8>(op, viscous);
   }
   if constexpr (D == 2) {
     switch (N) {
-      case 7: return lds_per_element<2, 7>(op, viscous);
-      case 8: return lds_per_element<2, 8>(op, viscous);
       case 9: return lds_per_element<2, 9>(op, viscous);
       case 10: return lds_per_element<2, 10>(op, viscous);
     }

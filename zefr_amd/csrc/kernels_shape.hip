@@ -1,5 +1,5 @@
 // kernels_shape.hip -- the kernel launches of one element shape: compiled once per
-// (nDims, points per direction) with -DZK_D=<2|3> -DZK_N=<1..10 for quads, 1..6 for
+// (nDims, points per direction) with -DZK_D=<2|3> -DZK_N=<1..10 for quads, 1..8 for
 // hexes>, so the shapes build in parallel and a change to one shape's code rebuilds one
 // object.
 #include "kernels.hpp"
@@ -62,7 +62,7 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   // the viscous adjoint runs one element per workgroup: its LDS then allows ~7
   // workgroups per CU, more than any larger tile
   const int EB = (op == Op::AdjMain && d->viscous) ? 1 : eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele);
-  if (sizeof(double) * (S::NOPS_PAD + EB * per_ele) > kMaxLds)
+  if (sizeof(double) * (S::NOPS_PAD + EB * per_ele) > max_lds<D, N>())
     return zfr::fail("kernel launch: one element needs more LDS than a workgroup may use (order too high)");
   KArgs a = make_args(d, EB);
   if (j.rk) {

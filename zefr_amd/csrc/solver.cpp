@@ -234,7 +234,9 @@ zfr_ele_desc_t Elements::desc(const zefr_input_t &in, int phase) const {
 // slot-1 normals by the partner's.
 void Faces::setup(const Elements &el, const Connectivity &c) {
   const int D = el.nDims, E = el.nEles, nF = el.nFpts, R = D + 1;
-  if ((long)E >= (1L << 23)) raise("element count exceeds the 23-bit partner index");
+  const int B = nbr_fpt_bits(nF);
+  if ((long)E >= (1L << (31 - B)))
+    raise("element count exceeds the " + std::to_string(31 - B) + "-bit partner index");
   nGfpts = c.nGfpts;
   fgeo.assign((size_t)E * R * nF, 0.0);
   nbr.assign((size_t)E * nF, 0u);
@@ -255,14 +257,14 @@ void Faces::setup(const Elements &el, const Connectivity &c) {
       fgeo[((size_t)e * R + D) * nF + f] = s ? -a : a;
       const int g = c.fpt2gfpt[(size_t)f * E + e];
       const int pe = c.gf_ele[(size_t)(1 - s) * c.nGfpts + g], pf = c.gf_fpt[(size_t)(1 - s) * c.nGfpts + g];
-      nbr[(size_t)e * nF + f] = ((unsigned)pe << 9) | ((unsigned)s << 8) | (unsigned)pf;
+      nbr[(size_t)e * nF + f] = ((unsigned)pe << (B + 1)) | ((unsigned)s << B) | (unsigned)pf;
     }
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < E; e++)
     for (int f = 0; f < nF; f++) {
       const unsigned p = nbr[(size_t)e * nF + f];
-      if (((p >> 8) & 1u) == 0) continue;
-      const size_t pe = p >> 9, pf = p & 255u;
+      if (((p >> B) & 1u) == 0) continue;
+      const size_t pe = p >> (B + 1), pf = p & ((1u << B) - 1u);
       for (int d = 0; d < D; d++) fgeo[((size_t)e * R + d) * nF + f] = fgeo[(pe * R + d) * nF + pf];
     }
 }
