@@ -28,8 +28,9 @@
 // Navier-Stokes with Rusanov + LDG, RK44 or forward Euler with a fixed dt.
 //
 // What pins each part (tests/test_oracle.py): operators against the reference's baked
-// GiMMiK hex p=3/p=4 constants and polynomial exactness; Gauss-Legendre rules against
-// numpy; metrics against affine closed forms and cell volumes; connectivity by brute
+// GiMMiK hex p=3/p=4 constants and polynomial exactness; Gauss-Legendre rules (1..10
+// points) against numpy, where the reference's 10-point table keeps one node pair 5e-4
+// off the true nodes (+-0.433895394129247; restated as the reference has it); metrics against affine closed forms and cell volumes; connectivity by brute
 // force (each global flux point has one slot-0 and one slot-1 point at the same
 // physical location); the slot of every element flux point against the first-incidence
 // rule derived from the mesh file by an independent reader (periodic faces included; the
