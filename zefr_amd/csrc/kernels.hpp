@@ -1072,6 +1072,14 @@ void k_divergence(const double *U, double *__restrict__ divF, KArgs a) {
   divergence_block<D, N, VISC, BM, FC, SELF>(U, divF, a, xcd_block(blockIdx.x, gridDim.x));
 }
 
+// The same kernel for workgroups of one wave (the small meshes of coarse p-multigrid
+// levels and small cases): its barriers need no cross-wave synchronisation.
+template <int D, int N, bool VISC, int BM, int FC, bool SELF = !VISC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DivWaves<D, N, VISC, FC>::value)))
+void k_divergence_w64(const double *U, double *__restrict__ divF, KArgs a) {
+  divergence_block<D, N, VISC, BM, FC, SELF>(U, divF, a, xcd_block(blockIdx.x, gridDim.x));
+}
+
 template <int D, int N, bool VISC, int BM, int FC>
 __global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_divergence_multi(const MultiCase *__restrict__ cs, const long *__restrict__ starts, int nc) {
   const int c = find_case(starts, nc, blockIdx.x);
@@ -1469,7 +1477,13 @@ inline int eles_per_block(const zfr_ele_desc_t *d, int defEB, size_t fixed, size
     // per CU, so that every SIMD holds a wave; the kernels are latency-bound there, and
     // partly filled waves cost less than idle SIMDs
     const long per_cu4 = ((long)d->nEles + 1023) / 1024;
-    if (per_cu4 < eb) eb = per_cu4 < 1 ? 1 : (int)per_cu4;
+    if (per_cu4 < eb) {
+      eb = per_cu4 < 1 ? 1 : (int)per_cu4;
+      // one-wave workgroups (barriers without cross-wave synchronisation) where they
+      // still leave >= 1024 of them
+      const int eb1 = (64 / d->nSpts) > 0 ? 64 / d->nSpts : 1;
+      if (eb1 < eb && ((long)d->nEles + eb1 - 1) / eb1 >= 1024) eb = eb1;
+    }
   }
   while (eb > 1 && (eb * d->nSpts > kMaxThreads || sizeof(double) * (fixed + eb * per_ele) > kMaxLds)) eb--;
   return eb < 1 ? 1 : eb;

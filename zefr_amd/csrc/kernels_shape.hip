@@ -17,7 +17,12 @@ namespace zk {
 template <int D, int N, int FC>
 void launch_div(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds, hipStream_t st, const double *U,
                 double *out, const KArgs &a) {
-  if (!d->viscous && rk_writes_input(a.rk, U))
+  if (!d->viscous && block.x <= 64) {  // one wave per workgroup
+    if (rk_writes_input(a.rk, U))
+      hipLaunchKernelGGL((k_divergence_w64<D, N, false, 0, FC, false>), grid, block, lds, st, U, out, a);
+    else
+      hipLaunchKernelGGL((k_divergence_w64<D, N, false, 0, FC, true>), grid, block, lds, st, U, out, a);
+  } else if (!d->viscous && rk_writes_input(a.rk, U))
     hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, false>), grid, block, lds, st, U, out, a);
   else if (!d->viscous) hipLaunchKernelGGL((k_divergence<D, N, false, 0, FC, true>), grid, block, lds, st, U, out, a);
   else if (d->ldg_b == 0.5) hipLaunchKernelGGL((k_divergence<D, N, true, 1, FC>), grid, block, lds, st, U, out, a);
