@@ -166,8 +166,8 @@ def test_batch_with_high_orders():
     del b
 
 
-# Hexes of order 6 and 7: one element per workgroup of 343 / 512 threads (up to 160 KiB of
-# LDS), partner-map entries with 9 flux-point bits (294 / 384 flux points per element).
+# Hexes of order 6 to 8: one element per workgroup of 343 / 512 / 729 threads (up to 160
+# KiB of LDS), partner-map entries with 9 flux-point bits (294 / 384 / 486 flux points).
 HEX_CASES = [
     dict(name="h_euler_p6_curved", n=(2, 2, 3), lo=(0.0,) * 3, hi=(1.0,) * 3, order=6, viscous=False,
          state="smooth", perturb=0.02),
@@ -177,6 +177,12 @@ HEX_CASES = [
          state="smooth", perturb=0.02, extra={"ldg_b": 0.2}),
     dict(name="h_ns_p7_roe_bm", n=(2, 2, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=7, viscous=True,
          state="smooth", extra={"fconv_type": "Roe", "ldg_b": -0.5}),
+    dict(name="h_euler_p8", n=(2, 2, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=8, viscous=False,
+         state="smooth", perturb=0.02),
+    dict(name="h_ns_p8_tgv", n=(2, 2, 2), lo=(0.0,) * 3, hi=(TWO_PI,) * 3, order=8, viscous=True,
+         state="tgv3d"),
+    dict(name="h_ns_p8_beta03", n=(2, 2, 2), lo=(0.0,) * 3, hi=(1.0,) * 3, order=8, viscous=True,
+         state="smooth", extra={"ldg_b": 0.3}),
 ]
 
 
@@ -186,7 +192,7 @@ def test_hex_one_residual(case):
 
 
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "separate"])
-@pytest.mark.parametrize("case", [HEX_CASES[0], HEX_CASES[1]], ids=lambda c: c["name"])
+@pytest.mark.parametrize("case", [HEX_CASES[0], HEX_CASES[1], HEX_CASES[5]], ids=lambda c: c["name"])
 def test_hex_rk4_50_steps(case, fused):
     print("[high order rk]", case["name"], fused, flush=True)
     z, o, _ = paired(dict(case, name=case["name"] + ("_f" if fused else "_s"), dt=1e-4))
@@ -245,3 +251,17 @@ def test_batch_with_high_order_hexes():
             err = float(np.abs(got - ref).max() / np.abs(ref).max())
             assert err <= 1e-15, (rep, case["name"], err)
     del b
+
+
+def test_hex_p8_vjp_is_refused_with_the_reason():
+    """At hex order 8 one element's viscous adjoint needs 199 KiB of LDS, more than a
+    workgroup can allocate: the VJP fails with that reason, and the forward residual of
+    the same solver still works afterwards."""
+    import torch
+    from test_gpu_vjp import noisy_state, to_dev
+    z, o, _ = paired(dict(HEX_CASES[5], name="h_ns_p8_tgv_vjp"))
+    U = torch.from_numpy(to_dev(noisy_state(o, 2, amp=0.01))).cuda()
+    with pytest.raises(RuntimeError, match="hexes to order 7"):
+        z.residual_vjp(U, torch.ones_like(U))
+    z.compute_residual(0)
+    assert np.isfinite(z.state("divF", 0)).all()

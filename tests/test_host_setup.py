@@ -188,10 +188,10 @@ def test_triangle_mesh_is_rejected_with_the_reason():
         z.setup_host()
 
 
-@pytest.mark.parametrize("D,order,ok", [(2, 9, True), (2, 10, False), (3, 7, True), (3, 8, False)])
+@pytest.mark.parametrize("D,order,ok", [(2, 9, True), (2, 10, False), (3, 8, True), (3, 9, False)])
 def test_order_limits(D, order, ok):
     """Quads run to order 9 (the reference's Gauss-Legendre tables end at 10 points); hexes
-    to order 7, with a message naming the limit."""
+    to order 8, with a message naming the limit."""
     from zefr_amd import zefr
     n = (2,) * D
     mesh = make_mesh(f"ordlim{D}.msh", n, (0.0,) * D, (1.0,) * D)
@@ -203,7 +203,7 @@ def test_order_limits(D, order, ok):
         z.setup_host()
         assert z.dims()["nSpts"] == (order + 1) ** D
     else:
-        with pytest.raises(RuntimeError, match="order must be in 0.." + ("9 for quads" if D == 2 else "7 for hexes")):
+        with pytest.raises(RuntimeError, match="order must be in 0.." + ("9 for quads" if D == 2 else "8 for hexes")):
             z.setup_host()
 
 

@@ -117,8 +117,9 @@ double lagrange_d1(const std::vector<double> &g, unsigned mode, double x) {
 void RefElement::build(int nd, int p) {
   if (nd != 2 && nd != 3) raise("nDims must be 2 or 3");
   // quads to order 9 (the reference's Gauss-Legendre tables end at 10 points); hexes to
-  // order 7: the kernels run one thread per solution point, and at order 8 the viscous
-  // adjoint's LDS per element (199 KiB) exceeds what a workgroup can allocate
+  // order 8: the kernels run one thread per solution point, and at order 9 the Euler
+  // divergence's LDS per element (180 KiB) exceeds what a workgroup can allocate (at
+  // order 8 the viscous adjoint's, 199 KiB, does: its launch refuses)
   if (p < 0 || p > max_order(nd)) raise("order must be in 0.." + std::to_string(max_order(nd)) + (nd == 2 ? " for quads" : " for hexes"));
   nDims = nd; order = p; N = p + 1;
   loc1d = gauss_legendre_points(N);

@@ -47,7 +47,7 @@ inline bool pmg_prolong_fits(int N, int n_fine) { return n_fine >= N && n_fine <
 
 // The partner map packs (partner element, own slot, partner flux point) into 32 bits:
 // ele << (B + 1) | slot << B | fpt, with B = 8 flux-point bits when an element has at most
-// 256 flux points and 9 otherwise (hexes of order 6 and 7).
+// 256 flux points and 9 otherwise (hexes of order 6 to 8).
 inline int nbr_fpt_bits(int nF) { return nF <= 256 ? 8 : 9; }
 
 // Convert a kernel-layer status into an exception carrying its message.

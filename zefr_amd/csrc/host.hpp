@@ -40,8 +40,8 @@ double lagrange_d1(const std::vector<double> &grid, unsigned mode, double x);
 // Reference element of a tensor-product quad/hex of order p: point sets in the
 // reference's orderings (src/quads.cpp:45-190, src/hexas.cpp:51-292), the 1D DFR tables
 // the kernels contract with, and linear shape functions.
-// Highest supported order per nDims: quads 9, hexes 7.
-inline int max_order(int nDims) { return nDims == 2 ? 9 : 7; }
+// Highest supported order per nDims: quads 9, hexes 8 (the residual VJP of hexes to 7).
+inline int max_order(int nDims) { return nDims == 2 ? 9 : 8; }
 
 struct RefElement {
   int nDims = 0, order = 0, N = 0, nSpts = 0, nFpts = 0, nFptsPerFace = 0, nFaces = 0, nNodes = 0;

@@ -228,9 +228,10 @@ int dispatch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipSt
       case 6: return launch<3, 6>(op, d, U, out, st, j);
       case 7: return launch<3, 7>(op, d, U, out, st, j);
       case 8: return launch<3, 8>(op, d, U, out, st, j);
+      case 9: return launch<3, 9>(op, d, U, out, st, j);
     }
   }
-  return zfr::fail("kernel launch: supported shapes are quads of order 0..9 and hexes of order 0..7");
+  return zfr::fail("kernel launch: supported shapes are quads of order 0..9 and hexes of order 0..8");
 }
 
 template <int D>
@@ -250,12 +251,10 @@ void dispatch_multi_n(const MultiGroup &g, hipStream_t st) {
   switch (g.N) {
     case 7: launch_multi<D, 7>(g, st); break;
     case 8: launch_multi<D, 8>(g, st); break;
+    case 9: launch_multi<D, 9>(g, st); break;
   }
   if constexpr (D == 2) {
-    switch (g.N) {
-      case 9: launch_multi<2, 9>(g, st); break;
-      case 10: launch_multi<2, 10>(g, st); break;
-    }
+    if (g.N == 10) launch_multi<2, 10>(g, st);
   }
 }
 
@@ -694,8 +693,8 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
   for (int i = 0; i < n; i++) {
     const zfr_ele_desc_t *d = &descs[i];
     const int D = d->nDims, N = d->order + 1;
-    if ((D != 2 && D != 3) || N < 1 || N > (D == 2 ? 10 : 8))
-      return zfr::fail("zfr_multi_create: supported shapes are quads of order 0..9 and hexes of order 0..7");
+    if ((D != 2 && D != 3) || N < 1 || N > (D == 2 ? 10 : 9))
+      return zfr::fail("zfr_multi_create: supported shapes are quads of order 0..9 and hexes of order 0..8");
     long nS = 1, nFF = 1;
     for (int k = 0; k < D; k++) nS *= N;
     for (int k = 0; k < D - 1; k++) nFF *= N;

@@ -60,7 +60,7 @@ struct Shape {
   static constexpr int NOPS_PAD = (NOPS + 7) & ~7;
   static constexpr int defaultEB() { return (kMaxThreads / nS) > 0 ? kMaxThreads / nS : 1; }
   // workgroup size bound of the per-shape kernels: 256 threads, or one element's points
-  // rounded up to whole waves where an element has more (hexes of order 6 and 7)
+  // rounded up to whole waves where an element has more (hexes of order 6 to 8)
   static constexpr int kThreads = (nS > kMaxThreads) ? ((nS + 63) / 64) * 64 : kMaxThreads;
 };
 
@@ -196,7 +196,7 @@ __device__ __forceinline__ int fpt_end(int a, int b, int c, int dim, int side) {
 using namespace zphys;
 
 // Partner map entries: ele << (B + 1) | slot << B | fpt, with B = 8 flux-point bits for
-// elements of at most 256 flux points and 9 above (hexes of order 6 and 7).
+// elements of at most 256 flux points and 9 above (hexes of order 6 to 8).
 template <int D, int N>
 struct NbrBits {
   static constexpr int B = (Shape<D, N>::nF <= 256) ? 8 : 9;
@@ -1456,7 +1456,7 @@ __global__ __launch_bounds__((Shape<D, N>::kThreads)) void k_face_correction_mul
 
 // ---------------------------------------------------------------- dispatch
 // LDS budget per workgroup; kept at 64 KiB so several workgroups share a CU.  Shapes whose
-// element alone needs more than 256 threads (hexes of order 6 and 7) may take up to the
+// element alone needs more than 256 threads (hexes of order 6 to 8) may take up to the
 // 160 KiB a gfx950 workgroup can allocate (one element per workgroup).
 constexpr size_t kMaxLds = 64 * 1024;
 constexpr size_t kMaxLdsBig = 160 * 1024;
@@ -1567,12 +1567,10 @@ size_t lds_per_element_n(int N, Op op, bool viscous) {
     case 6: return lds_per_element<D, 6>(op, viscous);
     case 7: return lds_per_element<D, 7>(op, viscous);
     case 8: return lds_per_element<D, 8>(op, viscous);
+    case 9: return lds_per_element<D, 9>(op, viscous);
   }
   if constexpr (D == 2) {
-    switch (N) {
-      case 9: return lds_per_element<2, 9>(op, viscous);
-      case 10: return lds_per_element<2, 10>(op, viscous);
-    }
+    if (N == 10) return lds_per_element<2, 10>(op, viscous);
   }
   return 0;
 }

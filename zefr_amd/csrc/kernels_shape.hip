@@ -1,5 +1,5 @@
 // kernels_shape.hip -- the kernel launches of one element shape: compiled once per
-// (nDims, points per direction) with -DZK_D=<2|3> -DZK_N=<1..10 for quads, 1..8 for
+// (nDims, points per direction) with -DZK_D=<2|3> -DZK_N=<1..10 for quads, 1..9 for
 // hexes>, so the shapes build in parallel and a change to one shape's code rebuilds one
 // object.
 #include "kernels.hpp"
@@ -47,6 +47,10 @@ void launch_adj_main(const zfr_ele_desc_t *d, dim3 grid, dim3 block, size_t lds,
     hipLaunchKernelGGL((k_adj_main<D, N, true, 2, FC>), grid, block, lds, st, U, j.w, j.Wtr, j.Xtr, out, a);
 }
 
+// Shapes with the adjoint main kernels (not hexes of order 8: 199 KiB of LDS per element).
+template <int D, int N>
+constexpr bool kAdjoint = !(D == 3 && N > 8);
+
 template <int D, int N>
 int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStream_t st, const AdjPtrs &j) {
   using S = Shape<D, N>;
@@ -56,6 +60,9 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
   const size_t per_ele = lds_per_element<D, N>(op, d->viscous != 0);
   const bool general = d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5;
   if (op == Op::AdjMain && general) return zfr::fail("k_adj_main is the one-sided LDG adjoint (beta = +-0.5)");
+  if ((op == Op::AdjMain || op == Op::AdjMainGen) && !kAdjoint<D, N>)
+    return zfr::fail("the residual VJP covers hexes to order 7: at order 8 one element's adjoint needs more LDS "
+                     "than a workgroup can allocate");
   if ((op == Op::AdjFace || op == Op::AdjMainGen) && !general)
     return zfr::fail("the general-beta adjoint launches need a viscous descriptor with beta != +-0.5");
   if ((op == Op::AdjFace || op == Op::AdjMainGen) && (!d->Utr || !d->Gtr))
@@ -125,8 +132,10 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       hipLaunchKernelGGL((k_adj_prep<D, N>), grid, block, lds, st, j.w, j.Wtr, a);
       break;
     case Op::AdjMain:
-      if (d->fconv_type == 2) launch_adj_main<D, N, 1>(d, grid, block, lds, st, U, j, out, a);
-      else launch_adj_main<D, N, 0>(d, grid, block, lds, st, U, j, out, a);
+      if constexpr (kAdjoint<D, N>) {
+        if (d->fconv_type == 2) launch_adj_main<D, N, 1>(d, grid, block, lds, st, U, j, out, a);
+        else launch_adj_main<D, N, 0>(d, grid, block, lds, st, U, j, out, a);
+      }
       break;
     case Op::AdjGather:
       if (general) hipLaunchKernelGGL((k_adj_gather<D, N, kAdjAllPoints>), grid, block, lds, st, j.Xtr, out, a);
@@ -141,7 +150,7 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
       break;
     }
     case Op::AdjMainGen:
-      hipLaunchKernelGGL((k_adj_main_gen<D, N>), grid, block, lds, st, U, j.w, j.Xtr, out, a);
+      if constexpr (kAdjoint<D, N>) hipLaunchKernelGGL((k_adj_main_gen<D, N>), grid, block, lds, st, U, j.w, j.Xtr, out, a);
       break;
   }
   return zfr::check_hip(hipGetLastError(), "kernel launch");
