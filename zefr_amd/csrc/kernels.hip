@@ -736,7 +736,8 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
         if (one) g.D = d0;
       }
       // one tile per order, chosen for that order's total element count in the group, as
-      // for one solver; the launch takes the largest LDS of its orders
+      // for one solver but never shrunk to one wave (a mixed launch's workgroups have
+      // kMaxThreads threads); the launch takes the largest LDS of its orders
       std::map<int, int> eb_of;  // tile per shape, keyed nDims * 16 + N
       g.lds = 0;
       g.EB = 0;
@@ -753,7 +754,7 @@ int zfr_multi_create(const zfr_ele_desc_t *descs, const double *const *U, double
           if (descs[i2].order + 1 == N && descs[i2].nDims == cD) agg.nEles += descs[i2].nEles;
         agg.eles_per_block = 0;
         const int defEB = (zk::kMaxThreads / nS) > 0 ? zk::kMaxThreads / nS : 1;
-        const int eb = zk::eles_per_block(&agg, defEB, nops_pad, per_ele);
+        const int eb = zk::eles_per_block(&agg, defEB, nops_pad, per_ele, false);
         eb_of[shape] = eb;
         const size_t lds = sizeof(double) * (nops_pad + eb * per_ele);
         if (lds > zk::max_lds_n(cD, N)) {

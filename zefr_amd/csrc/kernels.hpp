@@ -1470,7 +1470,10 @@ inline size_t max_lds_n(int D, int N) {
 
 // Elements per workgroup: the requested tile, limited by the thread count and the LDS
 // the kernel needs per element (per_ele doubles).
-inline int eles_per_block(const zfr_ele_desc_t *d, int defEB, size_t fixed, size_t per_ele) {
+// one_wave: the launch's workgroup is EB * nS threads, so that a one-wave tile runs as a
+// one-wave workgroup; false for launches of kMaxThreads-thread workgroups (multi-case
+// launches, persistent plans), where the threads beyond EB * nS would idle
+inline int eles_per_block(const zfr_ele_desc_t *d, int defEB, size_t fixed, size_t per_ele, bool one_wave = true) {
   int eb = d->eles_per_block > 0 ? d->eles_per_block : defEB;
   if (d->eles_per_block <= 0) {
     // small meshes (coarse multigrid levels, small cases): keep >= 1024 workgroups, four
@@ -1482,7 +1485,7 @@ inline int eles_per_block(const zfr_ele_desc_t *d, int defEB, size_t fixed, size
       // one-wave workgroups (barriers without cross-wave synchronisation) where they
       // still leave >= 1024 of them
       const int eb1 = (64 / d->nSpts) > 0 ? 64 / d->nSpts : 1;
-      if (eb1 < eb && ((long)d->nEles + eb1 - 1) / eb1 >= 1024) eb = eb1;
+      if (one_wave && eb1 < eb && ((long)d->nEles + eb1 - 1) / eb1 >= 1024) eb = eb1;
     }
   }
   while (eb > 1 && (eb * d->nSpts > kMaxThreads || sizeof(double) * (fixed + eb * per_ele) > kMaxLds)) eb--;

@@ -73,7 +73,7 @@ int launch(Op op, const zfr_ele_desc_t *d, const double *U, double *out, hipStre
     return zfr::fail("one-sided LDG kernels need ldg_b = +-0.5");
   // the viscous adjoint runs one element per workgroup: its LDS then allows ~7
   // workgroups per CU, more than any larger tile
-  const int EB = (op == Op::AdjMain && d->viscous) ? 1 : eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele);
+  const int EB = (op == Op::AdjMain && d->viscous) ? 1 : eles_per_block(d, S::defaultEB(), S::NOPS_PAD, per_ele, !g_persist);
   if (sizeof(double) * (S::NOPS_PAD + EB * per_ele) > max_lds<D, N>())
     return zfr::fail("kernel launch: one element needs more LDS than a workgroup may use (order too high)");
   KArgs a = make_args(d, EB);
