@@ -433,7 +433,7 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
   if (!rc && op == zk::Op::Div && d->viscous && d->ldg_b != 0.5 && d->ldg_b != -0.5) rc = need(d->Gtr, "Gtr");
   if (!rc && op == zk::Op::FaceCorr) rc = need(d->Ftr, "Ftr");
   if (rc) return rc;
-  if (update && (r->nStages < 1 || r->nStages > 4 || r->stage < 0 || r->stage >= r->nStages))
+  if (update && (r->nStages < 1 || r->nStages > zk::kMaxRkStages || r->stage < 0 || r->stage >= r->nStages))
     return zfr::fail("RK fusion: stage out of range");
   if (r->trace_out && reads_tr && r->trace_out == d->Utr)
     return zfr::fail("RK fusion: trace_out must not be the trace the divergence kernel reads (d->Utr)");
@@ -482,7 +482,7 @@ int zfr_rk_stage(const zfr_ele_desc_t *d, double *U, const double *U_ini, const 
 int zfr_rk_final_source(const zfr_ele_desc_t *d, double *U, const double *U_ini, const double *divF,
                         const double *source, const double *beta, int nStages, double dt, void *stream) {
   if (int rc = zk::refuse_while_recording("zfr_rk_final")) return rc;
-  if (nStages < 1 || nStages > 4) return zfr::fail("zfr_rk_final: nStages must be 1..4");
+  if (nStages < 1 || nStages > zk::kMaxRkStages) return zfr::fail("zfr_rk_final: nStages must be 1..4");
   const long total = (long)d->nEles * d->nVars * d->nSpts;
   if (total <= 0) return 0;
   double b[4] = {0, 0, 0, 0};

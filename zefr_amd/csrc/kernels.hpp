@@ -47,6 +47,7 @@
 namespace zk {
 
 constexpr int kMaxThreads = 256;
+constexpr int kMaxRkStages = 4;  // RK44; the stage tables of the RK update hold this many
 
 template <int D, int N>
 struct Shape {
@@ -103,7 +104,7 @@ struct KArgs {
     const double *Uini, *divF0;  // divF0: stage 0 of the [nStages][E][V][nS] residuals
     const double *src;           // p-multigrid source term added to every divF_s (modes 1, 2, 4), or null
     long total;
-    double alpha, dt, beta[4];
+    double alpha, dt, beta[kMaxRkStages];
     int nStages, stage, mode;
     double *trace;  // mode 1: also write the new state's trace here (the next stage's Utr), or null
     double *Uini_out;         // modes 1, 2: a copy of Uini (read before U is written), or null
@@ -417,7 +418,12 @@ __device__ __forceinline__ void rk_prefetch(const KArgs &a, long e, int t, int n
   }
 }
 
-template <int D>
+// BATCH: the final combination loads every other stage's residual before it consumes
+// the first (one memory round trip instead of one per stage and variable; the short
+// launches of the coarse p-multigrid levels are latency-bound); it holds (nStages - 1) V
+// more values, which the kernels that carry it without a prefetched state (face
+// correction, viscous divergence) would pay in occupancy, so they keep the plain loop.
+template <int D, bool BATCH = false>
 __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int nS, const double *acc,
                                             double *unew = nullptr, const RkPre<D> *pre = nullptr) {
   constexpr int V = D + 2, nM = D * D + 1;
@@ -428,6 +434,51 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
     return;
   }
   const double detJ = pre ? pre->detJ : a.metric[(e * nM + D * D) * nS + t];
+  if (BATCH && a.rk.mode == 2) {
+    const long i0 = e * V * nS + t;
+    double u[V], sv[V], rs[kMaxRkStages][V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      u[v] = pre ? pre->u0[v] : a.rk.Uini[i0 + v * nS];
+      sv[v] = a.rk.src ? (pre ? pre->src[v] : a.rk.src[i0 + v * nS]) : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxRkStages; s++) {
+      if (s < a.rk.nStages && s != a.rk.stage) {
+#pragma unroll
+        for (int v = 0; v < V; v++) rs[s][v] = a.rk.divF0[s * a.rk.total + i0 + v * nS];
+      }
+    }
+    double cb[V];
+    if (a.rk.corr) {
+#pragma unroll
+      for (int v = 0; v < V; v++) cb[v] = a.rk.corr_base[i0 + v * nS];
+    }
+    if (a.rk.Uini_out) {
+#pragma unroll
+      for (int v = 0; v < V; v++) a.rk.Uini_out[i0 + v * nS] = u[v];
+    }
+    // each variable's sum over the stages in order, as k_rk_final's
+#pragma unroll
+    for (int s = 0; s < kMaxRkStages; s++) {
+      if (s < a.rk.nStages) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          double r = (s == a.rk.stage) ? acc[v] : rs[s][v];
+          if (a.rk.src) r += sv[v];
+          u[v] -= a.rk.beta[s] * a.rk.dt / detJ * r;
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      const long i = i0 + v * nS;
+      a.rk.U[i] = u[v];
+      if (unew) unew[v] = u[v];
+      if (a.rk.corr) a.rk.corr[i] = u[v] - cb[v];
+    }
+    return;
+  }
 #pragma unroll
   for (int v = 0; v < V; v++) {
     const long i = (e * V + v) * nS + t;
@@ -533,17 +584,35 @@ __device__ __forceinline__ void pmg_epilogue(const KArgs &a, double *lops, const
   using S = Shape<D, N>;
   constexpr int nS = S::nS, V = S::V;
   const KArgs::Rk &r = a.rk;
-  // each (variable, output point) pair of the element is one work item
+  // each (variable, output point) pair of the element is one work item; a thread takes K
+  // of them at a time and, when it adds to the output, loads all K old values before it
+  // forms the first (one memory round trip per K items, not one per item)
   auto apply = [&](const double *blk, int no, double *out, double alpha, bool add) {
     int noD = no * no;
     if (D == 3) noD *= no;
     if (!valid) return;
+    const int total = V * noD;
+    double *oe = out + e * total;  // [V][no^D] of this element
+    constexpr int K = 4;
 #pragma unroll 1
-    for (int k = t; k < V * noD; k += nS) {
-      const int v = k / noD, o = k - v * noD;
-      const double x = tp_out<D, N>(blk + v * nS, lops, no, o);
-      double *g = out + (e * V + v) * noD + o;
-      *g = add ? __builtin_fma(alpha, x, 1.0 * *g) : x;
+    for (int k0 = t; k0 < total; k0 += K * nS) {
+      double old[K];
+      if (add) {
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+          const int k = k0 + j * nS;
+          old[j] = (k < total) ? oe[k] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        const int k = k0 + j * nS;
+        if (k < total) {
+          const int v = k / noD, o = k - v * noD;
+          const double x = tp_out<D, N>(blk + v * nS, lops, no, o);
+          oe[k] = add ? __builtin_fma(alpha, x, 1.0 * old[j]) : x;
+        }
+      }
     }
   };
   const bool restr = r.pmgRc && (r.mode == 0 || r.mode == 4);
@@ -1048,7 +1117,7 @@ __device__ __forceinline__ void divergence_block(const double *U, double *__rest
     for (int v = 0; v < V; v++) divF[(e * V + v) * nS + t] = acc[v];
 #pragma unroll
     for (int v = 0; v < V; v++) res[v] = acc[v];
-    rk_epilogue<D>(a, e, t, nS, acc, unew, VISC ? nullptr : &pre);
+    rk_epilogue<D, !VISC>(a, e, t, nS, acc, unew, VISC ? nullptr : &pre);
   }
   // the new state's full trace for the next stage, into a buffer other than the Utr this
   // launch reads; the element's U block is no longer read
