@@ -543,7 +543,8 @@ __device__ __forceinline__ void rk_trace(const KArgs &a, const double *lops, dou
 // in LDS.
 template <int D, int N>
 __device__ __forceinline__ double tp_out(const double *blk, const double *A, int no, int o) {
-  const int o1 = o % no, o2 = (o / no) % no, o3 = (D == 3) ? o / (no * no) : 0;
+  const unsigned uo = o, uno = no;  // unsigned: cheaper division sequences
+  const int o1 = uo % uno, o2 = (uo / uno) % uno, o3 = (D == 3) ? uo / (uno * uno) : 0;
   const double *A1 = A + o1 * N, *A2 = A + o2 * N, *A3 = A + o3 * N;
   double a1[N];
 #pragma unroll
@@ -572,6 +573,41 @@ __device__ __forceinline__ double tp_out(const double *blk, const double *A, int
   return s3;
 }
 
+// One element's transfer out [V][no^D] = A blk (add: out += alpha A blk) by the element's nS
+// threads; each (variable, output point) pair is one work item.  A thread takes K items at
+// a time and, when it adds to the output, loads all K old values before it forms the first
+// (one memory round trip per K items, not one per item).  NO > 0: no == NO, a constant.
+template <int D, int N, int NO>
+__device__ __forceinline__ void pmg_apply(const double *blk, const double *A, int no_rt, double *out, long e, int t,
+                                          double alpha, bool add) {
+  using S = Shape<D, N>;
+  constexpr int nS = S::nS, V = S::V, K = 4;
+  const int no = NO > 0 ? NO : no_rt;
+  const int noD = (D == 3) ? no * no * no : no * no;
+  const int total = V * noD;
+  double *oe = out + e * total;  // [V][no^D] of this element
+#pragma unroll 1
+  for (int k0 = t; k0 < total; k0 += K * nS) {
+    double old[K];
+    if (add) {
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        const int k = k0 + j * nS;
+        old[j] = (k < total) ? oe[k] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const int k = k0 + j * nS;
+      if (k < total) {
+        const int v = (unsigned)k / (unsigned)noD, o = k - v * noD;
+        const double x = tp_out<D, N>(blk + v * nS, A, no, o);
+        oe[k] = add ? __builtin_fma(alpha, x, 1.0 * old[j]) : x;
+      }
+    }
+  }
+}
+
 // The p-multigrid transfers fused at the end of a residual kernel (every thread of the
 // workgroup calls this; it holds barriers).  res: this thread's residual (after the source
 // term); unew: its new state (mode 2); ublk: the element's U block in LDS, or null (then U
@@ -584,36 +620,13 @@ __device__ __forceinline__ void pmg_epilogue(const KArgs &a, double *lops, const
   using S = Shape<D, N>;
   constexpr int nS = S::nS, V = S::V;
   const KArgs::Rk &r = a.rk;
-  // each (variable, output point) pair of the element is one work item; a thread takes K
-  // of them at a time and, when it adds to the output, loads all K old values before it
-  // forms the first (one memory round trip per K items, not one per item)
   auto apply = [&](const double *blk, int no, double *out, double alpha, bool add) {
-    int noD = no * no;
-    if (D == 3) noD *= no;
     if (!valid) return;
-    const int total = V * noD;
-    double *oe = out + e * total;  // [V][no^D] of this element
-    constexpr int K = 4;
-#pragma unroll 1
-    for (int k0 = t; k0 < total; k0 += K * nS) {
-      double old[K];
-      if (add) {
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-          const int k = k0 + j * nS;
-          old[j] = (k < total) ? oe[k] : 0.0;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < K; j++) {
-        const int k = k0 + j * nS;
-        if (k < total) {
-          const int v = k / noD, o = k - v * noD;
-          const double x = tp_out<D, N>(blk + v * nS, lops, no, o);
-          oe[k] = add ? __builtin_fma(alpha, x, 1.0 * old[j]) : x;
-        }
-      }
-    }
+    // the usual transfers (one order down, one up) with their output count known at compile
+    // time: the work item's index splits by multiplications instead of integer divisions
+    if (no == N - 1) pmg_apply<D, N, (N > 1 ? N - 1 : 0)>(blk, lops, no, out, e, t, alpha, add);
+    else if (no == N + 1) pmg_apply<D, N, N + 1>(blk, lops, no, out, e, t, alpha, add);
+    else pmg_apply<D, N, 0>(blk, lops, no, out, e, t, alpha, add);
   };
   const bool restr = r.pmgRc && (r.mode == 0 || r.mode == 4);
   const bool prol = r.pmgUf && r.mode == 2 && r.corr;
