@@ -375,7 +375,10 @@ typedef struct zfr_rk_args {
   /* Residual epilogues without an update, for p-multigrid (PMGrid::compute_source_term
    * and the restricted residual, src/multigrid.cpp:105-261, 474-518): mode 3 writes
    * source_out -= divF (source_out holding the restricted residual); mode 4 writes
-   * divF += source.  U and U_ini are not read or written in these modes. */
+   * divF += source.  U and U_ini are not read or written in these modes.  Mode 1 with
+   * source_out (and source): the stage first forms the source term, source_out = source -
+   * divF, then updates with it (stage 0 of a coarse level's first smoothing step, whose
+   * residual is the one compute_source_term forms; source_out may be source). */
   double *source_out;
   /* p-multigrid transfers fused into the same kernel (PMGrid::restrict_pmg / prolong_err,
    * src/multigrid.cpp:335-433), with zfr_pmg_transfer's arithmetic.  Restriction (modes 0

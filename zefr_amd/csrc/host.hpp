@@ -210,8 +210,13 @@ class FRSolver {
     double *out = nullptr;
   };
   // With fuse_rk: the step's final combination also prolongs the correction (corr and
-  // corr_base needed); the caller prolongs it otherwise.
-  void update(const double *source, double *save, double *corr, const double *corr_base, const Prolong &pro);
+  // corr_base needed); the caller prolongs it otherwise.  form_source (fuse_rk, several
+  // stages): source holds the restricted residual, and stage 0 first turns it into the
+  // source term (PMGrid::compute_source_term: source -= the stage-0 residual, which is
+  // the residual at the state the step starts from) before it uses it, so the source
+  // term needs no residual launch of its own.
+  void update(const double *source, double *save, double *corr, const double *corr_base, const Prolong &pro,
+              bool form_source = false);
   // With fuse_rk: the stage-0 residual (plus source when not null) and, in the same last
   // kernel, U and that residual restricted with R [n_out][order+1] per direction into Uc
   // and Rc (restrict_pmg, src/multigrid.cpp:335-392).
@@ -245,7 +250,7 @@ class FRSolver {
   std::array<int, 4> graph_tile = {-1, -1, -1, -1};
   bool graph_fused = false;
   const double *graph_source = nullptr;
-  std::array<const void *, 5> graph_extra = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::array<const void *, 6> graph_extra = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double graph_pro_alpha = 0.0;
   int graph_pro_n = 0;
   const double *rk_source = nullptr;  // source term of the step in progress
@@ -253,6 +258,7 @@ class FRSolver {
   double *rk_corr = nullptr;          // after the step: rk_corr = U - rk_corr_base
   const double *rk_corr_base = nullptr;
   Prolong rk_prolong;                  // with rk_corr: prolong the correction (fused steps)
+  double *rk_form_source = nullptr;   // stage 0 forms the source term here (update's form_source)
 };
 
 // p-multigrid (include/multigrid.hpp:17-44, src/multigrid.cpp:18-518) on quads/hexes:

@@ -389,6 +389,9 @@ int rk_fused(zk::Op op, const zfr_ele_desc_t *d, const double *U, double *divF, 
   if (update && (!r->U || !r->U_ini || (r->mode == 2 && r->nStages > 1 && !r->divF0)))
     return zfr::fail("RK fusion: missing buffer");
   if (r->mode == 3 && !r->source_out) return zfr::fail("RK fusion: mode 3 needs source_out");
+  if (r->source_out && r->mode != 3 && !(r->mode == 1 && r->source))
+    return zfr::fail("RK fusion: source_out is for mode 3, or for mode 1 with source (the stage forms the source "
+                     "term first)");
   if (r->mode == 4 && !r->source) return zfr::fail("RK fusion: mode 4 needs source");
   if (r->U_ini_out && !update) return zfr::fail("RK fusion: U_ini_out is for RK updates (modes 1, 2)");
   if (r->U_ini_out && r->U_ini_out == r->U_ini) return zfr::fail("RK fusion: U_ini_out must not be U_ini");

@@ -483,8 +483,12 @@ __device__ __forceinline__ void rk_epilogue(const KArgs &a, long e, int t, int n
   for (int v = 0; v < V; v++) {
     const long i = (e * V + v) * nS + t;
     const double u0 = pre ? pre->u0[v] : a.rk.Uini[i];
-    const double sv = a.rk.src ? (pre ? pre->src[v] : a.rk.src[i]) : 0.0;
+    double sv = a.rk.src ? (pre ? pre->src[v] : a.rk.src[i]) : 0.0;
     if (a.rk.Uini_out) a.rk.Uini_out[i] = u0;
+    if (a.rk.mode == 1 && a.rk.srcw) {  // form the source term first: srcw = src - divF
+      sv = sv - acc[v];
+      a.rk.srcw[i] = sv;
+    }
     if (a.rk.mode == 1) {
       const double r = a.rk.src ? acc[v] + sv : acc[v];
       const double u = u0 - a.rk.alpha * a.rk.dt / detJ * r;

@@ -202,14 +202,15 @@ void PMGrid::v_cycle(FRSolver &solver, int lvl) {
   };
   // steps smoothing steps of level n under its source term: the first stores the solution
   // (store), the last writes the correction and, fused, prolongs it (finish)
-  auto smooth = [&](int n, unsigned steps, bool store, bool finish) {
+  // (form: the first step's stage 0 forms the source term from the restricted residual)
+  auto smooth = [&](int n, unsigned steps, bool store, bool finish, bool form) {
     FRSolver &g = *grids[n];
     double *src = sourcesBT[n].as<double>(), *sol = solutionsBT[n].as<double>(),
            *corr = correctionsBT[n].as<double>();
     for (unsigned s = 0; s < steps; s++) {
       const bool last = finish && s + 1 == steps;
       g.update(src, (store && s == 0) ? sol : nullptr, last ? corr : nullptr, last ? sol : nullptr,
-               (last && fused_prolong(n)) ? prolong(n) : FRSolver::Prolong{});
+               (last && fused_prolong(n)) ? prolong(n) : FRSolver::Prolong{}, form && s == 0);
     }
   };
   // the correction of a level without smoothing steps, and the unfused prolongation
@@ -226,20 +227,24 @@ void PMGrid::v_cycle(FRSolver &solver, int lvl) {
   for (int n = lvl + 1; n < nLevels; n++) {
     FRSolver &g = *grids[n];
     Elements &e = *g.eles;
-    compute_source_term(g, sourcesBT[n].as<double>());
     const unsigned steps = input->mg_steps[n];
     const bool coarsest = n == nLevels - 1;
+    // the source term is the residual at the restricted state subtracted from the
+    // restricted residual; a fused first step forms it in its stage 0, whose residual is
+    // that residual
+    const bool form = g.fuse_rk && steps > 0 && g.input->nStages > 1;
+    if (!form) compute_source_term(g, sourcesBT[n].as<double>());
     if (steps == 0)
       hip_or_throw(hipMemcpyAsync(solutionsBT[n].p, e.U_spts_d.p, e.state_size() * sizeof(double),
                                   hipMemcpyDeviceToDevice, g.stream), "hipMemcpyAsync");
     // the coarsest level is not smoothed again on the way up: its last step here ends it
-    smooth(n, steps, true, coarsest);
+    smooth(n, steps, true, coarsest, form);
     if (coarsest) finish_level(n, steps > 0);
     else restrict_from(n, sourcesBT[n].as<double>());
   }
   for (int n = nLevels - 2; n > lvl; n--) {
     const unsigned steps = input->mg_steps[n];
-    smooth(n, steps, false, true);
+    smooth(n, steps, false, true, false);
     finish_level(n, steps > 0);
   }
 }

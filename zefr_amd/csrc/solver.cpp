@@ -490,6 +490,7 @@ void FRSolver::finish_with_rk(int stage, double *trace_out, bool split) {
   r.dt = input->dt;
   r.source = rk_source;
   if (split && stage == 0) r.U_ini_out = rk_save;
+  if (stage == 0 && rk_form_source) r.source_out = rk_form_source;  // source -= divF(0) first
   if (stage < ns - 1) {
     r.mode = 1;
     r.alpha = kRkAlpha[stage];
@@ -592,7 +593,8 @@ void FRSolver::step_RK() {
     step_RK_eager();
     return;
   }
-  const std::array<const void *, 5> extra = {rk_save, rk_corr, rk_corr_base, rk_prolong.A, rk_prolong.out};
+  const std::array<const void *, 6> extra = {rk_save, rk_corr, rk_corr_base, rk_prolong.A, rk_prolong.out,
+                                             rk_form_source};
   if (!graph_exec || graph_dt != input->dt || graph_tile != eles->tile_key() || graph_fused != fuse_rk ||
       graph_source != rk_source || graph_extra != extra || graph_pro_alpha != rk_prolong.alpha ||
       graph_pro_n != rk_prolong.n) {
@@ -635,21 +637,26 @@ void FRSolver::update(const double *source, double *save, double *corr, const do
 }
 
 void FRSolver::update(const double *source, double *save, double *corr, const double *corr_base,
-                      const Prolong &pro) {
+                      const Prolong &pro, bool form_source) {
   if (corr && !corr_base) raise("update: corr needs corr_base");
   if (pro.out && (!fuse_rk || !corr)) raise("update: the prolongation is fused into a fused step with corr");
+  if (form_source && (!fuse_rk || input->nStages < 2 || !source))
+    raise("update: the source term is formed by stage 0 of a fused step of several stages, from a source buffer");
   prev_time = flow_time;
   rk_source = source;
   rk_save = save;
   rk_corr = corr;
   rk_corr_base = corr_base;
   rk_prolong = pro;
+  // the buffer is the caller's own source array (PMGrid's sourcesBT), written in place
+  rk_form_source = form_source ? const_cast<double *>(source) : nullptr;
   auto clear = [&] {
     rk_source = nullptr;
     rk_save = nullptr;
     rk_corr = nullptr;
     rk_corr_base = nullptr;
     rk_prolong = Prolong{};
+    rk_form_source = nullptr;
   };
   try {
     step_RK();
