@@ -177,3 +177,48 @@ def test_persistent_plan_recording_rules():
     finally:
         assert L.zfr_persist_record_end(None) == 0  # discarded
     assert L.zfr_persist_record_end(None) != 0  # no longer recording
+
+
+class _RkArgs(C.Structure):
+    """zfr_rk_args_t of include/zefr_amd.h (layout checked against a C compile below)."""
+    _P = C.c_void_p
+    _fields_ = [("U", _P), ("U_ini", _P), ("divF0", _P), ("mode", C.c_int), ("stage", C.c_int),
+                ("nStages", C.c_int), ("alpha", C.c_double), ("dt", C.c_double), ("beta", C.c_double * 4),
+                ("source", _P), ("trace_out", _P), ("U_ini_out", _P), ("corr_out", _P), ("corr_base", _P),
+                ("source_out", _P), ("restrict_A", _P), ("restrict_src", _P), ("restrict_U", _P),
+                ("restrict_divF", _P), ("restrict_n", C.c_int), ("prolong_n", C.c_int), ("prolong_A", _P),
+                ("prolong_alpha", C.c_double), ("prolong_out", _P)]
+
+
+def test_rk_source_out_rules():
+    """source_out goes with mode 3 (source_out -= divF), or with mode 1 and a source (the
+    stage forms the p-multigrid source term, then updates with it); anything else is an
+    error before any HIP call."""
+    code = ('#include "zefr_amd.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+            'int main(void){printf("%zu %zu %zu\\n", sizeof(zfr_rk_args_t), offsetof(zfr_rk_args_t, source_out),'
+            ' offsetof(zfr_rk_args_t, prolong_out)); return 0;}\n')
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="zefr_rk_")
+    c, exe = os.path.join(tmp, "rk.c"), os.path.join(tmp, "rk")
+    with open(c, "w") as f:
+        f.write(code)
+    r = subprocess.run(["gcc", "-std=c11", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+                        "-I", "/opt/rocm/include", c, "-o", exe], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("the header needs hip_runtime_api.h from /opt/rocm/include: " + r.stderr[-200:])
+    got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [C.sizeof(_RkArgs), _RkArgs.source_out.offset, _RkArgs.prolong_out.offset]
+    L = _lib.lib()
+    buf = C.c_void_p(0x1000)  # never dereferenced: every call below fails its checks first
+    d = _lib.EleDesc(nDims=2, order=3, nVars=4, nEles=8, nSpts=16, nFpts=16, viscous=0, gamma=1.4,
+                     ops1d=buf, metric=buf, fgeo=buf, nbr=buf, Utr=buf)
+    for mode, source, msg in [(1, None, "source_out is for mode 3"), (2, buf, "source_out is for mode 3"),
+                              (4, buf, "source_out is for mode 3")]:
+        a = _RkArgs(U=buf, U_ini=buf, divF0=buf, mode=mode, stage=0, nStages=4, source=source, source_out=buf)
+        assert L.zfr_divergence_rk(C.byref(d), buf, buf, C.byref(a), None) != 0
+        assert msg in _lib.last_error(), (mode, _lib.last_error())
+    # mode 1 with a source passes the rule and fails a later check (the stage index), still
+    # before any launch
+    a = _RkArgs(U=buf, U_ini=buf, divF0=buf, mode=1, stage=7, nStages=4, source=buf, source_out=buf)
+    assert L.zfr_divergence_rk(C.byref(d), buf, buf, C.byref(a), None) != 0
+    assert "stage out of range" in _lib.last_error(), _lib.last_error()
