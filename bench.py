@@ -544,7 +544,7 @@ def flux_leg(workdir, mesh, order, steps, flux):
 def vjp_timing(z, reps):
     """The residual's vector-Jacobian product on the solver's mesh and state against the
     forward residual (residual_at), both on the torch stream, each call bracketed by HIP
-    events: medians over `reps` calls (BASELINE's VJP ratio is median over median)."""
+    events: medians over `reps` calls (the VJP ratio is a median over a median)."""
     import statistics
     import torch
     U = z.state_tensor()
