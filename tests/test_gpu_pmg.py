@@ -117,6 +117,14 @@ FUSE_CASES = [
               rel_fac=0.9),
 ]
 FUSE_CASES[2]["extra"]["ldg_b"] = 0.3  # the general-beta LDG path
+# a coarse level without smoothing steps, and forward-Euler steps: their source terms are
+# formed by a residual launch of their own, not by a first step's stage 0
+FUSE_CASES += [
+    _pmg_case("pmg_fuse_zero_steps", (5, 4), (0.0, 0.0), (1.0, 1.0), 3, False, "noisy:5", [3, 2, 0], [1, 0, 2],
+              dt=2e-4),
+    dict(_pmg_case("pmg_fuse_forward_euler", (4, 5), (0.0, 0.0), (1.0, 1.0), 3, False, "noisy:5", [3, 1, 0],
+                   [1, 2, 1], dt=5e-5), dt_scheme="Euler"),
+]
 
 
 @pytest.mark.parametrize("case", FUSE_CASES, ids=[c["name"] for c in FUSE_CASES])
